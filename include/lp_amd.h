@@ -1,0 +1,221 @@
+/*
+ * lp_amd.h -- C ABI of the MI355X (gfx950) implementation of the
+ * lidar_processing_lib per-frame hot path.
+ *
+ * The ABI is what the reference's C++ API (namespace lidar_processing_lib)
+ * binds to once its bodies are replaced by HIP kernels.  Plain pointers and
+ * sizes only; no exceptions cross it.  Paths are relative to the reference
+ * repository root.
+ *
+ *   lp_segment        replaces Segmenter::segment
+ *                     (lidar_processing_lib/include/lidar_processing_lib/segmenter.hpp:168-169,
+ *                      lidar_processing_lib/src/segmenter.cpp:87-101)
+ *   lp_cluster        replaces Clusterer::cluster
+ *                     (lidar_processing_lib/include/lidar_processing_lib/clusterer.hpp:92-93,
+ *                      lidar_processing_lib/src/clusterer.cpp:43-53)
+ *   lp_convex_hulls   replaces a loop of Polygonizer::convexHull calls
+ *                     (lidar_processing_lib/include/lidar_processing_lib/polygonizer.hpp:104-106,
+ *                      lidar_processing_lib/src/polygonizer.cpp:33-91)
+ *   lp_process_frames replaces the per-frame body of Processor::run from the
+ *                     segment call to the last hull (src/processor/src/processor.cpp:552-670)
+ *
+ * Conventions (mirroring the reference):
+ *   - the caller owns inputs and outputs; every output entry is written
+ *     (like labels.assign, segmenter.cpp:90 / clusterer.cpp:124);
+ *   - one context = one HIP stream and its preallocated device buffers, used
+ *     from one thread at a time (the reference objects are not thread safe);
+ *     separate contexts may run concurrently;
+ *   - errors are negative status codes; lp_last_error() gives the message of
+ *     the calling thread's last failure.  The reference's exceptions map to:
+ *     std::overflow_error (static_unordered_map.hpp:79-81) -> LP_ERR_VOXEL_OVERFLOW.
+ */
+#ifndef LP_AMD_H
+#define LP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LP_OK = 0,
+    LP_ERR_INVALID_ARGUMENT = -1,
+    LP_ERR_HIP = -2,
+    LP_ERR_CAPACITY = -3,             /* a context capacity (points, image, cell, hull) was exceeded */
+    LP_ERR_VOXEL_OVERFLOW = -4,       /* more than max_voxels occupied voxels (reference: std::overflow_error) */
+    LP_ERR_RANSAC_SINGLE_CANDIDATE = -5 /* exactly one RANSAC candidate: the reference's redraw loop
+                                           (segmenter.cpp:383-386) never terminates */
+};
+
+/* Label values (segmenter.hpp:69-74). */
+enum { LP_LABEL_UNKNOWN = 0, LP_LABEL_GROUND = 1, LP_LABEL_OBSTACLE = 2 };
+
+/* SegmenterConfiguration (segmenter.hpp:87-112): same fields, same defaults.
+ * display_recm_with_low_confidence_points (an OpenCV window) is not carried. */
+typedef struct lp_segmenter_config {
+    float elevation_up_deg;
+    float elevation_down_deg;
+    int32_t image_width;
+    int32_t image_height;
+    int32_t assume_unorganized_cloud;
+    float grid_radial_spacing_m;
+    float grid_slice_resolution_deg;
+    float ground_height_threshold_m;
+    float road_maximum_slope_m_per_m;
+    float min_distance_m;
+    float max_distance_m;
+    float sensor_height_m;
+    float kernel_threshold_distance_m;
+    float amplification_factor;
+    float z_min_m;
+    float z_max_m;
+} lp_segmenter_config;
+
+/* ClustererConfiguration (clusterer.hpp:61-68). */
+typedef struct lp_clusterer_config {
+    float voxel_grid_range_resolution_m;
+    float voxel_grid_azimuth_resolution_deg;
+    float voxel_grid_elevation_resolution_deg;
+    uint32_t min_cluster_size;
+} lp_clusterer_config;
+
+/* Defaults of segmenter.hpp:91-108. */
+void lp_segmenter_config_default(lp_segmenter_config* cfg);
+/* Header defaults of clusterer.hpp:61-68 (elevation resolution 1.5 deg). */
+void lp_clusterer_config_default(lp_clusterer_config* cfg);
+/* Shipped processor.param.yaml:31-35 values (elevation resolution 3.0 deg). */
+void lp_clusterer_config_shipped(lp_clusterer_config* cfg);
+
+/* Device buffers are preallocated for these limits at context creation
+ * (the reference reserves fixed capacities in its constructors:
+ * segmenter.cpp:38-71, clusterer.cpp:27-41). */
+typedef struct lp_capacity {
+    int32_t max_frames;            /* frames per batch */
+    int32_t max_points_per_frame;
+    int32_t max_image_height;
+    int32_t max_image_width;
+    int32_t max_voxels;            /* 200000 in the reference (clusterer.cpp:36) */
+    int32_t device;                /* HIP device ordinal */
+} lp_capacity;
+void lp_capacity_default(lp_capacity* cap);
+
+typedef struct lp_context lp_context;
+int lp_context_create(const lp_capacity* cap, lp_context** out);
+int lp_context_destroy(lp_context* ctx);
+const char* lp_last_error(void);
+
+/* A point cloud: x, y, z (float) and optional ring (uint16) addressed with
+ * byte strides, so both AoS pcl::PointXYZIR (stride sizeof(point)) and SoA
+ * arrays (stride 4 / 2) are accepted without a host-side copy.  ring == NULL
+ * selects the ring-less instantiations (PointXYZ / PointXYZI), whose image
+ * row comes from the elevation angle (segmenter.cpp:180-193). */
+typedef struct lp_cloud {
+    const void* x;
+    const void* y;
+    const void* z;
+    const void* ring;
+    int32_t xyz_stride;
+    int32_t ring_stride;
+    int32_t num_points;
+    int32_t on_device;      /* 1: the pointers are device pointers */
+} lp_cloud;
+
+/* The RANSAC plane the reference keeps in locals (segmenter.cpp:363-374). */
+typedef struct lp_ransac_result {
+    uint32_t candidates;
+    uint32_t best_inliers;
+    int32_t applied;        /* best_inliers > 0 */
+    float plane[4];         /* a, b, c, d after the sign normalisation (c >= 0) */
+} lp_ransac_result;
+
+/* Segmenter::segment: labels[num_points] receives LP_LABEL_*; ransac may be NULL. */
+int lp_segment(lp_context* ctx, const lp_segmenter_config* cfg, const lp_cloud* cloud, uint32_t* labels,
+               lp_ransac_result* ransac);
+
+/* Clusterer::cluster: labels[num_points] receives the cluster id or -1. */
+int lp_cluster(lp_context* ctx, const lp_clusterer_config* cfg, const lp_cloud* cloud, int32_t* labels);
+
+/* Polygonizer::convexHull on num_lists point lists packed by offsets
+ * (list k is points offsets[k] .. offsets[k+1]-1).  Output: hull_offsets
+ * [num_lists+1] and hull_indices (list-local indices, CCW from the
+ * lexicographically smallest point), capacity hull_capacity entries. */
+int lp_convex_hulls(lp_context* ctx, const double* x, const double* y, const int32_t* offsets, int32_t num_lists,
+                    int32_t* hull_offsets, int32_t* hull_indices, int32_t hull_capacity);
+
+/* Fused per-frame path of the processor: segment -> obstacle split (input
+ * order, processor.cpp:562-579) -> cluster -> per-cluster gather -> hull. */
+typedef struct lp_frame_outputs {
+    uint32_t* seg_labels;       /* [num_points] */
+    int32_t* obstacle_index;    /* [num_points]: input index of each obstacle point (first num_obstacles used) */
+    int32_t* cluster_labels;    /* [num_points]: cluster id per obstacle point (first num_obstacles used) */
+    int32_t* hull_offsets;      /* [num_points + 1] (first num_clusters+1 used) */
+    int32_t* hull_indices;      /* cluster-local hull vertex indices */
+    int32_t hull_capacity;
+} lp_frame_outputs;
+
+typedef struct lp_frame_result {
+    int32_t status;             /* LP_OK or an error for this frame only */
+    int32_t num_points;
+    int32_t num_obstacles;
+    int32_t num_voxels;
+    int32_t num_clusters;
+    int32_t num_hull_vertices;
+    int32_t jcp_queue;          /* pixels resolved by JCP */
+    lp_ransac_result ransac;
+} lp_frame_result;
+
+/* Processes num_frames frames as one batch (CSR-packed on the device, one
+ * launch sequence).  cfgs may hold one config (shared) or num_frames. */
+int lp_process_frames(lp_context* ctx, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
+                      const lp_clusterer_config* cluster_cfg, int32_t num_frames, const lp_cloud* clouds,
+                      const lp_frame_outputs* outputs, lp_frame_result* results);
+
+/* Split form of lp_process_frames for device-resident pipelines: stage
+ * inputs, run (asynchronously on the context stream, captured once per
+ * batch shape as a hipGraph and replayed), then fetch results. */
+int lp_batch_stage(lp_context* ctx, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
+                   const lp_clusterer_config* cluster_cfg, int32_t num_frames, const lp_cloud* clouds);
+int lp_batch_run(lp_context* ctx, int32_t use_graph);
+int lp_batch_fetch(lp_context* ctx, const lp_frame_outputs* outputs, lp_frame_result* results);
+int lp_synchronize(lp_context* ctx);
+
+/* Timing of the dominant kernel for roofline reporting: average duration (ms)
+ * of the last `runs` launches of the named stage, from HIP events recorded on
+ * the context stream. */
+int lp_stage_names(lp_context* ctx, char* buf, int32_t buf_len);
+int lp_profile_run(lp_context* ctx, int32_t runs, float* stage_ms, int32_t max_stages, int32_t* num_stages);
+
+/* Per-context device stream (hipStream_t) for callers that interleave work. */
+void* lp_context_stream(lp_context* ctx);
+
+/* Intermediate state of frame `frame` of the last run, for stage-by-stage
+ * parity checks.  No reference counterpart: the reference keeps these in
+ * private members (segmenter.hpp:190-215).  Returns the element count
+ * written (or needed, when larger than `capacity`). */
+enum {
+    LP_DEBUG_ELEVATION_MAP = 0, /* float[slices*rings], RECM elevation_map_ */
+    LP_DEBUG_MAPPING = 1,       /* int32[H*W], cloud_mapping_indices_ (-1 = empty) */
+    LP_DEBUG_STATE_RECM = 2,    /* uint8[H*W]: 0 empty, 1 ground, 2 obstacle after RECM + RANSAC */
+    LP_DEBUG_JCP_QUEUE = 3,     /* int32[queue]: JCP queue pixels in processing order */
+    LP_DEBUG_STATE_FINAL = 4    /* uint8[H*W]: 0 empty, 1 ground, 2 obstacle, 3 unresolved */
+};
+int lp_debug_fetch(lp_context* ctx, int32_t frame, int32_t which, void* dst, int32_t capacity);
+
+/* The device restatements of the C library functions the reference calls
+ * (expf: segmenter.cpp:588; atanf / atan2f: clusterer.cpp:74,86,
+ * segmenter.cpp:157,182) and of atan2Approx (common.hpp:33-62), evaluated on
+ * the GPU (on_device = 1) or from the same source on the host (0).  No
+ * reference counterpart: it exists so tests can compare them with the host
+ * libm bit for bit.  b is the second argument of the two-argument forms. */
+enum { LP_MATH_EXPF = 0, LP_MATH_ATANF = 1, LP_MATH_ATAN2F = 2, LP_MATH_ATAN2_APPROX = 3 };
+int lp_math_eval(int32_t which, const float* a, const float* b, float* out, int32_t n, int32_t on_device);
+/* Device check of a contiguous range of float bit patterns [lo, lo+count)
+ * against host results `ref_bits` (one-argument functions only). */
+int lp_math_check_range(int32_t which, uint32_t lo, uint32_t count, const uint32_t* ref_bits, uint64_t* mismatches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,709 @@
+// lp_api.hip -- context, staging, launch sequence and the C ABI.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lp_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define LP_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(LP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr float kDegToRad = static_cast<float>(M_PI / 180.0);  // segmenter.hpp:135
+constexpr float kTwoPi = static_cast<float>(2.0 * M_PI);        // segmenter.hpp:136
+
+int ceil_log2(int64_t v) {
+    int b = 0;
+    while ((int64_t{1} << b) < v) ++b;
+    return b;
+}
+
+}  // namespace
+
+namespace lp {
+
+void stage_mark(lp_context* c, const char* name) {
+    if (!c->profiling) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->events.push_back(e);
+    c->stage_names.push_back(name);
+}
+
+// Derived segmenter constants, computed with host libm in float exactly as
+// Segmenter::config / constructPolarGrid / RECM / RANSAC / JCP do.
+static bool derive_seg(const lp_segmenter_config& g, const Layout& L, SegConst& s, std::string& why) {
+    if (g.image_width <= 0 || g.image_height <= 0 || g.image_width > L.W || g.image_height > L.H) {
+        why = "image size outside the context capacity";
+        return false;
+    }
+    s.H = g.image_height;
+    s.W = g.image_width;
+    s.slice_res = g.grid_slice_resolution_deg * kDegToRad;
+    s.rings = static_cast<int32_t>(g.max_distance_m / g.grid_radial_spacing_m);
+    s.slices = static_cast<int32_t>(kTwoPi / s.slice_res);
+    s.cells = s.rings * s.slices;
+    if (s.rings < kRansacRadialBins || s.slices < 1) {
+        why = "polar grid needs at least 4 radial rings and one slice";
+        return false;
+    }
+    if (s.cells > L.cell_stride || s.slices * kRansacRadialBins > L.cand_stride ||
+        s.slices >= (1 << (32 - L.idx_bits))) {
+        why = "polar grid larger than the context capacity";
+        return false;
+    }
+    s.organized = g.assume_unorganized_cloud ? 0 : 1;
+    s.zlo = -g.sensor_height_m + g.z_min_m;
+    s.zhi = -g.sensor_height_m + g.z_max_m;
+    s.spacing = g.grid_radial_spacing_m;
+    s.min_d = g.min_distance_m;
+    s.max_d = g.max_distance_m;
+    const float up = g.elevation_up_deg * kDegToRad;
+    s.down_rad = g.elevation_down_deg * kDegToRad;
+    s.per_pixel = (up - s.down_rad) / g.image_height;
+    s.w_minus_1 = static_cast<float>(g.image_width - 1);
+    s.thr = g.ground_height_threshold_m;
+    s.two_thr = 2.0F * g.ground_height_threshold_m;
+    s.dz = std::min(g.grid_radial_spacing_m * std::tan(g.road_maximum_slope_m_per_m),
+                    g.ground_height_threshold_m - FLT_EPSILON);
+    s.e0 = -g.sensor_height_m + g.ground_height_threshold_m;
+    s.neg_h = -g.sensor_height_m;
+    s.cos_max = std::cos(std::tan(g.road_maximum_slope_m_per_m));
+    s.kthr2 = g.kernel_threshold_distance_m * g.kernel_threshold_distance_m;
+    s.neg_amp = -g.amplification_factor;
+    return true;
+}
+
+static void derive_clu(const lp_clusterer_config& g, int32_t max_voxels, CluConst& c) {
+    c.res_r = g.voxel_grid_range_resolution_m;
+    c.res_az = g.voxel_grid_azimuth_resolution_deg * kDegToRad;
+    c.res_el = g.voxel_grid_elevation_resolution_deg * kDegToRad;
+    c.min_size = g.min_cluster_size;
+    c.max_voxels = max_voxels;
+}
+
+// De-interleave a raw AoS record buffer into the SoA inputs.
+__global__ void k_deinterleave(const uint8_t* raw, int n, int stride, int ox, int oy, int oz, int oring,
+                               float* x, float* y, float* z, uint16_t* ring) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t* r = raw + static_cast<size_t>(i) * stride;
+    float v;
+    memcpy(&v, r + ox, 4);
+    x[i] = v;
+    memcpy(&v, r + oy, 4);
+    y[i] = v;
+    memcpy(&v, r + oz, 4);
+    z[i] = v;
+    if (oring >= 0) {
+        uint16_t q;
+        memcpy(&q, r + oring, 2);
+        ring[i] = q;
+    }
+}
+
+}  // namespace lp
+
+using namespace lp;
+
+namespace {
+
+template <typename T>
+T* carve(char*& p, size_t count) {
+    T* r = reinterpret_cast<T*>(p);
+    size_t bytes = (count * sizeof(T) + 255) & ~size_t{255};
+    p += bytes;
+    return r;
+}
+
+size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
+    char* p = base;
+    const size_t B = L.B, N = L.N, P = L.P, C = L.cell_stride, HC = L.hash_cap;
+    auto go = [&](auto*& dst, size_t count) {
+        using T = std::remove_reference_t<decltype(*dst)>;
+        T* r = carve<T>(p, count);
+        if (d) dst = r;
+    };
+    DevBuf tmp{};
+    DevBuf& x = d ? *d : tmp;
+    go(x.x, B * N);
+    go(x.y, B * N);
+    go(x.z, B * N);
+    go(x.ring, B * N);
+    go(x.pt_cell, B * N);
+    go(x.pt_pix, B * N);
+    go(x.labels, B * N);
+    go(x.obs_flag, B * N);
+    go(x.obs_src, B * N);
+    go(x.ox, B * N);
+    go(x.oy, B * N);
+    go(x.oz, B * N);
+    go(x.sr, B * N);
+    go(x.sa, B * N);
+    go(x.se, B * N);
+    go(x.vslot, B * N);
+    go(x.cl_flag, B * N);
+    go(x.clabel, B * N);
+    go(x.cell_count, B * C);
+    go(x.cell_start, B * (C + 1));
+    go(x.cell_cursor, B * C);
+    go(x.cell_items, B * N);
+    go(x.zmin, B * C);
+    go(x.elev, B * C);
+    go(x.big_cells, B * C);
+    go(x.cand_count, B * L.cand_stride);
+    go(x.cand_start, B * L.cand_stride);
+    go(x.cx, B * N);
+    go(x.cy, B * N);
+    go(x.cz, B * N);
+    go(x.hyp, B * kRansacIters * 4);
+    go(x.hyp_valid, B * kRansacIters);
+    go(x.hyp_count, B * kRansacIters);
+    go(x.mt_raw, kMtRawCount);
+    go(x.pix_key, B * P);
+    go(x.state, B * P);
+    go(x.mapping, B * P);
+    go(x.pxyz, B * P);
+    go(x.qflag, B * P);
+    go(x.qmap, B * P);
+    go(x.qlist, B * P);
+    go(x.slot_w, B * P * 24);
+    go(x.slot_src, B * P * 24);
+    go(x.qsum_ok, B * P);
+    go(x.fin, B * P);
+    go(x.hkey, B * HC);
+    go(x.hmin, B * HC);
+    go(x.hdense, B * HC);
+    go(x.hcount, B * HC);
+    go(x.vlist, B * N);
+    go(x.vpar, B * N);
+    go(x.vmin, B * N);
+    go(x.vcnt, B * N);
+    go(x.cl_root, B * N);
+    go(x.cl_size, B * N);
+    go(x.cl_off, B * N);
+    go(x.cl_cursor, B * N);
+    go(x.cl_members, B * N);
+    go(x.hcx, 2 * B * N);
+    go(x.hcy, 2 * B * N);
+    go(x.hci, 2 * B * N);
+    go(x.hc_count, 2 * B * N);
+    go(x.hc_chunks, B * N);
+    go(x.hc_choff, B * N);
+    go(x.hull_idx, B * N);
+    go(x.hull_count, B * N);
+    go(x.hull_off, B * N);
+    go(x.hull_local, B * N);
+    go(x.blk, B * L.blk_stride);
+    go(x.fs, B * kFsCount);
+    go(x.frames, B);
+    go(x.cc, 1);
+    return static_cast<size_t>(p - base);
+}
+
+int stage_cloud(lp_context* c, const lp_cloud& cl, int32_t off, uint8_t* raw_dev) {
+    DevBuf& d = c->d;
+    const int n = cl.num_points;
+    if (n == 0) return LP_OK;
+    const hipMemcpyKind kind = cl.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const bool soa = cl.xyz_stride == 4 && (cl.ring == nullptr || cl.ring_stride == 2);
+    if (soa) {
+        LP_HIP(hipMemcpyAsync(d.x + off, cl.x, n * 4, kind, c->stream));
+        LP_HIP(hipMemcpyAsync(d.y + off, cl.y, n * 4, kind, c->stream));
+        LP_HIP(hipMemcpyAsync(d.z + off, cl.z, n * 4, kind, c->stream));
+        if (cl.ring) LP_HIP(hipMemcpyAsync(d.ring + off, cl.ring, n * 2, kind, c->stream));
+        return LP_OK;
+    }
+    // AoS: all fields inside one record of xyz_stride bytes.
+    const uint8_t* px = static_cast<const uint8_t*>(cl.x);
+    const uint8_t* lo = px;
+    for (const void* q : {cl.y, cl.z, cl.ring})
+        if (q && static_cast<const uint8_t*>(q) < lo) lo = static_cast<const uint8_t*>(q);
+    auto field = [&](const void* q) { return static_cast<int>(static_cast<const uint8_t*>(q) - lo); };
+    const int fx = field(cl.x), fy = field(cl.y), fz = field(cl.z), fr = cl.ring ? field(cl.ring) : -1;
+    const int stride = cl.xyz_stride;
+    if (stride < 12 || fx + 4 > stride || fy + 4 > stride || fz + 4 > stride || (fr >= 0 && fr + 2 > stride) ||
+        (cl.ring && cl.ring_stride != stride))
+        return fail(LP_ERR_INVALID_ARGUMENT, "unsupported point layout: fields must share one record stride");
+    if (static_cast<size_t>(n) * stride > static_cast<size_t>(c->L.N) * 4 * 8)
+        return fail(LP_ERR_CAPACITY, "point record buffer too large");
+    LP_HIP(hipMemcpyAsync(raw_dev, lo, static_cast<size_t>(n) * stride, kind, c->stream));
+    hipLaunchKernelGGL(k_deinterleave, dim3((n + 255) / 256), dim3(256), 0, c->stream, raw_dev, n, stride, fx, fy,
+                       fz, fr, d.x + off, d.y + off, d.z + off, d.ring + off);
+    LP_HIP(hipGetLastError());
+    return LP_OK;
+}
+
+int run_pipeline(lp_context* c, int B, bool segment_only) {
+    c->L.B_active = B;
+    launch_segment_front(c, B);
+    launch_jcp(c, B);
+    if (!segment_only) {
+        launch_cluster(c, B, true);
+        launch_hull(c, B);
+    }
+    LP_HIP(hipGetLastError());
+    return LP_OK;
+}
+
+int status_of(uint32_t err) {
+    if (err & kErrVoxelOverflow) return fail(LP_ERR_VOXEL_OVERFLOW, "more occupied voxels than max_voxels");
+    if (err & kErrRansacSingle)
+        return fail(LP_ERR_RANSAC_SINGLE_CANDIDATE, "exactly one RANSAC candidate (reference never terminates)");
+    if (err & (kErrCellCapacity | kErrQueueCapacity | kErrHullCapacity | kErrRngExhausted))
+        return fail(LP_ERR_CAPACITY, "a device capacity was exceeded (cell, JCP queue, hull or RNG stream)");
+    return LP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lp_segmenter_config_default(lp_segmenter_config* c) {
+    c->elevation_up_deg = 2.0F;
+    c->elevation_down_deg = -24.8F;
+    c->image_width = 2048;
+    c->image_height = 64;
+    c->assume_unorganized_cloud = 0;
+    c->grid_radial_spacing_m = 2.0F;
+    c->grid_slice_resolution_deg = 1.0F;
+    c->ground_height_threshold_m = 0.2F;
+    c->road_maximum_slope_m_per_m = 0.2F;
+    c->min_distance_m = 2.0F;
+    c->max_distance_m = 100.0F;
+    c->sensor_height_m = 1.73F;
+    c->kernel_threshold_distance_m = 1.0F;
+    c->amplification_factor = 5.0F;
+    c->z_min_m = -3.0F;
+    c->z_max_m = 4.0F;
+}
+
+void lp_clusterer_config_default(lp_clusterer_config* c) {
+    c->voxel_grid_range_resolution_m = 0.4F;
+    c->voxel_grid_azimuth_resolution_deg = 1.0F;
+    c->voxel_grid_elevation_resolution_deg = 1.5F;
+    c->min_cluster_size = 3;
+}
+
+void lp_clusterer_config_shipped(lp_clusterer_config* c) {
+    lp_clusterer_config_default(c);
+    c->voxel_grid_elevation_resolution_deg = 3.0F;
+}
+
+void lp_capacity_default(lp_capacity* cap) {
+    cap->max_frames = 1;
+    cap->max_points_per_frame = 200000;  // Segmenter::MAX_CLOUD_SIZE, segmenter.hpp:151
+    cap->max_image_height = 64;
+    cap->max_image_width = 2048;
+    cap->max_voxels = 200000;            // clusterer.cpp:36
+    cap->device = 0;
+}
+
+const char* lp_last_error(void) { return g_last_error.c_str(); }
+
+int lp_context_create(const lp_capacity* cap, lp_context** out) {
+    if (!cap || !out) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (cap->max_frames < 1 || cap->max_points_per_frame < 1 || cap->max_image_height < 1 ||
+        cap->max_image_width < 1 || cap->max_image_height > 1024 || cap->max_voxels < 1)
+        return fail(LP_ERR_INVALID_ARGUMENT, "invalid capacity");
+    int ndev = 0;
+    LP_HIP(hipGetDeviceCount(&ndev));
+    if (cap->device < 0 || cap->device >= ndev) return fail(LP_ERR_INVALID_ARGUMENT, "no such HIP device");
+    LP_HIP(hipSetDevice(cap->device));
+    auto* c = new lp_context();
+    c->device = cap->device;
+    c->cap = *cap;
+    Layout& L = c->L;
+    L.B = cap->max_frames;
+    L.N = cap->max_points_per_frame;
+    L.H = cap->max_image_height;
+    L.W = cap->max_image_width;
+    L.P = L.H * L.W;
+    L.B_active = 0;
+    L.idx_bits = std::max(1, ceil_log2(L.N));
+    if (L.idx_bits > 24) {
+        delete c;
+        return fail(LP_ERR_INVALID_ARGUMENT, "max_points_per_frame above 2^24");
+    }
+    L.cell_stride = 32768;
+    L.cand_stride = 4 * 4096;
+    L.hash_cap = 1 << ceil_log2(2 * static_cast<int64_t>(L.N));
+    L.blk_stride = (std::max(L.N, L.P) + 2047) / 2048 + 1;
+    const size_t bytes = layout_bytes(L, nullptr, nullptr);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->arena, bytes);
+    if (e != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(LP_ERR_HIP, std::string("context allocation: ") + hipGetErrorString(e));
+    }
+    c->arena_bytes = bytes;
+    layout_bytes(L, &c->d, static_cast<char*>(c->arena));
+    launch_mt19937(c->d.mt_raw, kMtRawCount, c->stream);
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        lp_context_destroy(c);
+        return fail(LP_ERR_HIP, std::string("context init: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return LP_OK;
+}
+
+int lp_context_destroy(lp_context* c) {
+    if (!c) return LP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto e : c->events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LP_OK;
+}
+
+void* lp_context_stream(lp_context* c) { return c ? c->stream : nullptr; }
+
+int lp_batch_stage(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
+                   const lp_clusterer_config* ccfg, int32_t num_frames, const lp_cloud* clouds) {
+    if (!c || !seg_cfgs || !ccfg || !clouds) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_frames < 1 || num_frames > c->L.B) return fail(LP_ERR_CAPACITY, "batch larger than max_frames");
+    if (num_seg_cfgs != 1 && num_seg_cfgs != num_frames)
+        return fail(LP_ERR_INVALID_ARGUMENT, "num_seg_cfgs must be 1 or num_frames");
+    LP_HIP(hipSetDevice(c->device));
+    c->frames.assign(num_frames, FrameDesc{});
+    int32_t off = 0;
+    for (int f = 0; f < num_frames; ++f) {
+        const lp_cloud& cl = clouds[f];
+        if (cl.num_points < 0 || cl.num_points > c->L.N)
+            return fail(LP_ERR_CAPACITY, "frame has more points than max_points_per_frame");
+        if (cl.num_points > 0 && (!cl.x || !cl.y || !cl.z)) return fail(LP_ERR_INVALID_ARGUMENT, "null coordinates");
+        FrameDesc& fd = c->frames[f];
+        std::string why;
+        if (!derive_seg(seg_cfgs[num_seg_cfgs == 1 ? 0 : f], c->L, fd.sc, why))
+            return fail(LP_ERR_INVALID_ARGUMENT, why);
+        fd.pt_off = off;
+        fd.n = cl.num_points;
+        fd.pix_off = f * c->L.P;
+        fd.has_ring = cl.ring != nullptr ? 1 : 0;
+        off += cl.num_points;
+    }
+    // Inputs: SoA copies straight in; AoS records through a device staging
+    // area (the slot-weight arrays are free at this point of the pipeline).
+    uint8_t* raw = reinterpret_cast<uint8_t*>(c->d.slot_w);
+    for (int f = 0; f < num_frames; ++f) {
+        const int st = stage_cloud(c, clouds[f], c->frames[f].pt_off, raw);
+        if (st != LP_OK) return st;
+    }
+    derive_clu(*ccfg, c->cap.max_voxels, c->cc);
+    LP_HIP(hipMemcpyAsync(c->d.frames, c->frames.data(), sizeof(FrameDesc) * num_frames, hipMemcpyHostToDevice,
+                          c->stream));
+    LP_HIP(hipMemcpyAsync(c->d.cc, &c->cc, sizeof(CluConst), hipMemcpyHostToDevice, c->stream));
+    c->num_frames = num_frames;
+    c->staged_points = off;
+    return LP_OK;
+}
+
+int lp_batch_run(lp_context* c, int32_t use_graph) {
+    if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+    LP_HIP(hipSetDevice(c->device));
+    const int B = c->num_frames;
+    if (!use_graph) return run_pipeline(c, B, false);
+    if (!c->graph_exec || c->graph_frames != B) {
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        LP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int st = run_pipeline(c, B, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (st != LP_OK) return st;
+        if (e != hipSuccess) return fail(LP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        c->graph = g;
+        LP_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+        c->graph_frames = B;
+    }
+    LP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    return LP_OK;
+}
+
+int lp_synchronize(lp_context* c) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    LP_HIP(hipStreamSynchronize(c->stream));
+    return LP_OK;
+}
+
+int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* results) {
+    if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+    LP_HIP(hipSetDevice(c->device));
+    const int B = c->num_frames;
+    const Layout& L = c->L;
+    std::vector<uint32_t> fs(static_cast<size_t>(B) * kFsCount);
+    LP_HIP(hipMemcpyAsync(fs.data(), c->d.fs, fs.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    int overall = LP_OK;
+    for (int f = 0; f < B; ++f) {
+        const uint32_t* s = fs.data() + static_cast<size_t>(f) * kFsCount;
+        const FrameDesc& fd = c->frames[f];
+        lp_frame_result r{};
+        r.status = status_of(s[kFsErr]);
+        r.num_points = fd.n;
+        r.num_obstacles = static_cast<int32_t>(s[kFsObstacles]);
+        r.num_voxels = static_cast<int32_t>(s[kFsVoxels]);
+        r.num_clusters = static_cast<int32_t>(s[kFsClusters]);
+        r.num_hull_vertices = static_cast<int32_t>(s[kFsHullTotal]);
+        r.jcp_queue = static_cast<int32_t>(s[kFsQueue]);
+        r.ransac.candidates = s[kFsCands];
+        r.ransac.best_inliers = s[kFsBest];
+        r.ransac.applied = static_cast<int32_t>(s[kFsApplied]);
+        uint32_t pl[4] = {s[kFsPlaneA], s[kFsPlaneB], s[kFsPlaneC], s[kFsPlaneD]};
+        if (!r.ransac.applied) {
+            const float dflt[4] = {0.0F, 0.0F, 1.0F, 0.0F};
+            std::memcpy(pl, dflt, sizeof(pl));
+        }
+        std::memcpy(r.ransac.plane, pl, sizeof(pl));
+        if (results) results[f] = r;
+        if (r.status != LP_OK && overall == LP_OK) overall = r.status;
+        if (!outputs) continue;
+        const lp_frame_outputs& o = outputs[f];
+        const size_t ob = static_cast<size_t>(f) * L.N;
+        if (o.seg_labels && fd.n)
+            LP_HIP(hipMemcpyAsync(o.seg_labels, c->d.labels + fd.pt_off, fd.n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (r.status != LP_OK) continue;
+        const int no = r.num_obstacles, K = r.num_clusters, V = r.num_hull_vertices;
+        if (o.obstacle_index && no)
+            LP_HIP(hipMemcpyAsync(o.obstacle_index, c->d.obs_src + ob, no * 4, hipMemcpyDeviceToHost, c->stream));
+        if (o.cluster_labels && no)
+            LP_HIP(hipMemcpyAsync(o.cluster_labels, c->d.clabel + ob, no * 4, hipMemcpyDeviceToHost, c->stream));
+        if (o.hull_offsets) {
+            if (K) LP_HIP(hipMemcpyAsync(o.hull_offsets, c->d.hull_off + ob, K * 4, hipMemcpyDeviceToHost, c->stream));
+            o.hull_offsets[K] = V;
+        }
+        if (o.hull_indices && V) {
+            if (V > o.hull_capacity) {
+                if (results) results[f].status = fail(LP_ERR_CAPACITY, "hull_capacity too small");
+                overall = LP_ERR_CAPACITY;
+                continue;
+            }
+            LP_HIP(hipMemcpyAsync(o.hull_indices, c->d.hull_local + ob, V * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    LP_HIP(hipStreamSynchronize(c->stream));
+    return overall;
+}
+
+int lp_process_frames(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
+                      const lp_clusterer_config* ccfg, int32_t num_frames, const lp_cloud* clouds,
+                      const lp_frame_outputs* outputs, lp_frame_result* results) {
+    int st = lp_batch_stage(c, seg_cfgs, num_seg_cfgs, ccfg, num_frames, clouds);
+    if (st != LP_OK) return st;
+    st = lp_batch_run(c, 0);
+    if (st != LP_OK) return st;
+    return lp_batch_fetch(c, outputs, results);
+}
+
+int lp_segment(lp_context* c, const lp_segmenter_config* cfg, const lp_cloud* cloud, uint32_t* labels,
+               lp_ransac_result* ransac) {
+    if (!c || !cfg || !cloud || (!labels && cloud->num_points > 0))
+        return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    lp_clusterer_config cc;
+    lp_clusterer_config_shipped(&cc);
+    int st = lp_batch_stage(c, cfg, 1, &cc, 1, cloud);
+    if (st != LP_OK) return st;
+    st = run_pipeline(c, 1, true);
+    if (st != LP_OK) return st;
+    uint32_t fs[kFsCount];
+    LP_HIP(hipMemcpyAsync(fs, c->d.fs, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    if (cloud->num_points)
+        LP_HIP(hipMemcpyAsync(labels, c->d.labels, cloud->num_points * 4, hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    st = status_of(fs[kFsErr] & ~(kErrVoxelOverflow | kErrHullCapacity));
+    if (ransac) {
+        ransac->candidates = fs[kFsCands];
+        ransac->best_inliers = fs[kFsBest];
+        ransac->applied = static_cast<int32_t>(fs[kFsApplied]);
+        if (ransac->applied) {
+            std::memcpy(ransac->plane, &fs[kFsPlaneA], 16);
+        } else {
+            const float dflt[4] = {0.0F, 0.0F, 1.0F, 0.0F};
+            std::memcpy(ransac->plane, dflt, 16);
+        }
+    }
+    return st;
+}
+
+int lp_cluster(lp_context* c, const lp_clusterer_config* cfg, const lp_cloud* cloud, int32_t* labels) {
+    if (!c || !cfg || !cloud || (!labels && cloud->num_points > 0))
+        return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (cloud->num_points < 0 || cloud->num_points > c->L.N)
+        return fail(LP_ERR_CAPACITY, "cloud has more points than max_points_per_frame");
+    LP_HIP(hipSetDevice(c->device));
+    FrameDesc fd{};
+    fd.pt_off = 0;
+    fd.n = cloud->num_points;
+    fd.pix_off = 0;
+    fd.has_ring = 0;  // no polar grid, no image: the clear stage only resets the voxel state
+    c->frames.assign(1, fd);
+    lp_cloud xyz = *cloud;
+    xyz.ring = nullptr;
+    int st = stage_cloud(c, xyz, 0, reinterpret_cast<uint8_t*>(c->d.slot_w));
+    if (st != LP_OK) return st;
+    derive_clu(*cfg, c->cap.max_voxels, c->cc);
+    LP_HIP(hipMemcpyAsync(c->d.frames, c->frames.data(), sizeof(FrameDesc), hipMemcpyHostToDevice, c->stream));
+    LP_HIP(hipMemcpyAsync(c->d.cc, &c->cc, sizeof(CluConst), hipMemcpyHostToDevice, c->stream));
+    c->num_frames = 1;
+    c->L.B_active = 1;
+    launch_clear(c, 1);
+    launch_cluster(c, 1, false);
+    LP_HIP(hipGetLastError());
+    uint32_t fs[kFsCount];
+    LP_HIP(hipMemcpyAsync(fs, c->d.fs, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    if (cloud->num_points)
+        LP_HIP(hipMemcpyAsync(labels, c->d.clabel, cloud->num_points * 4, hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    c->num_frames = 0;  // the staged frame is not a full-pipeline batch
+    return status_of(fs[kFsErr]);
+}
+
+int lp_convex_hulls(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_lists,
+                    int32_t* hull_offsets, int32_t* hull_indices, int32_t hull_capacity) {
+    if (!c || !offsets || !hull_offsets || num_lists < 0) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_lists == 0) {
+        hull_offsets[0] = 0;
+        return LP_OK;
+    }
+    const int32_t total = offsets[num_lists];
+    if (offsets[0] != 0 || total < 0 || total > c->L.N || num_lists > c->L.N)
+        return fail(LP_ERR_CAPACITY, "point lists exceed max_points_per_frame");
+    for (int k = 0; k < num_lists; ++k)
+        if (offsets[k + 1] < offsets[k]) return fail(LP_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+    if (total > 0 && (!x || !y)) return fail(LP_ERR_INVALID_ARGUMENT, "null coordinates");
+    LP_HIP(hipSetDevice(c->device));
+    c->L.B_active = 1;
+    LP_HIP(hipMemcpyAsync(c->d.hc_choff, offsets, (num_lists + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                          c->stream));
+    if (total) {
+        LP_HIP(hipMemcpyAsync(c->d.hcx, x, total * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        LP_HIP(hipMemcpyAsync(c->d.hcy, y, total * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    launch_hull_lists(c, num_lists, total);
+    LP_HIP(hipGetLastError());
+    uint32_t fs[kFsCount];
+    LP_HIP(hipMemcpyAsync(fs, c->d.fs, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    const int32_t V = static_cast<int32_t>(fs[kFsHullTotal]);
+    if (fs[kFsErr] & kErrHullCapacity) return fail(LP_ERR_CAPACITY, "hull candidates exceed the device capacity");
+    if (V > hull_capacity || (V > 0 && !hull_indices)) return fail(LP_ERR_CAPACITY, "hull_capacity too small");
+    LP_HIP(hipMemcpyAsync(hull_offsets, c->d.hull_off, num_lists * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          c->stream));
+    if (V) LP_HIP(hipMemcpyAsync(hull_indices, c->d.hull_local, V * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    hull_offsets[num_lists] = V;
+    c->num_frames = 0;
+    return LP_OK;
+}
+
+int lp_debug_fetch(lp_context* c, int32_t frame, int32_t which, void* dst, int32_t capacity) {
+    if (!c || frame < 0 || frame >= static_cast<int32_t>(c->frames.size()) || !dst)
+        return fail(LP_ERR_INVALID_ARGUMENT, "bad debug request");
+    LP_HIP(hipSetDevice(c->device));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    const FrameDesc& fd = c->frames[frame];
+    const Layout& L = c->L;
+    const int32_t P = fd.sc.H * fd.sc.W;
+    const size_t po = static_cast<size_t>(frame) * L.P;
+    uint32_t fs[kFsCount];
+    LP_HIP(hipMemcpy(fs, c->d.fs + static_cast<size_t>(frame) * kFsCount, sizeof(fs), hipMemcpyDeviceToHost));
+    const int32_t nq = static_cast<int32_t>(fs[kFsQueue]);
+    switch (which) {
+        case LP_DEBUG_ELEVATION_MAP: {
+            const int32_t n = fd.sc.cells;
+            if (n <= capacity)
+                LP_HIP(hipMemcpy(dst, c->d.elev + static_cast<size_t>(frame) * L.cell_stride, n * 4,
+                                 hipMemcpyDeviceToHost));
+            return n;
+        }
+        case LP_DEBUG_MAPPING:
+            if (P <= capacity) LP_HIP(hipMemcpy(dst, c->d.mapping + po, P * 4, hipMemcpyDeviceToHost));
+            return P;
+        case LP_DEBUG_STATE_RECM:
+            if (P <= capacity) LP_HIP(hipMemcpy(dst, c->d.state + po, P, hipMemcpyDeviceToHost));
+            return P;
+        case LP_DEBUG_JCP_QUEUE:
+            if (nq <= capacity && nq > 0) LP_HIP(hipMemcpy(dst, c->d.qlist + po, nq * 4, hipMemcpyDeviceToHost));
+            return nq;
+        case LP_DEBUG_STATE_FINAL: {
+            if (P > capacity) return P;
+            std::vector<uint8_t> st(P), fin(std::max(nq, 1));
+            std::vector<int32_t> qm(P);
+            LP_HIP(hipMemcpy(st.data(), c->d.state + po, P, hipMemcpyDeviceToHost));
+            LP_HIP(hipMemcpy(qm.data(), c->d.qmap + po, P * 4, hipMemcpyDeviceToHost));
+            if (nq > 0) LP_HIP(hipMemcpy(fin.data(), c->d.fin + po, nq, hipMemcpyDeviceToHost));
+            uint8_t* o = static_cast<uint8_t*>(dst);
+            for (int32_t p = 0; p < P; ++p) o[p] = qm[p] >= 0 ? fin[qm[p]] : st[p];
+            return P;
+        }
+        default:
+            return fail(LP_ERR_INVALID_ARGUMENT, "unknown debug buffer");
+    }
+}
+
+int lp_stage_names(lp_context* c, char* buf, int32_t buf_len) {
+    if (!c || !buf || buf_len <= 0) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    std::string s;
+    for (size_t i = 0; i < c->stage_names.size(); ++i) {
+        if (i) s += ",";
+        s += c->stage_names[i];
+    }
+    std::snprintf(buf, static_cast<size_t>(buf_len), "%s", s.c_str());
+    return static_cast<int>(c->stage_names.size());
+}
+
+int lp_profile_run(lp_context* c, int32_t runs, float* stage_ms, int32_t max_stages, int32_t* num_stages) {
+    if (!c || c->num_frames < 1 || runs < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+    LP_HIP(hipSetDevice(c->device));
+    std::vector<float> acc;
+    for (int r = 0; r < runs; ++r) {
+        for (auto e : c->events) (void)hipEventDestroy(e);
+        c->events.clear();
+        c->stage_names.clear();
+        c->profiling = true;
+        const int st = run_pipeline(c, c->num_frames, false);
+        stage_mark(c, "end");
+        c->profiling = false;
+        if (st != LP_OK) return st;
+        LP_HIP(hipStreamSynchronize(c->stream));
+        acc.resize(c->events.size() - 1, 0.0f);
+        for (size_t i = 0; i + 1 < c->events.size(); ++i) {
+            float ms = 0.0f;
+            LP_HIP(hipEventElapsedTime(&ms, c->events[i], c->events[i + 1]));
+            acc[i] += ms;
+        }
+    }
+    const int ns = static_cast<int>(acc.size());
+    for (int i = 0; i < ns && i < max_stages; ++i) stage_ms[i] = acc[i] / runs;
+    if (num_stages) *num_stages = ns;
+    c->stage_names.pop_back();
+    return LP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+// lp_cluster.hip -- curved-voxel clustering on gfx950.
+//
+// Reference: lidar_processing_lib/src/clusterer.cpp:55-239.  The reference
+// hashes (range, azimuth, elevation) voxel keys into a linear-probing map
+// (static_unordered_map.hpp) and labels voxels by BFS from each unlabelled
+// point in input order over the 26-neighbourhood (range and elevation
+// bounded, azimuth wrapping by one), numbering components in the order of
+// their first point; removeSmallClusters then drops components with fewer
+// than min_cluster_size points and renumbers the rest in order.  The result
+// is therefore: label = rank, among surviving components, of the
+// component's minimum point index.
+//
+// Here: voxel keys go into an open-addressing HBM hash set by atomicCAS
+// (one dense id per occupied voxel, per-voxel point count and minimum point
+// index by atomics on the voxel's own slot), components come from a
+// lock-free union-find over dense ids held in LDS (one workgroup per frame;
+// a global-memory union-find takes over above kCcLds voxels), and labels
+// from a scan over "first point of a surviving component" flags.
+#include "lp_kernels.h"
+
+namespace lp {
+
+using namespace lpd;
+
+constexpr float kTwoPiF = 6.28318548202514648438f;   // Clusterer::TWO_M_PIf
+constexpr float kHalfPiF = 1.57079637050628662109f;  // M_PI_2f
+constexpr int kCcLds = 32768;                        // voxels per frame for the LDS union-find
+
+// Obstacle split: input-order compaction of OBSTACLE points
+// (processor.cpp:562-579); cl_flag holds the scan of obs_flag on entry.
+__global__ void k_obs_gather(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n) return;
+    const int gi = fd.pt_off + i;
+    if (d.labels[gi] != LP_LABEL_OBSTACLE) return;
+    const size_t ob = (size_t)f * L.N;
+    const uint32_t k = d.cl_flag[ob + i];
+    d.obs_src[ob + k] = i;
+    d.ox[ob + k] = d.x[gi];
+    d.oy[ob + k] = d.y[gi];
+    d.oz[ob + k] = d.z[gi];
+}
+
+// Standalone Clusterer::cluster input: the staged cloud is the obstacle cloud.
+__global__ void k_cluster_input(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) d.fs[f * kFsCount + kFsObstacles] = static_cast<uint32_t>(fd.n);
+    if (i >= fd.n) return;
+    const size_t ob = (size_t)f * L.N;
+    const int gi = fd.pt_off + i;
+    d.obs_src[ob + i] = i;
+    d.ox[ob + i] = d.x[gi];
+    d.oy[ob + i] = d.y[gi];
+    d.oz[ob + i] = d.z[gi];
+}
+
+__device__ __forceinline__ uint32_t max_bits(float v) {
+    // Non-negative floats order like their bit patterns; -0 counts as +0
+    // (only ceilf(max / res) is used, which is the same for both).
+    const uint32_t u = f2u(v);
+    return u == 0x80000000u ? 0u : u;
+}
+
+// cartesianToSpherical (clusterer.cpp:55-100).
+__global__ void k_spherical(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= static_cast<unsigned>(nobs)) return;
+    const size_t ob = (size_t)f * L.N;
+    uint32_t mr = 0, ma = 0, me = 0;
+    if (k < nobs) {
+        const float x = d.ox[ob + k], y = d.oy[ob + k], z = d.oz[ob + k];
+        float az = fdlibm_atan2f(y, x);
+        az = (az < 0) ? (az + kTwoPiF) : az;
+        const float dsq = x * x + y * y;
+        const float dist = sqrtf(dsq);
+        const float range = sqrtf(dsq + z * z);
+        const float el = fdlibm_atanf(z / dist) + kHalfPiF;
+        d.sr[ob + k] = range;
+        d.sa[ob + k] = az;
+        d.se[ob + k] = el;
+        mr = max_bits(range);
+        ma = max_bits(az);
+        me = max_bits(el);
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        mr = max(mr, __shfl_xor(mr, o, kWave));
+        ma = max(ma, __shfl_xor(ma, o, kWave));
+        me = max(me, __shfl_xor(me, o, kWave));
+    }
+    if (lane_id() == 0) {
+        atomicMax(&d.fs[f * kFsCount + kFsMaxR], mr);
+        atomicMax(&d.fs[f * kFsCount + kFsMaxA], ma);
+        atomicMax(&d.fs[f * kFsCount + kFsMaxE], me);
+    }
+}
+
+struct Dims {
+    int32_t nr, na, ne;
+};
+
+// num_range / num_azimuth / num_elevation (clusterer.cpp:94-99).
+__device__ __forceinline__ Dims dims_of(const DevBuf& d, int f, const CluConst& cc) {
+    const uint32_t* fs = d.fs + f * kFsCount;
+    Dims dm;
+    dm.nr = static_cast<int32_t>(ceilf(u2f(fs[kFsMaxR]) / cc.res_r) + 1);
+    dm.na = static_cast<int32_t>(ceilf(u2f(fs[kFsMaxA]) / cc.res_az) + 1);
+    dm.ne = static_cast<int32_t>(ceilf(u2f(fs[kFsMaxE]) / cc.res_el) + 1);
+    return dm;
+}
+
+__device__ __forceinline__ uint32_t hash_slot(int32_t key, uint32_t mask) {
+    uint32_t h = static_cast<uint32_t>(key) * 0x9E3779B1u;
+    h ^= h >> 15;
+    return h & mask;
+}
+
+// Voxel keys (clusterer.cpp:102-120, flatVoxelIndex clusterer.hpp:135-141)
+// inserted into the per-frame hash set.
+__global__ void k_voxel_insert(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nobs) return;
+    const CluConst cc = *d.cc;
+    const Dims dm = dims_of(d, f, cc);
+    const size_t ob = (size_t)f * L.N;
+    const int32_t ri = static_cast<int32_t>(d.sr[ob + k] / cc.res_r);
+    const int32_t ai = static_cast<int32_t>(d.sa[ob + k] / cc.res_az);
+    const int32_t ei = static_cast<int32_t>(d.se[ob + k] / cc.res_el);
+    const int32_t key = dm.nr * (dm.na * ei + ai) + ri;
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    const size_t ho = (size_t)f * L.hash_cap;
+    uint32_t h = hash_slot(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+        if (prev == -1) {
+            const uint32_t pos = atomicAdd(&d.fs[f * kFsCount + kFsVoxels], 1u);
+            if (pos < static_cast<uint32_t>(L.N)) {
+                d.vlist[ob + pos] = static_cast<int32_t>(h);
+                d.vpar[ob + pos] = static_cast<int32_t>(pos);
+                d.vmin[ob + pos] = 0x7fffffff;
+                d.vcnt[ob + pos] = 0;
+            }
+            d.hdense[ho + h] = static_cast<int32_t>(pos);
+            break;
+        }
+        if (prev == key) break;
+        h = (h + 1) & mask;
+    }
+    atomicMin(&d.hmin[ho + h], k);
+    atomicAdd(&d.hcount[ho + h], 1u);
+    d.vslot[ob + k] = static_cast<int32_t>(h);
+}
+
+__global__ void k_voxel_check(DevBuf d, Layout L) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= L.B_active) return;
+    if (static_cast<int32_t>(d.fs[f * kFsCount + kFsVoxels]) > d.cc->max_voxels)
+        atomicOr(&d.fs[f * kFsCount + kFsErr], kErrVoxelOverflow);
+}
+
+__device__ __forceinline__ int32_t hash_find(const int32_t* hkey, uint32_t mask, int32_t key) {
+    uint32_t h = hash_slot(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t k = hkey[h];
+        if (k == key) return static_cast<int32_t>(h);
+        if (k == -1) return -1;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+// Half of the 26-neighbourhood (the other half is covered from the
+// neighbour's side): offsets (dr, da, de) with (de, da, dr) > (0, 0, 0).
+__constant__ static const int8_t kHalfNb[13][3] = {
+    {1, 0, 0},  {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},  {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+    {-1, 0, 1}, {0, 0, 1},  {1, 0, 1},  {-1, 1, 1}, {0, 1, 1},   {1, 1, 1}};
+
+// Dense id of neighbour q of the voxel with flat key `key`
+// (clusterer.cpp:150-176 bounds: range and elevation in [0, num), azimuth
+// wraps by one step); -1 if that voxel is empty.
+__device__ __forceinline__ int32_t voxel_neighbour(const DevBuf& d, size_t ho, uint32_t mask, const Dims& dm,
+                                                   int32_t key, int q) {
+    const int32_t ri = key % dm.nr;
+    const int32_t rest = key / dm.nr;
+    const int32_t ai = rest % dm.na;
+    const int32_t ei = rest / dm.na;
+    const int32_t rr = ri + kHalfNb[q][0];
+    const int32_t ee = ei + kHalfNb[q][2];
+    if (rr < 0 || rr >= dm.nr || ee < 0 || ee >= dm.ne) return -1;
+    int32_t aa = ai + kHalfNb[q][1];
+    if (aa < 0)
+        aa += dm.na;
+    else if (aa >= dm.na)
+        aa -= dm.na;
+    const int32_t g = hash_find(d.hkey + ho, mask, dm.nr * (dm.na * ee + aa) + rr);
+    return g < 0 ? -1 : d.hdense[ho + g];
+}
+
+// ---- union-find in LDS ----------------------------------------------------
+__device__ __forceinline__ int32_t lds_find(volatile int32_t* par, int32_t x) {
+    for (;;) {
+        const int32_t p = par[x];
+        if (p == x) return x;
+        const int32_t gp = par[p];
+        if (gp == p) return p;
+        par[x] = gp;  // path halving; x is not a root, so no CAS targets it
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ void lds_union(int32_t* par, int32_t a, int32_t b) {
+    for (;;) {
+        a = lds_find(par, a);
+        b = lds_find(par, b);
+        if (a == b) return;
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(par + b, b, a) == b) return;  // hook the larger root under the smaller
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_voxel_cc(DevBuf d, Layout L) {
+    __shared__ int32_t par[kCcLds];
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
+    if (U > static_cast<uint32_t>(kCcLds)) {
+        if (tid == 0) d.fs[f * kFsCount + kFsCcGlobal] = 1;
+        return;
+    }
+    const CluConst cc = *d.cc;
+    const Dims dm = dims_of(d, f, cc);
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    for (uint32_t v = tid; v < U; v += 1024) par[v] = static_cast<int32_t>(v);
+    __syncthreads();
+    for (uint32_t v = tid; v < U; v += 1024) {
+        const int32_t key = d.hkey[ho + d.vlist[ob + v]];
+        for (int q = 0; q < 13; ++q) {
+            const int32_t u = voxel_neighbour(d, ho, mask, dm, key, q);
+            if (u >= 0) lds_union(par, static_cast<int32_t>(v), u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t v = tid; v < U; v += 1024) d.vpar[ob + v] = lds_find(par, static_cast<int32_t>(v));
+}
+
+// ---- union-find in global memory (frames with more than kCcLds voxels) ----
+__device__ __forceinline__ int32_t load_parent(const int32_t* p, int32_t x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t g_find(int32_t* parent, int32_t x) {
+    int32_t p = load_parent(parent, x);
+    while (p != x) {
+        const int32_t gp = load_parent(parent, p);
+        if (gp != p) atomicCAS(parent + x, p, gp);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__global__ void k_voxel_cc_global(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    if (!d.fs[f * kFsCount + kFsCcGlobal]) return;
+    const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= U) return;
+    const CluConst cc = *d.cc;
+    const Dims dm = dims_of(d, f, cc);
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    int32_t* par = d.vpar + ob;
+    const int32_t key = d.hkey[ho + d.vlist[ob + v]];
+    for (int q = 0; q < 13; ++q) {
+        const int32_t u = voxel_neighbour(d, ho, mask, dm, key, q);
+        if (u < 0) continue;
+        int32_t a = static_cast<int32_t>(v), b = u;
+        for (;;) {
+            a = g_find(par, a);
+            b = g_find(par, b);
+            if (a == b) break;
+            if (a > b) {
+                const int32_t t = a;
+                a = b;
+                b = t;
+            }
+            if (atomicCAS(par + b, b, a) == b) break;
+        }
+    }
+}
+
+__global__ void k_voxel_compress_global(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    if (!d.fs[f * kFsCount + kFsCcGlobal]) return;
+    const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= U) return;
+    int32_t* par = d.vpar + (size_t)f * L.N;
+    par[v] = g_find(par, static_cast<int32_t>(v));
+}
+
+// Component minimum point index and point count (removeSmallClusters counts
+// points, clusterer.cpp:214-220): one atomic per voxel.
+__global__ void k_voxel_stats(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= U) return;
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const int32_t r = d.vpar[ob + v];
+    const int32_t h = d.vlist[ob + v];
+    atomicMin(&d.vmin[ob + r], d.hmin[ho + h]);
+    atomicAdd(&d.vcnt[ob + r], d.hcount[ho + h]);
+}
+
+__device__ __forceinline__ int32_t root_of_point(const DevBuf& d, size_t ob, size_t ho, int k) {
+    return d.vpar[ob + d.hdense[ho + d.vslot[ob + k]]];
+}
+
+__global__ void k_component_flag(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nobs) return;
+    const CluConst cc = *d.cc;
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const int32_t r = root_of_point(d, ob, ho, k);
+    d.cl_flag[ob + k] = (d.vmin[ob + r] == k && d.vcnt[ob + r] >= cc.min_size) ? 1u : 0u;
+}
+
+// cl_flag holds the exclusive scan of the first-point flags on entry.
+__global__ void k_component_label(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nobs) return;
+    const CluConst cc = *d.cc;
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const int32_t r = root_of_point(d, ob, ho, k);
+    const uint32_t cnt = d.vcnt[ob + r];
+    int32_t lab = -1;
+    if (cnt >= cc.min_size) {
+        const int32_t first = d.vmin[ob + r];
+        lab = static_cast<int32_t>(d.cl_flag[ob + first]);
+        if (first == k) {
+            d.cl_root[ob + lab] = r;
+            d.cl_size[ob + lab] = cnt;
+            d.cl_cursor[ob + lab] = 0;
+        }
+    }
+    d.clabel[ob + k] = lab;
+}
+
+// Members of each cluster (unordered within the cluster) and the round-0
+// hull input: the cluster's points as PointXY doubles (processor.cpp:647).
+__global__ void k_cluster_scatter(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nobs) return;
+    const size_t ob = (size_t)f * L.N;
+    const int32_t lab = d.clabel[ob + k];
+    if (lab < 0) return;
+    const uint32_t pos = atomicAdd(&d.cl_cursor[ob + lab], 1u);
+    const uint32_t dst = d.cl_off[ob + lab] + pos;
+    d.cl_members[ob + dst] = k;
+    d.hcx[ob + dst] = static_cast<double>(d.ox[ob + k]);
+    d.hcy[ob + dst] = static_cast<double>(d.oy[ob + k]);
+    d.hci[ob + dst] = k;
+}
+
+void launch_cluster(lp_context* c, int B, bool from_split) {
+    DevBuf& d = c->d;
+    const Layout& L = c->L;
+    hipStream_t st = c->stream;
+    const int nb = (L.N + 255) / 256;
+    stage_mark(c, "split");
+    if (from_split) {
+        launch_scan(c, B, d.obs_flag, d.cl_flag, L.N, kCountPoints, 0, kFsObstacles);
+        hipLaunchKernelGGL(k_obs_gather, dim3(nb, B), dim3(256), 0, st, d, L);
+    } else {
+        hipLaunchKernelGGL(k_cluster_input, dim3(nb, B), dim3(256), 0, st, d, L);
+    }
+    stage_mark(c, "voxels");
+    hipLaunchKernelGGL(k_spherical, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_voxel_insert, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_voxel_check, dim3((B + 63) / 64), dim3(64), 0, st, d, L);
+    stage_mark(c, "components");
+    hipLaunchKernelGGL(k_voxel_cc, dim3(B), dim3(1024), 0, st, d, L);
+    hipLaunchKernelGGL(k_voxel_cc_global, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_voxel_compress_global, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_voxel_stats, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_component_flag, dim3(nb, B), dim3(256), 0, st, d, L);
+    launch_scan(c, B, d.cl_flag, d.cl_flag, L.N, kCountSlot, kFsObstacles, kFsClusters);
+    hipLaunchKernelGGL(k_component_label, dim3(nb, B), dim3(256), 0, st, d, L);
+    launch_scan(c, B, d.cl_size, d.cl_off, L.N, kCountSlot, kFsClusters, -1);
+    hipLaunchKernelGGL(k_cluster_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
+}
+
+}  // namespace lp

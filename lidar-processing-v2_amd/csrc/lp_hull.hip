@@ -1,0 +1,445 @@
+// lp_hull.hip -- per-cluster monotone-chain convex hulls on gfx950.
+//
+// Reference: lidar_processing_lib/src/polygonizer.cpp:33-91 (Andrew's
+// monotone chain over an (x, y)-sorted index list, popping while the
+// double-precision cross product is <= 0, result resized to k-1) fed by the
+// per-label gather of src/processor/src/processor.cpp:627-663 (cluster points
+// as PointXY doubles in obstacle-index order; hull indices are positions in
+// that list).
+//
+// With sign-exact orientation tests the chain returns the strictly convex
+// hull vertices, counter-clockwise from the lexicographically smallest
+// point (and [min, max] for collinear input).  That set is unchanged when
+// points that are not vertices of a subset's hull are dropped, so hulls are
+// built in rounds: every 256-point chunk of a cluster is sorted and chained
+// by one wavefront, the chunk hull vertices become the next round's input,
+// and a cluster whose input fits one chunk gets its final hull.  The
+// orientation test evaluates the reference's expression in double and, when
+// Shewchuk's error bound cannot certify its sign, falls back to an exact
+// expansion sum (TwoProduct via FMA).
+#include <algorithm>
+
+#include "lp_kernels.h"
+
+namespace lp {
+
+using namespace lpd;
+
+// ---- exact orientation ---------------------------------------------------
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_diff(double a, double b, double& s, double& e) {
+    s = a - b;
+    const double bb = a - s;
+    e = (a - (s + bb)) + (bb - b);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double& p, double& e) {
+    p = a * b;
+    e = fma(a, b, -p);
+}
+
+// Sign of (bx-ax)*(cy-ay) - (by-ay)*(cx-ax), exact.  *naive_sign receives
+// the sign of the expression evaluated in double the way the reference does.
+__device__ int orient_exact(double ax, double ay, double bx, double by, double cx, double cy, int* naive_sign) {
+    const double t1 = (bx - ax) * (cy - ay);
+    const double t2 = (by - ay) * (cx - ax);
+    const double det = t1 - t2;
+    const int ns = (det > 0.0) - (det < 0.0);
+    *naive_sign = ns;
+    const double bound = 3.3306690738754716e-16 * (fabs(t1) + fabs(t2));
+    if (det > bound || -det > bound) return ns;
+    double d[4][2];
+    two_diff(bx, ax, d[0][0], d[0][1]);  // bx - ax
+    two_diff(cy, ay, d[1][0], d[1][1]);  // cy - ay
+    two_diff(by, ay, d[2][0], d[2][1]);  // by - ay
+    two_diff(cx, ax, d[3][0], d[3][1]);  // cx - ax
+    if (d[0][1] == 0.0 && d[1][1] == 0.0 && d[2][1] == 0.0 && d[3][1] == 0.0) {
+        // Exact differences: det = (p1 + e1) - (p2 + e2) exactly, with the
+        // product errors from FMA.  If both products are exact, the rounded
+        // difference already has the exact sign (duplicates land here).
+        double p1, e1, p2, e2;
+        two_prod(d[0][0], d[1][0], p1, e1);
+        two_prod(d[2][0], d[3][0], p2, e2);
+        if (e1 == 0.0 && e2 == 0.0) return ns;
+        // Four-term exact sum: grow-expansion over {p1, e1, -p2, -e2}.
+        double t[4] = {e1, -e2, p1, -p2};
+        double ex[4];
+        int ne = 0;
+        for (int k = 0; k < 4; ++k) {
+            double q = t[k];
+            for (int i = 0; i < ne; ++i) {
+                double s, h;
+                two_sum(q, ex[i], s, h);
+                ex[i] = h;
+                q = s;
+            }
+            ex[ne++] = q;
+        }
+        for (int i = ne - 1; i >= 0; --i) {
+            if (ex[i] > 0.0) return 1;
+            if (ex[i] < 0.0) return -1;
+        }
+        return 0;
+    }
+    double terms[16];
+    int nt = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            double p, e;
+            two_prod(d[0][i], d[1][j], p, e);
+            terms[nt++] = p;
+            terms[nt++] = e;
+            two_prod(d[2][i], d[3][j], p, e);
+            terms[nt++] = -p;
+            terms[nt++] = -e;
+        }
+    // Grow-expansion (Shewchuk 1997): exact nonoverlapping sum.
+    double ex[17];
+    int ne = 0;
+    for (int t = 0; t < nt; ++t) {
+        double q = terms[t];
+        for (int i = 0; i < ne; ++i) {
+            double s, h;
+            two_sum(q, ex[i], s, h);
+            ex[i] = h;
+            q = s;
+        }
+        ex[ne++] = q;
+    }
+    for (int i = ne - 1; i >= 0; --i) {
+        if (ex[i] > 0.0) return 1;
+        if (ex[i] < 0.0) return -1;
+    }
+    return 0;
+}
+
+// ---- chunk hull ----------------------------------------------------------
+// (x, y, idx) lexicographic order; idx breaks exact-duplicate ties.
+__device__ __forceinline__ bool pt_less(const double* X, const double* Y, const int32_t* I, int a, int b) {
+    if (X[a] != X[b]) return X[a] < X[b];
+    if (Y[a] != Y[b]) return Y[a] < Y[b];
+    return I[a] < I[b];
+}
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kHullRounds = 3;
+constexpr int kHullBig = 4096;  // candidates one workgroup finishes after the rounds
+
+// Andrew's monotone chain over the n points P[0..n) (sorted by (x, y, idx))
+// with sign-exact orientation (polygonizer.cpp:66-90).  Writes the hull to
+// S and returns its length; *inexact counts tests where the reference's
+// double expression has a different sign than the exact one.
+__device__ int chain_hull(const double* X, const double* Y, const int32_t* I, const uint16_t* P, int n, uint16_t* S,
+                          uint32_t* inexact) {
+    int kk = 0;
+    if (n < 3) {
+        // polygonizer.cpp:36-41: fewer than three points are returned as
+        // they are (in obstacle-index order, the reference's list order).
+        for (int i = 0; i < n; ++i) S[kk++] = static_cast<uint16_t>(i);
+        for (int a = 1; a < kk; ++a)
+            for (int b = a; b > 0 && I[S[b - 1]] > I[S[b]]; --b) {
+                const uint16_t tmp = S[b];
+                S[b] = S[b - 1];
+                S[b - 1] = tmp;
+            }
+        return kk;
+    }
+    int ns;
+    for (int i = 0; i < n; ++i) {
+        const int p = P[i];
+        while (kk > 1) {
+            const int o = orient_exact(X[S[kk - 2]], Y[S[kk - 2]], X[S[kk - 1]], Y[S[kk - 1]], X[p], Y[p], &ns);
+            *inexact += (o != ns) ? 1u : 0u;
+            if (o > 0) break;
+            --kk;
+        }
+        S[kk++] = static_cast<uint16_t>(p);
+    }
+    const int tt = kk + 1;
+    for (int i = n - 2; i >= 0; --i) {
+        const int p = P[i];
+        while (kk >= tt) {
+            const int o = orient_exact(X[S[kk - 2]], Y[S[kk - 2]], X[S[kk - 1]], Y[S[kk - 1]], X[p], Y[p], &ns);
+            *inexact += (o != ns) ? 1u : 0u;
+            if (o > 0) break;
+            --kk;
+        }
+        S[kk++] = static_cast<uint16_t>(p);
+    }
+    return kk - 1;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int upper_bound_u32(const uint32_t* a, int n, uint32_t v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Chunk counts for round r: a cluster that is not final yet needs
+// ceil(cnt / 256) chunks (one if it fits: that chunk's hull is final).
+__global__ void k_hull_chunks(DevBuf d, Layout L, int r) {
+    const int f = blockIdx.y;
+    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const size_t ob = (size_t)f * L.N;
+    const size_t BN = (size_t)L.B * L.N;
+    uint32_t nch = 0;
+    if (r == 0) d.hull_count[ob + c] = 0;
+    if (r == 0 || d.hull_count[ob + c] == 0) {
+        const uint32_t cnt = r == 0 ? d.cl_size[ob + c] : d.hc_count[(r & 1) * BN + ob + c];
+        nch = cnt <= static_cast<uint32_t>(kHullChunk) ? 1u : (cnt + kHullChunk - 1) / kHullChunk;
+    }
+    d.hc_chunks[ob + c] = nch;
+    d.hc_count[((r + 1) & 1) * BN + ob + c] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
+    __shared__ double sx[kWavesPerBlock][kHullChunk];
+    __shared__ double sy[kWavesPerBlock][kHullChunk];
+    __shared__ int32_t si[kWavesPerBlock][kHullChunk];
+    __shared__ uint16_t perm[kWavesPerBlock][kHullChunk];
+    __shared__ uint16_t stk[kWavesPerBlock][2 * kHullChunk];
+    __shared__ int nout[kWavesPerBlock];
+    const int f = blockIdx.y;
+    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
+    const uint32_t total = d.fs[f * kFsCount + kFsHullChunks];
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const size_t ob = (size_t)f * L.N;
+    const size_t BN = (size_t)L.B * L.N;
+    const uint32_t* choff = d.hc_choff + ob;
+    double* X = sx[w];
+    double* Y = sy[w];
+    int32_t* I = si[w];
+    uint16_t* P = perm[w];
+    uint16_t* S = stk[w];
+    const size_t in_b = (size_t)(r & 1) * BN;
+    const size_t out_b = (size_t)((r + 1) & 1) * BN;
+    for (uint32_t t = blockIdx.x * kWavesPerBlock + w; t < total; t += gridDim.x * kWavesPerBlock) {
+        const int c = upper_bound_u32(choff, K, t) - 1;
+        const uint32_t ci = t - choff[c];
+        const uint32_t cnt = r == 0 ? d.cl_size[ob + c] : d.hc_count[in_b + ob + c];
+        const uint32_t base = d.cl_off[ob + c];
+        const bool final_chunk = cnt <= static_cast<uint32_t>(kHullChunk);
+        const uint32_t lo = ci * kHullChunk;
+        const int n = static_cast<int>(min(cnt - lo, static_cast<uint32_t>(kHullChunk)));
+        int m = 1;
+        while (m < n) m <<= 1;
+        for (int i = lane; i < m; i += kWave) {
+            if (i < n) {
+                const size_t src = in_b + ob + base + lo + i;
+                X[i] = d.hcx[src];
+                Y[i] = d.hcy[src];
+                I[i] = d.hci[src];
+            }
+            P[i] = static_cast<uint16_t>(i);
+        }
+        wave_sync();
+        // Bitonic sort of the permutation; padding entries (>= n) sort last.
+        for (int k = 2; k <= m; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int q = lane; q < (m >> 1); q += kWave) {
+                    const int a = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                    const int b = a + j;
+                    const int pa = P[a], pb = P[b];
+                    const bool gt = (pa >= n) ? (pb < n || pa > pb) : (pb < n && pt_less(X, Y, I, pb, pa));
+                    const bool up = (a & k) == 0;
+                    if (gt == up) {
+                        P[a] = static_cast<uint16_t>(pb);
+                        P[b] = static_cast<uint16_t>(pa);
+                    }
+                }
+                wave_sync();
+            }
+        }
+        if (lane == 0) {
+            uint32_t inexact = 0;
+            nout[w] = chain_hull(X, Y, I, P, n, S, &inexact);
+            if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+        }
+        wave_sync();
+        const int kk = nout[w];
+        if (final_chunk) {
+            for (int i = lane; i < kk; i += kWave) d.hull_idx[ob + base + i] = I[S[i]];
+            if (lane == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
+        } else {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&d.hc_count[out_b + ob + c], static_cast<uint32_t>(kk));
+            pos = __shfl(pos, 0, kWave);
+            for (int i = lane; i < kk; i += kWave) {
+                const size_t dst = out_b + ob + base + pos + i;
+                d.hcx[dst] = X[S[i]];
+                d.hcy[dst] = Y[S[i]];
+                d.hci[dst] = I[S[i]];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Clusters whose candidates still exceed one chunk after the rounds (hulls
+// with hundreds of vertices): one workgroup sorts up to kHullBig candidates
+// in LDS and chains them.
+__global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in) {
+    __shared__ double X[kHullBig];
+    __shared__ double Y[kHullBig];
+    __shared__ int32_t I[kHullBig];
+    __shared__ uint16_t P[kHullBig];
+    __shared__ uint16_t S[2 * kHullBig];
+    __shared__ int nout;
+    const int f = blockIdx.y;
+    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
+    const int tid = threadIdx.x;
+    const size_t ob = (size_t)f * L.N;
+    const size_t BN = (size_t)L.B * L.N;
+    const size_t in_b = (size_t)(r_in & 1) * BN;
+    for (int c = blockIdx.x; c < K; c += gridDim.x) {
+        if (d.hull_count[ob + c] != 0) continue;
+        const uint32_t cnt = d.hc_count[in_b + ob + c];
+        if (cnt > static_cast<uint32_t>(kHullBig)) {
+            if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
+            continue;
+        }
+        const int n = static_cast<int>(cnt);
+        const uint32_t base = d.cl_off[ob + c];
+        int m = 1;
+        while (m < n) m <<= 1;
+        for (int i = tid; i < m; i += 1024) {
+            if (i < n) {
+                const size_t src = in_b + ob + base + i;
+                X[i] = d.hcx[src];
+                Y[i] = d.hcy[src];
+                I[i] = d.hci[src];
+            }
+            P[i] = static_cast<uint16_t>(i);
+        }
+        __syncthreads();
+        for (int k = 2; k <= m; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int q = tid; q < (m >> 1); q += 1024) {
+                    const int a = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                    const int b = a + j;
+                    const int pa = P[a], pb = P[b];
+                    const bool gt = (pa >= n) ? (pb < n || pa > pb) : (pb < n && pt_less(X, Y, I, pb, pa));
+                    if (gt == ((a & k) == 0)) {
+                        P[a] = static_cast<uint16_t>(pb);
+                        P[b] = static_cast<uint16_t>(pa);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            uint32_t inexact = 0;
+            nout = chain_hull(X, Y, I, P, n, S, &inexact);
+            if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+        }
+        __syncthreads();
+        const int kk = nout;
+        for (int i = tid; i < kk; i += 1024) d.hull_idx[ob + base + i] = I[S[i]];
+        if (tid == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
+        __syncthreads();
+    }
+}
+
+__global__ void k_hull_check(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    if (d.hull_count[(size_t)f * L.N + c] == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
+}
+
+// Hull vertices as cluster-local indices: rank of the obstacle index among
+// the cluster's members (the gather order of processor.cpp:640-658).  One
+// wavefront per output vertex.
+__global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
+    const uint32_t total = d.fs[f * kFsCount + kFsHullTotal];
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const size_t ob = (size_t)f * L.N;
+    for (uint32_t v = blockIdx.x * kWavesPerBlock + w; v < total; v += gridDim.x * kWavesPerBlock) {
+        const int c = upper_bound_u32(d.hull_off + ob, K, v) - 1;
+        const uint32_t i = v - d.hull_off[ob + c];
+        const uint32_t base = d.cl_off[ob + c];
+        const uint32_t size = d.cl_size[ob + c];
+        const int32_t idx = d.hull_idx[ob + base + i];
+        uint32_t cnt = 0;
+        for (uint32_t q = lane; q < size; q += kWave) cnt += d.cl_members[ob + base + q] < idx ? 1u : 0u;
+        cnt = wave_sum(cnt);
+        if (lane == 0) d.hull_local[ob + v] = static_cast<int32_t>(cnt);
+    }
+}
+
+// Standalone lists: per point, its list and list-local index; per list, the
+// offset and size (cl_off / cl_size) and identity members.
+__global__ void k_hull_lists_init(DevBuf d, Layout L, const int32_t* offsets, int K, int T) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        d.fs[kFsClusters] = static_cast<uint32_t>(K);
+        d.fs[kFsErr] = 0;
+        d.fs[kFsHullInexact] = 0;
+    }
+    if (t < K) {
+        d.cl_off[t] = static_cast<uint32_t>(offsets[t]);
+        d.cl_size[t] = static_cast<uint32_t>(offsets[t + 1] - offsets[t]);
+    }
+    if (t < T) {
+        int lo = 0, hi = K;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (offsets[mid] <= t)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const int local = t - offsets[lo - 1];
+        d.hci[t] = local;
+        d.cl_members[t] = local;
+    }
+}
+
+void launch_hull_lists(lp_context* c, int num_lists, int total_points) {
+    const int n = std::max(num_lists, total_points);
+    const int32_t* offs = reinterpret_cast<const int32_t*>(c->d.hc_choff);  // staged by the caller
+    hipLaunchKernelGGL(k_hull_lists_init, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d, c->L, offs,
+                       num_lists, total_points);
+    launch_hull(c, 1);
+}
+
+void launch_hull(lp_context* c, int B) {
+    DevBuf& d = c->d;
+    const Layout& L = c->L;
+    hipStream_t st = c->stream;
+    const int nb = (L.N + 255) / 256;
+    stage_mark(c, "hull");
+    for (int r = 0; r < kHullRounds; ++r) {
+        hipLaunchKernelGGL(k_hull_chunks, dim3(nb, B), dim3(256), 0, st, d, L, r);
+        launch_scan(c, B, d.hc_chunks, d.hc_choff, L.N, kCountSlot, kFsClusters, kFsHullChunks);
+        hipLaunchKernelGGL(k_hull_round, dim3(512, B), dim3(256), 0, st, d, L, r);
+    }
+    hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
+    hipLaunchKernelGGL(k_hull_check, dim3(nb, B), dim3(256), 0, st, d, L);
+    launch_scan(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
+    hipLaunchKernelGGL(k_hull_local, dim3(512, B), dim3(256), 0, st, d, L);
+}
+
+}  // namespace lp

@@ -1,0 +1,211 @@
+// lp_internal.h -- shared definitions of the HIP implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "lp_amd.h"
+
+namespace lp {
+
+// Pixel / point label codes (segmenter.hpp:69-74); PENDING is the JCP
+// "intersection" colour of a queue pixel whose weights never resolved.
+enum : uint8_t { kEmpty = 0, kGround = 1, kObstacle = 2, kPending = 3 };
+
+constexpr int kRansacIters = 60;      // segmenter.cpp:324
+constexpr int kRansacRadialBins = 4;  // segmenter.cpp:323
+constexpr int kMtRawCount = 4096;     // precomputed mt19937{42} outputs
+constexpr int kSmallCell = 512;       // cells up to this size are sorted by one wave
+constexpr int kBigCell = 8192;        // larger cells: one workgroup; beyond: capacity error
+constexpr int kHullChunk = 256;       // points per chunk-hull wave
+constexpr int kHullFinal = 8192;      // candidates a final hull workgroup can hold
+
+// Per-frame error bits (device), mapped to status codes on fetch.
+enum : uint32_t {
+    kErrCellCapacity = 1u << 0,
+    kErrRansacSingle = 1u << 1,
+    kErrRngExhausted = 1u << 2,
+    kErrQueueCapacity = 1u << 3,
+    kErrVoxelOverflow = 1u << 4,
+    kErrHullCapacity = 1u << 5,
+};
+
+// Segmenter constants derived on the host exactly as the reference derives
+// them (segmenter.cpp:42-46, 106-109, 121-122, 209-211, 359-360, 532-533).
+struct SegConst {
+    int32_t H, W;
+    int32_t rings, slices, cells;
+    int32_t organized;        // use the ring field as the image row
+    float slice_res;          // rad
+    float zlo, zhi;           // -h + z_min, -h + z_max
+    float spacing, min_d, max_d;
+    float down_rad, per_pixel;
+    float w_minus_1;          // (float)(W - 1)
+    float thr;                // ground_height_threshold_m
+    float two_thr;            // 2.0f * thr
+    float dz;                 // min(spacing * tanf(slope), thr - FLT_EPSILON)
+    float e0;                 // -h + thr
+    float neg_h;              // -sensor_height_m
+    float cos_max;            // cosf(tanf(slope))
+    float kthr2;              // kernel_threshold^2
+    float neg_amp;            // -amplification_factor
+};
+
+// Clusterer constants (clusterer.hpp:178-185).
+struct CluConst {
+    float res_r, res_az, res_el;
+    uint32_t min_size;
+    int32_t max_voxels;
+};
+
+// Per-frame descriptor of a batch (device copy).
+struct FrameDesc {
+    int32_t pt_off;   // CSR offset of the frame's points in the packed inputs
+    int32_t n;        // points
+    int32_t pix_off;  // CSR offset of the frame's pixels
+    int32_t has_ring;
+    SegConst sc;
+};
+
+// Device pointers of one context (all sized for the capacity).
+struct DevBuf {
+    // packed inputs (CSR by FrameDesc::pt_off)
+    float *x, *y, *z;
+    uint16_t* ring;
+    // per point
+    int32_t* pt_cell;
+    int32_t* pt_pix;
+    uint32_t* labels;
+    uint32_t* obs_flag;      // scan scratch over points
+    int32_t* obs_src;        // [frame stride N] input index of obstacle k
+    float *ox, *oy, *oz;     // obstacle xyz (frame stride N)
+    float *sr, *sa, *se;     // spherical coords of obstacle points
+    int32_t* vslot;          // hash slot of each obstacle point
+    uint32_t* cl_flag;       // first-point flags / scan
+    int32_t* clabel;         // cluster label per obstacle point
+    // per cell (frame stride cell_stride)
+    uint32_t* cell_count;
+    uint32_t* cell_start;    // stride cell_stride + 1 used
+    uint32_t* cell_cursor;
+    int32_t* cell_items;     // frame stride N
+    float* zmin;
+    float* elev;
+    uint32_t* big_cells;     // per frame list of cells needing a workgroup sort
+    // RANSAC (per frame)
+    uint32_t* cand_count;    // frame stride cand_stride
+    uint32_t* cand_start;
+    float *cx, *cy, *cz;     // candidates, frame stride N
+    float* hyp;              // [B][60][4]
+    uint32_t* hyp_valid;     // [B][60]
+    uint32_t* hyp_count;     // [B][60]
+    uint32_t* mt_raw;        // [kMtRawCount]
+    // per pixel (CSR by pix_off)
+    unsigned long long* pix_key;
+    uint8_t* state;
+    int32_t* mapping;
+    float4* pxyz;
+    uint32_t* qflag;
+    int32_t* qmap;
+    // JCP queue (frame stride pix_stride)
+    int32_t* qlist;
+    float* slot_w;           // [q][24]
+    int32_t* slot_src;       // [q][24]
+    uint8_t* qsum_ok;
+    uint8_t* fin;
+    // voxel hash (frame stride hash_cap)
+    int32_t* hkey;
+    int32_t* hmin;           // minimum obstacle index of the voxel's points
+    int32_t* hdense;         // slot -> dense voxel id
+    uint32_t* hcount;        // points in the voxel
+    // dense voxels (frame stride N)
+    int32_t* vlist;          // dense id -> hash slot (insertion order)
+    int32_t* vpar;           // union-find parent / component root (dense ids)
+    int32_t* vmin;           // per root: minimum obstacle index of the component
+    uint32_t* vcnt;          // per root: points in the component
+    // clusters (frame stride N)
+    int32_t* cl_root;
+    uint32_t* cl_size;
+    uint32_t* cl_off;
+    uint32_t* cl_cursor;
+    int32_t* cl_members;
+    // hull
+    double* hcx;             // candidate coordinates, frame stride 2N (two buffers)
+    double* hcy;
+    int32_t* hci;            // candidate obstacle index
+    uint32_t* hc_count;      // per cluster candidates (two buffers)
+    uint32_t* hc_chunks;     // per cluster chunks of the current round
+    uint32_t* hc_choff;      // their exclusive scan
+    int32_t* hull_idx;       // final hull (obstacle indices), frame stride N
+    uint32_t* hull_count;    // per cluster
+    uint32_t* hull_off;      // per cluster (+1)
+    int32_t* hull_local;     // cluster-local indices, frame stride N
+    // generic scan scratch: block sums
+    uint32_t* blk;
+    // per-frame scalars
+    uint32_t* fs;            // [B][kFsCount]
+    FrameDesc* frames;       // [B]
+    CluConst* cc;            // [1]
+};
+
+// Per-frame scalar slots in DevBuf::fs.
+enum FsSlot : int {
+    kFsCands = 0,      // RANSAC candidates
+    kFsApplied,        // RANSAC plane applied
+    kFsBest,           // best inlier count
+    kFsPlaneA, kFsPlaneB, kFsPlaneC, kFsPlaneD,
+    kFsNumBig,         // big cells
+    kFsQueue,          // JCP queue length
+    kFsObstacles,
+    kFsMaxR, kFsMaxA, kFsMaxE,
+    kFsVoxels,
+    kFsClusters,
+    kFsHullTotal,
+    kFsHullChunks,
+    kFsHullInexact,    // predicates whose double evaluation had the wrong sign
+    kFsCcGlobal,       // 1: too many voxels for the LDS union-find
+    kFsErr,
+    kFsCount
+};
+
+struct Layout {
+    int32_t B, N, P, H, W;  // capacity: frames, points/frame, pixels/frame
+    int32_t B_active;       // frames in the current batch
+    int32_t idx_bits;       // bits of the point index in a pixel winner key
+    int32_t cell_stride;    // cells per frame (capacity)
+    int32_t cand_stride;    // slices*4 capacity
+    int32_t hash_cap;       // power of two
+    int32_t blk_stride;     // scan block sums per frame
+};
+
+}  // namespace lp
+
+struct lp_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    lp_capacity cap{};
+    lp::Layout L{};
+    lp::DevBuf d{};
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    // host staging
+    std::vector<lp::FrameDesc> frames;
+    int32_t num_frames = 0;
+    lp::CluConst cc{};
+    int32_t staged_points = 0;
+    // pinned host buffers for inputs / outputs
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    // graph per batch shape
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int32_t graph_frames = -1;
+    // stage timing
+    std::vector<hipEvent_t> events;
+    std::vector<std::string> stage_names;
+    bool profiling = false;
+    std::vector<float> stage_ms_acc;
+    int32_t profiled_runs = 0;
+};

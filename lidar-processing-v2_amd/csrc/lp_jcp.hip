@@ -1,0 +1,384 @@
+// lp_jcp.hip -- Jump-Convolution-Process on the range image, exact.
+//
+// Reference: lidar_processing_lib/src/segmenter.cpp:481-638 (JCP) and
+// :640-669 (populateLabels).  The reference dilates the obstacle mask with a
+// 5x5 rectangle (out-of-image pixels never contribute), queues every ground
+// pixel next to an obstacle in raster order, and then visits the queue
+// sequentially.  Each visit reads the CURRENT label of its 24 neighbours and
+// writes 24 weight / label slots that are member arrays reset only once per
+// frame: a slot whose neighbour falls outside the image is skipped without
+// being rewritten, so it keeps the value of the last earlier queue pixel for
+// which that slot was inside the image ("stale" slots).
+//
+// Parallel form used here (exactly equivalent):
+//   * every dependency of a queue pixel p is on raster-earlier queue pixels;
+//   * all of them lie in earlier image rows, except the two same-row
+//     neighbours (0,-1) and (0,-2) -- a stale slot's writer and the writer's
+//     neighbour are always in earlier rows or static;
+//   * so rows are solved in order, and inside a row the state of pixel j is
+//     T_j(state(j-2), state(j-1)) over 3 states {ground, obstacle, pending}:
+//     an associative scan of 9-entry transfer maps solves the row at once.
+// The weights w_i = u_i / sum do not depend on labels and are precomputed
+// for all queue pixels in parallel (k_jcp_slots); the per-row solve only
+// re-runs the ordered float sums that the reference performs.
+#include "lp_kernels.h"
+
+namespace lp {
+
+using namespace lpd;
+
+__constant__ static const int8_t kOffR[24] = {-2, -2, -2, -2, -2, -1, -1, -1, -1, -1, 0, 0,
+                                              0,  0,  1,  1,  1,  1,  1,  2,  2,  2,  2, 2};
+__constant__ static const int8_t kOffC[24] = {-2, -1, 0, 1, 2, -2, -1, 0, 1, 2, -2, -1,
+                                              1,  2,  -2, -1, 0, 1, 2, -2, -1, 0, 1, 2};
+
+// Slot label sources: >= 0 a queue index (dynamic), else a static label.
+constexpr int32_t kSrcUnknown = -1;
+constexpr int32_t kSrcGround = -2;
+constexpr int32_t kSrcObstacle = -3;
+
+// Internal solve states.
+constexpr uint32_t kSG = 0, kSO = 1, kSU = 2;
+
+// Queue membership: a ground pixel with an obstacle pixel in its clipped 5x5
+// window (cv::dilate of the red channel, then the raster scan of
+// segmenter.cpp:491-514).
+__global__ void k_jcp_mark(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int H = fd.sc.H, W = fd.sc.W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= H * W) return;
+    const uint8_t* st = d.state + fd.pix_off;
+    uint32_t q = 0;
+    if (st[p] == kGround) {
+        const int r = p / W, c = p % W;
+        for (int dr = -2; dr <= 2 && !q; ++dr) {
+            const int rr = r + dr;
+            if (rr < 0 || rr >= H) continue;
+            for (int dc = -2; dc <= 2; ++dc) {
+                const int cc = c + dc;
+                if (cc < 0 || cc >= W) continue;
+                if (st[rr * W + cc] == kObstacle) {
+                    q = 1;
+                    break;
+                }
+            }
+        }
+    }
+    d.qflag[(size_t)f * L.P + p] = q;
+}
+
+// qmap holds the exclusive scan of qflag on entry.
+__global__ void k_jcp_fill(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int P = fd.sc.H * fd.sc.W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const size_t o = (size_t)f * L.P + p;
+    const int pos = d.qmap[o];
+    if (d.qflag[o]) {
+        d.qlist[(size_t)f * L.P + pos] = p;
+        d.qmap[o] = pos;
+    } else {
+        d.qmap[o] = -1;
+    }
+}
+
+__device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+struct SlotVal {
+    float u;
+    int32_t src;
+    bool near;
+};
+
+// Slot value computed by evaluator pixel `pe` (core point `core`) for its
+// neighbour pixel np (segmenter.cpp:558-606).
+__device__ __forceinline__ SlotVal eval_slot(const DevBuf& d, size_t po, size_t qo, const SegConst& s, int pe,
+                                             float4 core, int np) {
+    SlotVal v{0.0f, kSrcUnknown, false};
+    if (d.mapping[po + np] == -1) return v;
+    const float4 nb = d.pxyz[po + np];
+    const float dx = core.x - nb.x;
+    const float dy = core.y - nb.y;
+    const float dz = core.z - nb.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > s.kthr2) return v;
+    v.u = glibc_expf(s.neg_amp * sqrtf(d2));
+    v.near = true;
+    const int qm = d.qmap[qo + np];
+    if (qm >= 0) {
+        v.src = (np < pe) ? qm : kSrcUnknown;  // later queue pixels are still pending
+    } else {
+        const uint8_t st = d.state[po + np];
+        v.src = st == kGround ? kSrcGround : (st == kObstacle ? kSrcObstacle : kSrcUnknown);
+    }
+    return v;
+}
+
+// Per queue pixel: the 24 slot weights w_i = u_i / sum and label sources,
+// including stale slots resolved through their writer.
+__global__ void k_jcp_slots(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const int nq = static_cast<int>(d.fs[f * kFsCount + kFsQueue]);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nq) return;
+    const int H = s.H, W = s.W;
+    const size_t po = fd.pix_off;
+    const size_t qo = (size_t)f * L.P;
+    const int32_t* ql = d.qlist + qo;
+    const int p = ql[j];
+    const int r = p / W, c = p % W;
+    const float4 core = d.pxyz[po + p];
+    float u[24];
+    int32_t src[24];
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        const int dr = kOffR[i], dc = kOffC[i];
+        const int nr = r + dr, nc = c + dc;
+        if (nr >= 0 && nr < H && nc >= 0 && nc < W) {
+            const SlotVal v = eval_slot(d, po, qo, s, p, core, nr * W + nc);
+            u[i] = v.u;
+            src[i] = v.src;
+            if (v.near) sum += v.u;
+        } else {
+            // Stale slot: find the last earlier queue pixel for which this
+            // slot was inside the image.
+            const int rlo = max(0, -dr), rhi = min(H, H - dr);
+            const int clo = max(0, -dc), chi = min(W, W - dc);
+            int khi = j - 1;
+            if (rhi < H) khi = min(khi, lower_bound_i32(ql, nq, rhi * W) - 1);
+            const int klo = rlo > 0 ? lower_bound_i32(ql, nq, rlo * W) : 0;
+            int k = khi;
+            for (; k >= klo; --k) {
+                const int qc = ql[k] % W;
+                if (qc >= clo && qc < chi) break;
+            }
+            if (k >= klo) {
+                const int q = ql[k];
+                const int qr = q / W, qc = q % W;
+                const SlotVal v = eval_slot(d, po, qo, s, q, d.pxyz[po + q], (qr + dr) * W + (qc + dc));
+                u[i] = v.u;
+                src[i] = v.src;
+            } else {
+                u[i] = 0.0f;  // slots start at zero / UNKNOWN (segmenter.cpp:523-525)
+                src[i] = kSrcUnknown;
+            }
+        }
+    }
+    const bool ok = fabsf(sum) > 1.1920928955078125e-07f;  // FLT_EPSILON
+    float* wo = d.slot_w + ((size_t)f * L.P + j) * 24;
+    int32_t* so = d.slot_src + ((size_t)f * L.P + j) * 24;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        wo[i] = ok ? u[i] / sum : 0.0f;
+        so[i] = src[i];
+    }
+    d.qsum_ok[(size_t)f * L.P + j] = ok ? 1 : 0;
+}
+
+// 9-entry transfer map over pair states ps = a*3 + b, 4 bits per entry.
+__device__ __forceinline__ uint64_t map_compose(uint64_t fm, uint64_t gm) {  // f after g
+    uint64_t r = 0;
+#pragma unroll
+    for (int ps = 0; ps < 9; ++ps) {
+        const uint32_t gi = static_cast<uint32_t>(gm >> (4 * ps)) & 0xf;
+        const uint32_t fi = static_cast<uint32_t>(fm >> (4 * gi)) & 0xf;
+        r |= static_cast<uint64_t>(fi) << (4 * ps);
+    }
+    return r;
+}
+
+constexpr uint64_t kIdentityMap = 0x876543210ull;
+constexpr int kSolveThreads = 256;
+constexpr int kSolveWaves = kSolveThreads / kWave;
+constexpr int kFinLds = 65536;
+
+__device__ __forceinline__ void add_slot(uint32_t l, float w, float& wg, float& wo) {
+    if (l == kSG)
+        wg += w;
+    else if (l == kSO)
+        wo += w;
+}
+
+// Transfer map of queue pixel j (pair state (state(j-2), state(j-1)) ->
+// (state(j-1), state(j))).  Labels of earlier rows come from `fin`.
+__device__ __forceinline__ uint64_t pixel_map(const DevBuf& d, size_t qo, int j, int rs, const uint8_t* fin) {
+    const float4* w4 = reinterpret_cast<const float4*>(d.slot_w + (qo + j) * 24);
+    const int4* s4 = reinterpret_cast<const int4*>(d.slot_src + (qo + j) * 24);
+    const bool ok = d.qsum_ok[qo + j] != 0;
+    float wv[24];
+    int32_t src[24];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const float4 a = w4[q];
+        const int4 b = s4[q];
+        wv[4 * q] = a.x;
+        wv[4 * q + 1] = a.y;
+        wv[4 * q + 2] = a.z;
+        wv[4 * q + 3] = a.w;
+        src[4 * q] = b.x;
+        src[4 * q + 1] = b.y;
+        src[4 * q + 2] = b.z;
+        src[4 * q + 3] = b.w;
+    }
+    // Static and earlier-row labels.  A source in this row can only be the
+    // queue pixel before (j-1) or two before (j-2), through slots (0,-2) and
+    // (0,-1) (indices 10, 11): stale slots always read earlier rows.
+    uint32_t lab[24];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        const int32_t sv = src[i];
+        uint32_t l = kSU;
+        if (sv == kSrcGround)
+            l = kSG;
+        else if (sv == kSrcObstacle)
+            l = kSO;
+        else if (sv >= 0 && sv < rs)
+            l = fin[sv];
+        lab[i] = l;
+    }
+    const int sel10 = src[10] == j - 2 ? 1 : (src[10] == j - 1 ? 2 : 0);  // 1: state(j-2), 2: state(j-1)
+    const bool sel11 = src[11] == j - 1;
+    if (!ok) {
+        uint64_t m = 0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m |= static_cast<uint64_t>((q % 3) * 3 + kSU) << (4 * q);
+        return m;
+    }
+    // Ordered sums of segmenter.cpp:613-626: slots 0..9 are fixed, 10 and 11
+    // may depend on the row's previous states, 12..23 are fixed again.
+    float wg0 = 0.0f, wo0 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) add_slot(lab[i], wv[i], wg0, wo0);
+    uint64_t m = 0;
+    const int combos = (sel10 || sel11) ? 9 : 1;
+#pragma unroll 1
+    for (int ps = 0; ps < combos; ++ps) {
+        const uint32_t a = ps / 3, b = ps % 3;
+        float wg = wg0, wo = wo0;
+        add_slot(sel10 == 1 ? a : (sel10 == 2 ? b : lab[10]), wv[10], wg, wo);
+        add_slot(sel11 ? b : lab[11], wv[11], wg, wo);
+#pragma unroll
+        for (int i = 12; i < 24; ++i) add_slot(lab[i], wv[i], wg, wo);
+        const uint32_t t = (wo > wg) ? kSO : kSG;
+        if (combos == 1) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) m |= static_cast<uint64_t>((q % 3) * 3 + t) << (4 * q);
+        } else {
+            m |= static_cast<uint64_t>(b * 3 + t) << (4 * ps);
+        }
+    }
+    return m;
+}
+
+// Row-ordered exact solve, one workgroup per frame: per row, transfer maps
+// of its queue pixels, a wavefront shuffle scan of the maps, one LDS step
+// across the four wavefronts.
+__global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L) {
+    __shared__ uint8_t fin[kFinLds];
+    __shared__ int row_start[1025];
+    __shared__ uint64_t wtot[kSolveWaves];
+    const int f = blockIdx.x;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const int nq = static_cast<int>(d.fs[f * kFsCount + kFsQueue]);
+    const int tid = threadIdx.x;
+    const int w = tid / kWave;
+    const int lane = lane_id();
+    if (nq > kFinLds || s.H > 1024) {
+        if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrQueueCapacity);
+        return;
+    }
+    const size_t qo = (size_t)f * L.P;
+    const int32_t* ql = d.qlist + qo;
+    for (int r = tid; r <= s.H; r += kSolveThreads) row_start[r] = lower_bound_i32(ql, nq, r * s.W);
+    __syncthreads();
+    for (int r = 0; r < s.H; ++r) {
+        const int rs = row_start[r], re = row_start[r + 1];
+        if (rs == re) continue;
+        uint32_t carry = kSU * 3 + kSU;
+        for (int base = rs; base < re; base += kSolveThreads) {
+            const int j = base + tid;
+            uint64_t m = j < re ? pixel_map(d, qo, j, rs, fin) : kIdentityMap;
+            // Inclusive scan inside the wavefront: m := m_j o ... o m_first.
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint64_t prev = __shfl_up(m, o, kWave);
+                if (lane >= o) m = map_compose(m, prev);
+            }
+            if (lane == kWave - 1) wtot[w] = m;
+            __syncthreads();
+            uint64_t pre = kIdentityMap;
+            for (int q = 0; q < w; ++q) pre = map_compose(wtot[q], pre);
+            m = map_compose(m, pre);
+            const uint32_t ps = static_cast<uint32_t>(m >> (4 * carry)) & 0xf;
+            if (j < re) fin[j] = static_cast<uint8_t>(ps % 3);
+            uint64_t all = kIdentityMap;
+            for (int q = 0; q < kSolveWaves; ++q) all = map_compose(wtot[q], all);
+            carry = static_cast<uint32_t>(all >> (4 * carry)) & 0xf;
+            __syncthreads();
+        }
+    }
+    for (int j = tid; j < nq; j += kSolveThreads) {
+        const uint32_t v = fin[j];
+        d.fin[qo + j] = v == kSG ? kGround : (v == kSO ? kObstacle : kPending);
+    }
+}
+
+// Labels per input point (populateLabels, segmenter.cpp:640-669): only the
+// winner of a pixel gets its pixel's final label; PENDING stays UNKNOWN.
+// Also flags obstacle points for the input-order split of processor.cpp:562-579.
+__global__ void k_seg_labels(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n) return;
+    const int gi = fd.pt_off + i;
+    uint32_t lab = LP_LABEL_UNKNOWN;
+    const int pix = d.pt_pix[gi];
+    if (pix >= 0) {
+        const unsigned long long key = d.pix_key[fd.pix_off + pix];
+        if (static_cast<int>(key & ((1ull << L.idx_bits) - 1)) == i) {
+            const int qm = d.qmap[(size_t)f * L.P + pix];
+            const uint8_t st = qm >= 0 ? d.fin[(size_t)f * L.P + qm] : d.state[fd.pix_off + pix];
+            lab = st == kGround ? LP_LABEL_GROUND : (st == kObstacle ? LP_LABEL_OBSTACLE : LP_LABEL_UNKNOWN);
+        }
+    }
+    d.labels[gi] = lab;
+    d.obs_flag[(size_t)f * L.N + i] = lab == LP_LABEL_OBSTACLE ? 1u : 0u;
+}
+
+void launch_jcp(lp_context* c, int B) {
+    DevBuf& d = c->d;
+    const Layout& L = c->L;
+    hipStream_t st = c->stream;
+    const int pb = (L.P + 255) / 256;
+    stage_mark(c, "jcp_queue");
+    hipLaunchKernelGGL(k_jcp_mark, dim3(pb, B), dim3(256), 0, st, d, L);
+    launch_scan(c, B, d.qflag, reinterpret_cast<uint32_t*>(d.qmap), L.P, kCountPixels, 0, kFsQueue);
+    hipLaunchKernelGGL(k_jcp_fill, dim3(pb, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "jcp_slots");
+    hipLaunchKernelGGL(k_jcp_slots, dim3(pb, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "jcp_solve");
+    hipLaunchKernelGGL(k_jcp_solve, dim3(B), dim3(kSolveThreads), 0, st, d, L);
+    stage_mark(c, "labels");
+    hipLaunchKernelGGL(k_seg_labels, dim3((L.N + 255) / 256, B), dim3(256), 0, st, d, L);
+}
+
+}  // namespace lp

@@ -1,0 +1,33 @@
+// lp_kernels.h -- launchers of the HIP stages (host side).
+#pragma once
+
+#include "lp_device.h"
+#include "lp_internal.h"
+
+namespace lp {
+
+// Records a named stage boundary on the context stream when profiling.
+void stage_mark(lp_context* c, const char* name);
+
+// Which per-frame count bounds a scan.
+enum ScanCount : int { kCountPoints = 0, kCountPixels = 1, kCountSlot = 2 };
+
+// Exclusive scan of `in` into `out` for every frame of the batch; frame f
+// covers in[f*stride .. f*stride + count_f).  The total goes to
+// fs[f][total_slot] (if >= 0).
+void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
+                 int count_slot, int total_slot);
+
+void launch_mt19937(uint32_t* out, int count, hipStream_t st);
+void launch_segment_front(lp_context* c, int B);
+void launch_jcp(lp_context* c, int B);
+void launch_cluster(lp_context* c, int B, bool from_split);
+// Fill kernel of the per-batch device state (first stage of every run).
+void launch_clear(lp_context* c, int B);
+void launch_hull(lp_context* c, int B);
+
+// Standalone hull entry (lp_convex_hulls): `num_lists` lists staged in the
+// round-0 hull buffers of frame 0 with offsets in cl_off.
+void launch_hull_lists(lp_context* c, int num_lists, int total_points);
+
+}  // namespace lp

@@ -1,0 +1,669 @@
+// lp_segment.hip -- polar grid, range-image winner, RECM and RANSAC on gfx950.
+//
+// Reference semantics (paths relative to the reference root):
+//   constructPolarGrid   lidar_processing_lib/src/segmenter.cpp:103-204
+//   RECM elevation map   lidar_processing_lib/src/segmenter.cpp:206-267
+//   obstacle test        lidar_processing_lib/src/segmenter.cpp:269-283
+//   RANSAC correction    lidar_processing_lib/src/segmenter.cpp:321-479
+//   image transfer       lidar_processing_lib/src/segmenter.cpp:289-318
+//
+// Layout: points are CSR-packed per frame (FrameDesc::pt_off); per-frame
+// arrays of cells / pixels / queue entries use fixed frame strides.  Grids
+// are (tiles, frames) and sized for the batch capacity so one launch
+// sequence (and one captured graph) serves every batch of a shape bucket.
+#include "lp_kernels.h"
+
+namespace lp {
+
+using namespace lpd;
+
+constexpr float kTwoPi = 6.28318548202514648438f;  // (float)(2*M_PI), segmenter.hpp:136
+constexpr uint32_t kEmptyZ = 0x7fc00000u;           // NaN marks an empty cell
+
+// ---------------------------------------------------------------------------
+// Per-batch clear (fill kernels instead of memsets so the whole sequence can
+// be captured in a graph).
+__global__ void k_clear(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc fd = d.frames[f];
+    const int cells = fd.sc.cells;
+    const int P = fd.sc.H * fd.sc.W;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = t0; c < cells; c += stride) {
+        d.cell_count[(size_t)f * L.cell_stride + c] = 0;
+        d.cell_cursor[(size_t)f * L.cell_stride + c] = 0;
+    }
+    for (int p = t0; p < P; p += stride) d.pix_key[fd.pix_off + p] = ~0ull;
+    for (int h = t0; h < L.hash_cap; h += stride) {
+        const size_t o = (size_t)f * L.hash_cap + h;
+        d.hkey[o] = -1;
+        d.hmin[o] = 0x7fffffff;
+        d.hcount[o] = 0;
+    }
+    for (int k = t0; k < kRansacIters; k += stride) d.hyp_count[f * kRansacIters + k] = 0;
+    if (t0 < kFsCount) d.fs[f * kFsCount + t0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// One thread per point: polar cell, range-image pixel, cell histogram and the
+// pixel winner.  The winner of a pixel is the first point, in (cell, input)
+// order, with the smallest x*x+y*y (segmenter.cpp:291-318, strict `<`).
+// Equal depth implies equal radial cell, so the order reduces to
+// (depth, slice, index): packed into one 64-bit key and resolved with a
+// single atomicMin.
+__global__ void k_project(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n) return;
+    const SegConst& s = fd.sc;
+    const int gi = fd.pt_off + i;
+    const float x = d.x[gi], y = d.y[gi], z = d.z[gi];
+    int cell = -1, pix = -1;
+    const bool finite = isfinite(x) && isfinite(y) && isfinite(z);
+    if (finite && !(z < s.zlo || z > s.zhi)) {
+        const float dd = x * x + y * y;
+        const float dist = sqrtf(dd);
+        const int radial = static_cast<int>(dist / s.spacing);
+        if (!(dist < s.min_d || dist > s.max_d || radial >= s.rings)) {
+            float az = atan2_approx(y, x);
+            az = (az < 0) ? (az + kTwoPi) : az;
+            const int slice = min(static_cast<int>(az / s.slice_res), s.slices - 1);
+            int row = -1;
+            if (s.organized && fd.has_ring) {
+                const int h = d.ring[gi];
+                if (h < s.H) row = h;
+            } else {
+                const float elev = fdlibm_atanf(z / dist);
+                const int hs = static_cast<int>((elev - s.down_rad) / s.per_pixel);
+                if (hs >= 0 && hs < s.H) row = hs;
+            }
+            if (row >= 0) {
+                const int col = static_cast<uint16_t>((s.w_minus_1 * az) / kTwoPi);
+                cell = slice * s.rings + radial;
+                pix = row * s.W + col;
+                atomicAdd(&d.cell_count[(size_t)f * L.cell_stride + cell], 1u);
+                const unsigned long long key = (static_cast<unsigned long long>(f2u(dd)) << 32) |
+                                               (static_cast<unsigned long long>(slice) << L.idx_bits) |
+                                               static_cast<unsigned long long>(i);
+                atomicMin(&d.pix_key[fd.pix_off + pix], key);
+            }
+        }
+    }
+    d.pt_cell[gi] = cell;
+    d.pt_pix[gi] = pix;
+}
+
+// Exclusive scan of the cell histogram, one 1024-thread workgroup per frame.
+__global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[1024 / kWave + 1];
+    const int f = blockIdx.x;
+    const int cells = d.frames[f].sc.cells;
+    const uint32_t* cnt = d.cell_count + (size_t)f * L.cell_stride;
+    uint32_t* st = d.cell_start + (size_t)f * (L.cell_stride + 1);
+    uint32_t carry = 0;
+    for (int base = 0; base < cells; base += 1024 * 8) {
+        uint32_t v[8];
+        uint32_t local = 0;
+        const int b = base + threadIdx.x * 8;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = (b + k < cells) ? cnt[b + k] : 0u;
+            local += v[k];
+        }
+        uint32_t total;
+        uint32_t ex = block_exclusive_sum<1024>(local, lds, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (b + k < cells) st[b + k] = ex;
+            ex += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) st[cells] = carry;
+}
+
+// Scatter point indices into their cells (unordered; sorted per cell next).
+__global__ void k_cell_fill(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n) return;
+    const int cell = d.pt_cell[fd.pt_off + i];
+    if (cell < 0) return;
+    const size_t co = (size_t)f * L.cell_stride;
+    const uint32_t slot = atomicAdd(&d.cell_cursor[co + cell], 1u);
+    const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+    d.cell_items[(size_t)f * L.N + start + slot] = i;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bitonic sort of m (power of two) u32 keys in LDS by NT cooperating
+// threads (tid in [0, NT)); SYNC() orders the stages.
+template <int NT, typename Sync>
+__device__ __forceinline__ void bitonic_u32(uint32_t* a, int m, int tid, Sync sync) {
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < (m >> 1); t += NT) {
+                const int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int hi = lo + j;
+                const uint32_t x = a[lo], y = a[hi];
+                const bool up = (lo & k) == 0;
+                if ((x > y) == up) {
+                    a[lo] = y;
+                    a[hi] = x;
+                }
+            }
+            sync();
+        }
+    }
+}
+
+// Robust minimum of a sorted cell (segmenter.cpp:249-260): s[0], replaced by
+// s[i] for the largest i in [1, n/2] with s[i] - s[i-1] > 0.5f.  `keys` hold
+// ord_f32(z) sorted ascending.  Returns via max-reduction over candidate i.
+__device__ __forceinline__ float unord_f32(uint32_t k) {
+    return u2f((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+template <int NT, typename Sync, typename MaxReduce>
+__device__ __forceinline__ float robust_min(const uint32_t* keys, int n, int tid, Sync sync, MaxReduce maxred) {
+    int best = 0;
+    for (int i = 1 + tid; i <= n / 2; i += NT) {
+        const float a = unord_f32(keys[i]);
+        const float b = unord_f32(keys[i - 1]);
+        if (a - b > 0.5F) best = i;  // i grows with the loop, keep the last hit
+    }
+    best = maxred(best);
+    return unord_f32(keys[best]);
+}
+
+// One wavefront per cell: sorts the cell's point indices ascending (so the
+// cell lists follow the reference's input order) for the RANSAC cells, and
+// sorts z for the robust minimum.  Cells larger than kSmallCell are queued
+// for k_cell_sort_big.
+__global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[4][kSmallCell];
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const int cell = blockIdx.x * 4 + w;
+    if (cell >= s.cells) return;
+    const size_t co = (size_t)f * L.cell_stride;
+    const uint32_t n = d.cell_count[co + cell];
+    float* zmin = d.zmin + co;
+    if (n == 0) {
+        if (lane == 0) zmin[cell] = u2f(kEmptyZ);
+        return;
+    }
+    if (n > static_cast<uint32_t>(kSmallCell)) {
+        if (lane == 0) {
+            const uint32_t k = atomicAdd(&d.fs[f * kFsCount + kFsNumBig], 1u);
+            if (k < static_cast<uint32_t>(L.cell_stride)) d.big_cells[co + k] = cell;
+        }
+        return;
+    }
+    const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+    int32_t* items = d.cell_items + (size_t)f * L.N + start;
+    uint32_t* a = lds[w];
+    int m = 1;
+    while (m < static_cast<int>(n)) m <<= 1;
+    const int radial = cell % s.rings;
+    auto sync = [] { wave_lds_sync(); };
+    if (radial < kRansacRadialBins) {
+        for (int t = lane; t < m; t += kWave) a[t] = t < static_cast<int>(n) ? static_cast<uint32_t>(items[t]) : 0xffffffffu;
+        wave_lds_sync();
+        bitonic_u32<kWave>(a, m, lane, sync);
+        for (int t = lane; t < static_cast<int>(n); t += kWave) items[t] = static_cast<int32_t>(a[t]);
+        wave_lds_sync();
+    }
+    if (radial >= 1) {
+        for (int t = lane; t < m; t += kWave)
+            a[t] = t < static_cast<int>(n) ? ord_f32(d.z[fd.pt_off + items[t]]) : 0xffffffffu;
+        wave_lds_sync();
+        bitonic_u32<kWave>(a, m, lane, sync);
+        const float r = robust_min<kWave>(a, static_cast<int>(n), lane, sync, [](int v) {
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+            return v;
+        });
+        if (lane == 0) zmin[cell] = r;
+    }
+}
+
+// Cells with more than kSmallCell points: one 1024-thread workgroup each.
+__global__ __launch_bounds__(1024) void k_cell_sort_big(DevBuf d, Layout L) {
+    __shared__ uint32_t a[kBigCell];
+    __shared__ int red[1024 / kWave];
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const size_t co = (size_t)f * L.cell_stride;
+    const uint32_t nbig = min(d.fs[f * kFsCount + kFsNumBig], static_cast<uint32_t>(L.cell_stride));
+    const int tid = threadIdx.x;
+    auto sync = [] { __syncthreads(); };
+    for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int cell = d.big_cells[co + b];
+        const uint32_t n = d.cell_count[co + cell];
+        if (n > static_cast<uint32_t>(kBigCell)) {
+            if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrCellCapacity);
+            continue;
+        }
+        const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+        int32_t* items = d.cell_items + (size_t)f * L.N + start;
+        int m = 1;
+        while (m < static_cast<int>(n)) m <<= 1;
+        const int radial = cell % s.rings;
+        if (radial < kRansacRadialBins) {
+            for (int t = tid; t < m; t += 1024) a[t] = t < static_cast<int>(n) ? static_cast<uint32_t>(items[t]) : 0xffffffffu;
+            __syncthreads();
+            bitonic_u32<1024>(a, m, tid, sync);
+            for (int t = tid; t < static_cast<int>(n); t += 1024) items[t] = static_cast<int32_t>(a[t]);
+            __syncthreads();
+        }
+        if (radial >= 1) {
+            for (int t = tid; t < m; t += 1024)
+                a[t] = t < static_cast<int>(n) ? ord_f32(d.z[fd.pt_off + items[t]]) : 0xffffffffu;
+            __syncthreads();
+            bitonic_u32<1024>(a, m, tid, sync);
+            const float r = robust_min<1024>(a, static_cast<int>(n), tid, sync, [&](int v) {
+#pragma unroll
+                for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+                if (lane_id() == 0) red[tid / kWave] = v;
+                __syncthreads();
+                int m2 = 0;
+                for (int i = 0; i < 1024 / kWave; ++i) m2 = max(m2, red[i]);
+                __syncthreads();
+                return m2;
+            });
+            if (tid == 0) d.zmin[co + cell] = r;
+        }
+        __syncthreads();
+    }
+}
+
+// RECM recurrence along each azimuth slice (segmenter.cpp:215-267).  The
+// 49 float additions must happen in order (a log-step min-plus scan would
+// re-associate them), so each slice is one sequential lane.
+__global__ void k_recm(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const SegConst& s = d.frames[f].sc;
+    const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= s.slices) return;
+    const size_t co = (size_t)f * L.cell_stride;
+    const int off = sl * s.rings;
+    float* E = d.elev + co;
+    const float* zm = d.zmin + co;
+    E[off] = s.e0;
+    float prev = s.e0;
+    for (int r = 1; r < s.rings; ++r) {
+        const float z = zm[off + r];
+        float e;
+        if (f2u(z) == kEmptyZ) {
+            e = prev + s.dz;
+        } else {
+            const float b = prev + s.dz;
+            e = (b < z) ? b : z;  // std::min(curr_z_min, prev + dz)
+        }
+        E[off + r] = e;
+        prev = e;
+    }
+}
+
+// RANSAC candidate count per near cell, one wave per cell, in the
+// reference's iteration order: k = slice*4 + radial (segmenter.cpp:328-351).
+__global__ __launch_bounds__(256) void k_cand_count(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const int k = blockIdx.x * 4 + w;
+    if (k >= s.slices * kRansacRadialBins) return;
+    const int cell = (k / kRansacRadialBins) * s.rings + (k % kRansacRadialBins);
+    const size_t co = (size_t)f * L.cell_stride;
+    const uint32_t n = d.cell_count[co + cell];
+    const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+    const int32_t* items = d.cell_items + (size_t)f * L.N + start;
+    const float e = d.elev[co + cell];
+    uint32_t c = 0;
+    for (uint32_t t = lane; t < n; t += kWave) {
+        const float z = d.z[fd.pt_off + items[t]];
+        c += (fabsf(e - z) < s.two_thr) ? 1u : 0u;
+    }
+    c = wave_sum(c);
+    if (lane == 0) d.cand_count[(size_t)f * L.cand_stride + k] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_cand_scan(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[1024 / kWave + 1];
+    const int f = blockIdx.x;
+    const SegConst& s = d.frames[f].sc;
+    const int cnt = s.slices * kRansacRadialBins;
+    const uint32_t* in = d.cand_count + (size_t)f * L.cand_stride;
+    uint32_t* out = d.cand_start + (size_t)f * L.cand_stride;
+    uint32_t carry = 0;
+    for (int base = 0; base < cnt; base += 1024 * 4) {
+        uint32_t v[4];
+        uint32_t local = 0;
+        const int b = base + threadIdx.x * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = (b + q < cnt) ? in[b + q] : 0u;
+            local += v[q];
+        }
+        uint32_t total;
+        uint32_t ex = block_exclusive_sum<1024>(local, lds, &total) + carry;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (b + q < cnt) out[b + q] = ex;
+            ex += v[q];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) d.fs[f * kFsCount + kFsCands] = carry;
+}
+
+// Candidate compaction: each wave writes its cell's candidates in input
+// order at the cell's scanned offset.
+__global__ __launch_bounds__(256) void k_cand_fill(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const int k = blockIdx.x * 4 + w;
+    if (k >= s.slices * kRansacRadialBins) return;
+    const int cell = (k / kRansacRadialBins) * s.rings + (k % kRansacRadialBins);
+    const size_t co = (size_t)f * L.cell_stride;
+    const uint32_t n = d.cell_count[co + cell];
+    const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+    const int32_t* items = d.cell_items + (size_t)f * L.N + start;
+    const float e = d.elev[co + cell];
+    uint32_t pos = d.cand_start[(size_t)f * L.cand_stride + k];
+    const size_t ob = (size_t)f * L.N;
+    for (uint32_t t0 = 0; t0 < n; t0 += kWave) {
+        const uint32_t t = t0 + lane;
+        bool pred = false;
+        float x = 0, y = 0, z = 0;
+        if (t < n) {
+            const int gi = fd.pt_off + items[t];
+            x = d.x[gi];
+            y = d.y[gi];
+            z = d.z[gi];
+            pred = fabsf(e - z) < s.two_thr;
+        }
+        const uint64_t mask = __ballot(pred);
+        if (pred) {
+            const uint32_t o = pos + lanes_below(mask);
+            d.cx[ob + o] = x;
+            d.cy[ob + o] = y;
+            d.cz[ob + o] = z;
+        }
+        pos += static_cast<uint32_t>(__popcll(mask));
+    }
+}
+
+// mt19937 with seed 42 (the reference's std::mt19937 gen{42},
+// segmenter.cpp:369): the raw output stream does not depend on the frame, so
+// it is generated once per context.  Standard MT19937 recurrence
+// (Matsumoto & Nishimura 1998).
+__global__ void k_mt19937(uint32_t* out, int count) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t mt[624];
+    mt[0] = 42u;
+    for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + static_cast<uint32_t>(i);
+    int idx = 624;
+    for (int k = 0; k < count; ++k) {
+        if (idx >= 624) {
+            for (int i = 0; i < 624; ++i) {
+                const uint32_t yv = (mt[i] & 0x80000000u) | (mt[(i + 1) % 624] & 0x7fffffffu);
+                mt[i] = mt[(i + 397) % 624] ^ (yv >> 1) ^ ((yv & 1u) ? 0x9908b0dfu : 0u);
+            }
+            idx = 0;
+        }
+        uint32_t yv = mt[idx++];
+        yv ^= yv >> 11;
+        yv ^= (yv << 7) & 0x9d2c5680u;
+        yv ^= (yv << 15) & 0xefc60000u;
+        yv ^= yv >> 18;
+        out[k] = yv;
+    }
+}
+
+// Hypotheses: lane 0 maps the raw stream to indices with libstdc++'s
+// uniform_int_distribution<uint32_t> for a 32-bit engine (Lemire's
+// nearly-divisionless method, bits/uniform_int_dist.h), redrawing p3 while
+// it equals p2 (segmenter.cpp:379-386); lanes 0..59 then build the planes
+// (segmenter.cpp:389-418).
+__global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
+    __shared__ uint32_t idx2[kRansacIters], idx3[kRansacIters];
+    __shared__ uint32_t ok;
+    const int f = blockIdx.x;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const uint32_t n = d.fs[f * kFsCount + kFsCands];
+    const int lane = threadIdx.x;
+    if (n == 0) return;  // no candidates: the reference returns (segmenter.cpp:353-356)
+    if (n == 1) {
+        if (lane == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrRansacSingle);
+        return;
+    }
+    if (lane == 0) {
+        int pos = 0;
+        bool good = true;
+        auto draw = [&]() -> uint32_t {
+            if (pos >= kMtRawCount) {
+                good = false;
+                return 0;
+            }
+            uint64_t prod = static_cast<uint64_t>(d.mt_raw[pos++]) * n;
+            uint32_t low = static_cast<uint32_t>(prod);
+            if (low < n) {
+                const uint32_t thresh = (0u - n) % n;
+                while (low < thresh && good) {
+                    if (pos >= kMtRawCount) {
+                        good = false;
+                        break;
+                    }
+                    prod = static_cast<uint64_t>(d.mt_raw[pos++]) * n;
+                    low = static_cast<uint32_t>(prod);
+                }
+            }
+            return static_cast<uint32_t>(prod >> 32);
+        };
+        for (int it = 0; it < kRansacIters && good; ++it) {
+            const uint32_t a = draw();
+            uint32_t b = draw();
+            while (b == a && good) b = draw();
+            idx2[it] = a;
+            idx3[it] = b;
+        }
+        ok = good ? 1u : 0u;
+        if (!good) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrRngExhausted);
+    }
+    __syncthreads();
+    if (!ok || lane >= kRansacIters) return;
+    const size_t ob = (size_t)f * L.N;
+    const float p1x = 0.0f, p1y = 0.0f, p1z = s.neg_h;
+    const uint32_t i2 = idx2[lane], i3 = idx3[lane];
+    const float p2x = d.cx[ob + i2], p2y = d.cy[ob + i2], p2z = d.cz[ob + i2];
+    const float p3x = d.cx[ob + i3], p3y = d.cy[ob + i3], p3z = d.cz[ob + i3];
+    float nx = ((p2y - p1y) * (p3z - p1z)) - ((p2z - p1z) * (p3y - p1y));
+    float ny = ((p2z - p1z) * (p3x - p1x)) - ((p2x - p1x) * (p3z - p1z));
+    float nz = ((p2x - p1x) * (p3y - p1y)) - ((p2y - p1y) * (p3x - p1x));
+    const float den2 = (nx * nx) + (ny * ny) + (nz * nz);
+    uint32_t valid = 0;
+    float pd = 0.0f;
+    if (!(den2 < 1.0e-5F)) {
+        const float norm = 1.0F / sqrtf(den2);
+        nz *= norm;
+        if (!(fabsf(nz) < s.cos_max)) {
+            nx *= norm;
+            ny *= norm;
+            pd = (nx * p1x) + (ny * p1y) + (nz * p1z);
+            valid = 1;
+        }
+    }
+    float* h = d.hyp + ((size_t)f * kRansacIters + lane) * 4;
+    h[0] = nx;
+    h[1] = ny;
+    h[2] = nz;
+    h[3] = pd;
+    d.hyp_valid[f * kRansacIters + lane] = valid;
+}
+
+// Inlier counts of all 60 hypotheses at once: one candidate per thread,
+// ballot + popcount per hypothesis, LDS totals, one atomic per block and
+// hypothesis (segmenter.cpp:420-431).
+__global__ __launch_bounds__(256) void k_ransac_count(DevBuf d, Layout L) {
+    __shared__ float hp[kRansacIters][4];
+    __shared__ uint32_t hv[kRansacIters];
+    __shared__ uint32_t acc[kRansacIters];
+    const int f = blockIdx.y;
+    const SegConst& s = d.frames[f].sc;
+    const uint32_t n = d.fs[f * kFsCount + kFsCands];
+    if (n < 2 || blockIdx.x * 256u >= n) return;
+    if ((d.fs[f * kFsCount + kFsErr] & (kErrRansacSingle | kErrRngExhausted)) != 0) return;
+    const int tid = threadIdx.x;
+    if (tid < kRansacIters) {
+        const float* h = d.hyp + ((size_t)f * kRansacIters + tid) * 4;
+        hp[tid][0] = h[0];
+        hp[tid][1] = h[1];
+        hp[tid][2] = h[2];
+        hp[tid][3] = h[3];
+        hv[tid] = d.hyp_valid[f * kRansacIters + tid];
+        acc[tid] = 0;
+    }
+    __syncthreads();
+    const uint32_t t = blockIdx.x * 256u + tid;
+    const size_t ob = (size_t)f * L.N;
+    const bool act = t < n;
+    float x = 0, y = 0, z = 0;
+    if (act) {
+        x = d.cx[ob + t];
+        y = d.cy[ob + t];
+        z = d.cz[ob + t];
+    }
+    for (int it = 0; it < kRansacIters; ++it) {
+        if (!hv[it]) continue;
+        const float od = fabsf((hp[it][0] * x) + (hp[it][1] * y) + (hp[it][2] * z) - hp[it][3]);
+        const uint64_t m = __ballot(act && od < s.thr);
+        if (lane_id() == 0) atomicAdd(&acc[it], static_cast<uint32_t>(__popcll(m)));
+    }
+    __syncthreads();
+    if (tid < kRansacIters && hv[tid] && acc[tid]) atomicAdd(&d.hyp_count[f * kRansacIters + tid], acc[tid]);
+}
+
+// Best hypothesis: strict `>` in iteration order, then the sign flip
+// (segmenter.cpp:433-453).
+__global__ void k_ransac_best(DevBuf d, Layout L) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= L.B_active) return;
+    uint32_t* fs = d.fs + f * kFsCount;
+    if (fs[kFsCands] < 2 || (fs[kFsErr] & (kErrRansacSingle | kErrRngExhausted))) return;
+    uint32_t best = 0;
+    float a = 0.0f, b = 0.0f, c = 1.0f, dd = 0.0f;
+    for (int it = 0; it < kRansacIters; ++it) {
+        if (!d.hyp_valid[f * kRansacIters + it]) continue;
+        const uint32_t cnt = d.hyp_count[f * kRansacIters + it];
+        if (cnt > best) {
+            best = cnt;
+            const float* h = d.hyp + ((size_t)f * kRansacIters + it) * 4;
+            a = h[0];
+            b = h[1];
+            c = h[2];
+            dd = h[3];
+        }
+    }
+    if (best > 0 && c < 0) {
+        a = -a;
+        b = -b;
+        c = -c;
+        dd = -dd;
+    }
+    fs[kFsBest] = best;
+    fs[kFsApplied] = best > 0 ? 1u : 0u;
+    fs[kFsPlaneA] = f2u(a);
+    fs[kFsPlaneB] = f2u(b);
+    fs[kFsPlaneC] = f2u(c);
+    fs[kFsPlaneD] = f2u(dd);
+}
+
+// Pixel states after RECM + RANSAC: the winner's label (segmenter.cpp:
+// 278-281 and 466-475), its index (cloud_mapping_indices_) and xyz.
+__global__ void k_pixel_label(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const SegConst& s = fd.sc;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= s.H * s.W) return;
+    const unsigned long long key = d.pix_key[fd.pix_off + p];
+    const size_t po = fd.pix_off + p;
+    if (key == ~0ull) {
+        d.state[po] = kEmpty;
+        d.mapping[po] = -1;
+        return;
+    }
+    const int i = static_cast<int>(key & ((1ull << L.idx_bits) - 1));
+    const int gi = fd.pt_off + i;
+    const float x = d.x[gi], y = d.y[gi], z = d.z[gi];
+    const int cell = d.pt_cell[gi];
+    const float e = d.elev[(size_t)f * L.cell_stride + cell];
+    uint8_t lab = (z >= e + s.thr) ? kObstacle : kGround;
+    const uint32_t* fs = d.fs + f * kFsCount;
+    if (fs[kFsApplied] && (cell % s.rings) < kRansacRadialBins) {
+        const float a = u2f(fs[kFsPlaneA]), b = u2f(fs[kFsPlaneB]), c = u2f(fs[kFsPlaneC]), dd = u2f(fs[kFsPlaneD]);
+        const float sd = (a * x) + (b * y) + (c * z) - dd;
+        if (sd < s.thr) lab = kGround;
+    }
+    d.state[po] = lab;
+    d.mapping[po] = i;
+    d.pxyz[po] = make_float4(x, y, z, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+void launch_segment_front(lp_context* c, int B) {
+    DevBuf& d = c->d;
+    const Layout& L = c->L;
+    hipStream_t st = c->stream;
+    const int nb = (L.N + 255) / 256;
+    launch_clear(c, B);
+    stage_mark(c, "project");
+    hipLaunchKernelGGL(k_project, dim3(nb, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "cells");
+    hipLaunchKernelGGL(k_cell_scan, dim3(B), dim3(1024), 0, st, d, L);
+    hipLaunchKernelGGL(k_cell_fill, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_cell_sort, dim3((L.cell_stride + 3) / 4, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_cell_sort_big, dim3(64, B), dim3(1024), 0, st, d, L);
+    stage_mark(c, "recm");
+    hipLaunchKernelGGL(k_recm, dim3((L.cand_stride / kRansacRadialBins + 255) / 256, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "ransac");
+    hipLaunchKernelGGL(k_cand_count, dim3((L.cand_stride + 3) / 4, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_cand_scan, dim3(B), dim3(1024), 0, st, d, L);
+    hipLaunchKernelGGL(k_cand_fill, dim3((L.cand_stride + 3) / 4, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_ransac_hyp, dim3(B), dim3(64), 0, st, d, L);
+    hipLaunchKernelGGL(k_ransac_count, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_ransac_best, dim3((B + 63) / 64), dim3(64), 0, st, d, L);
+    stage_mark(c, "pixels");
+    hipLaunchKernelGGL(k_pixel_label, dim3((L.P + 255) / 256, B), dim3(256), 0, st, d, L);
+}
+
+void launch_clear(lp_context* c, int B) {
+    stage_mark(c, "clear");
+    hipLaunchKernelGGL(k_clear, dim3(256, B), dim3(256), 0, c->stream, c->d, c->L);
+}
+
+void launch_mt19937(uint32_t* out, int count, hipStream_t st) {
+    hipLaunchKernelGGL(k_mt19937, dim3(1), dim3(1), 0, st, out, count);
+}
+
+}  // namespace lp

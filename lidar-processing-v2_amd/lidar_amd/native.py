@@ -1,0 +1,404 @@
+"""ctypes binding of liblp_amd.so, the C ABI declared in include/lp_amd.h.
+
+The library is the product: every call below runs HIP kernels on the GPU.
+Loading fails loudly if the library is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_HIP = -2
+ERR_CAPACITY = -3
+ERR_VOXEL_OVERFLOW = -4
+ERR_RANSAC_SINGLE_CANDIDATE = -5
+
+LABEL_UNKNOWN, LABEL_GROUND, LABEL_OBSTACLE = 0, 1, 2
+
+# Every symbol include/lp_amd.h declares.
+EXPORTED = [
+    "lp_segmenter_config_default", "lp_clusterer_config_default", "lp_clusterer_config_shipped",
+    "lp_capacity_default", "lp_context_create", "lp_context_destroy", "lp_last_error", "lp_segment",
+    "lp_cluster", "lp_convex_hulls", "lp_process_frames", "lp_batch_stage", "lp_batch_run", "lp_batch_fetch",
+    "lp_synchronize", "lp_stage_names", "lp_profile_run", "lp_context_stream", "lp_debug_fetch",
+    "lp_math_eval", "lp_math_check_range",
+]
+
+MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
+
+
+def math_eval(which, a, b=None, on_device=True):
+    """Evaluate a libm restatement on the GPU (or the host copy of its source)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = None if b is None else np.ascontiguousarray(b, np.float32)
+    out = np.zeros(len(a), np.float32)
+    check(lib().lp_math_eval(which, _p(a), _p(b), _p(out), len(a), 1 if on_device else 0))
+    return out
+
+
+def math_check_range(which, lo, ref_bits):
+    """Count device results differing from host results over float bit
+    patterns [lo, lo + len(ref_bits))."""
+    ref_bits = np.ascontiguousarray(ref_bits, np.uint32)
+    bad = ctypes.c_uint64(0)
+    check(lib().lp_math_check_range(which, lo, len(ref_bits), _p(ref_bits), ctypes.byref(bad)))
+    return bad.value
+
+DEBUG_ELEVATION_MAP, DEBUG_MAPPING, DEBUG_STATE_RECM, DEBUG_JCP_QUEUE, DEBUG_STATE_FINAL = range(5)
+_DEBUG_DTYPES = {0: np.float32, 1: np.int32, 2: np.uint8, 3: np.int32, 4: np.uint8}
+
+
+class SegmenterConfig(ctypes.Structure):
+    """lp_segmenter_config = SegmenterConfiguration (segmenter.hpp:87-112)."""
+    _fields_ = [
+        ("elevation_up_deg", ctypes.c_float),
+        ("elevation_down_deg", ctypes.c_float),
+        ("image_width", ctypes.c_int32),
+        ("image_height", ctypes.c_int32),
+        ("assume_unorganized_cloud", ctypes.c_int32),
+        ("grid_radial_spacing_m", ctypes.c_float),
+        ("grid_slice_resolution_deg", ctypes.c_float),
+        ("ground_height_threshold_m", ctypes.c_float),
+        ("road_maximum_slope_m_per_m", ctypes.c_float),
+        ("min_distance_m", ctypes.c_float),
+        ("max_distance_m", ctypes.c_float),
+        ("sensor_height_m", ctypes.c_float),
+        ("kernel_threshold_distance_m", ctypes.c_float),
+        ("amplification_factor", ctypes.c_float),
+        ("z_min_m", ctypes.c_float),
+        ("z_max_m", ctypes.c_float),
+    ]
+
+
+class ClustererConfig(ctypes.Structure):
+    """lp_clusterer_config = ClustererConfiguration (clusterer.hpp:61-68)."""
+    _fields_ = [
+        ("voxel_grid_range_resolution_m", ctypes.c_float),
+        ("voxel_grid_azimuth_resolution_deg", ctypes.c_float),
+        ("voxel_grid_elevation_resolution_deg", ctypes.c_float),
+        ("min_cluster_size", ctypes.c_uint32),
+    ]
+
+
+class Capacity(ctypes.Structure):
+    _fields_ = [
+        ("max_frames", ctypes.c_int32),
+        ("max_points_per_frame", ctypes.c_int32),
+        ("max_image_height", ctypes.c_int32),
+        ("max_image_width", ctypes.c_int32),
+        ("max_voxels", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class Cloud(ctypes.Structure):
+    _fields_ = [
+        ("x", ctypes.c_void_p),
+        ("y", ctypes.c_void_p),
+        ("z", ctypes.c_void_p),
+        ("ring", ctypes.c_void_p),
+        ("xyz_stride", ctypes.c_int32),
+        ("ring_stride", ctypes.c_int32),
+        ("num_points", ctypes.c_int32),
+        ("on_device", ctypes.c_int32),
+    ]
+
+
+class RansacResult(ctypes.Structure):
+    _fields_ = [
+        ("candidates", ctypes.c_uint32),
+        ("best_inliers", ctypes.c_uint32),
+        ("applied", ctypes.c_int32),
+        ("plane", ctypes.c_float * 4),
+    ]
+
+
+class FrameOutputs(ctypes.Structure):
+    _fields_ = [
+        ("seg_labels", ctypes.c_void_p),
+        ("obstacle_index", ctypes.c_void_p),
+        ("cluster_labels", ctypes.c_void_p),
+        ("hull_offsets", ctypes.c_void_p),
+        ("hull_indices", ctypes.c_void_p),
+        ("hull_capacity", ctypes.c_int32),
+    ]
+
+
+class FrameResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("num_points", ctypes.c_int32),
+        ("num_obstacles", ctypes.c_int32),
+        ("num_voxels", ctypes.c_int32),
+        ("num_clusters", ctypes.c_int32),
+        ("num_hull_vertices", ctypes.c_int32),
+        ("jcp_queue", ctypes.c_int32),
+        ("ransac", RansacResult),
+    ]
+
+
+def library_path():
+    return os.path.join(_HERE, "liblp_amd.so")
+
+
+def lib():
+    """Load liblp_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError("liblp_amd.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        I = ctypes.c_int32
+        L.lp_segmenter_config_default.argtypes = [ctypes.POINTER(SegmenterConfig)]
+        L.lp_segmenter_config_default.restype = None
+        for fn in ("lp_clusterer_config_default", "lp_clusterer_config_shipped"):
+            getattr(L, fn).argtypes = [ctypes.POINTER(ClustererConfig)]
+            getattr(L, fn).restype = None
+        L.lp_capacity_default.argtypes = [ctypes.POINTER(Capacity)]
+        L.lp_capacity_default.restype = None
+        L.lp_context_create.argtypes = [ctypes.POINTER(Capacity), ctypes.POINTER(P)]
+        L.lp_context_destroy.argtypes = [P]
+        L.lp_last_error.restype = ctypes.c_char_p
+        L.lp_segment.argtypes = [P, ctypes.POINTER(SegmenterConfig), ctypes.POINTER(Cloud), P,
+                                 ctypes.POINTER(RansacResult)]
+        L.lp_cluster.argtypes = [P, ctypes.POINTER(ClustererConfig), ctypes.POINTER(Cloud), P]
+        L.lp_convex_hulls.argtypes = [P, P, P, P, I, P, P, I]
+        L.lp_process_frames.argtypes = [P, ctypes.POINTER(SegmenterConfig), I, ctypes.POINTER(ClustererConfig), I,
+                                        ctypes.POINTER(Cloud), ctypes.POINTER(FrameOutputs),
+                                        ctypes.POINTER(FrameResult)]
+        L.lp_batch_stage.argtypes = [P, ctypes.POINTER(SegmenterConfig), I, ctypes.POINTER(ClustererConfig), I,
+                                     ctypes.POINTER(Cloud)]
+        L.lp_batch_run.argtypes = [P, I]
+        L.lp_batch_fetch.argtypes = [P, ctypes.POINTER(FrameOutputs), ctypes.POINTER(FrameResult)]
+        L.lp_synchronize.argtypes = [P]
+        L.lp_stage_names.argtypes = [P, ctypes.c_char_p, I]
+        L.lp_profile_run.argtypes = [P, I, P, I, ctypes.POINTER(I)]
+        L.lp_context_stream.argtypes = [P]
+        L.lp_context_stream.restype = P
+        L.lp_debug_fetch.argtypes = [P, I, I, P, I]
+        L.lp_math_eval.argtypes = [I, P, P, P, I, I]
+        L.lp_math_check_range.argtypes = [I, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.POINTER(ctypes.c_uint64)]
+        _LIB = L
+    return _LIB
+
+
+class LidarError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lp error {code}: {msg}")
+        self.code = code
+
+
+def check(code):
+    if code != OK:
+        raise LidarError(code, lib().lp_last_error().decode())
+    return code
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def segmenter_config(**kw):
+    c = SegmenterConfig()
+    lib().lp_segmenter_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def clusterer_config(shipped=True, **kw):
+    """shipped=True: processor.param.yaml values (3.0 deg elevation voxels)."""
+    c = ClustererConfig()
+    (lib().lp_clusterer_config_shipped if shipped else lib().lp_clusterer_config_default)(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def soa_cloud(x, y, z, ring=None):
+    c = Cloud()
+    c.x, c.y, c.z = (a.ctypes.data for a in (x, y, z))
+    c.ring = ring.ctypes.data if ring is not None else None
+    c.xyz_stride = 4
+    c.ring_stride = 2
+    c.num_points = len(x)
+    c.on_device = 0
+    return c
+
+
+def aos_cloud(records, has_ring=True):
+    """records: structured numpy array with fields x, y, z (f4) and ring (u2)."""
+    base = records.ctypes.data
+    c = Cloud()
+    c.x = base + records.dtype.fields["x"][1]
+    c.y = base + records.dtype.fields["y"][1]
+    c.z = base + records.dtype.fields["z"][1]
+    c.ring = base + records.dtype.fields["ring"][1] if has_ring else None
+    c.xyz_stride = records.dtype.itemsize
+    c.ring_stride = records.dtype.itemsize
+    c.num_points = len(records)
+    c.on_device = 0
+    return c
+
+
+class Context:
+    """One device context: a HIP stream plus preallocated buffers."""
+
+    def __init__(self, max_frames=1, max_points=200000, max_height=64, max_width=2048, max_voxels=200000, device=0):
+        cap = Capacity()
+        lib().lp_capacity_default(ctypes.byref(cap))
+        cap.max_frames = max_frames
+        cap.max_points_per_frame = max_points
+        cap.max_image_height = max_height
+        cap.max_image_width = max_width
+        cap.max_voxels = max_voxels
+        cap.device = device
+        h = ctypes.c_void_p()
+        check(lib().lp_context_create(ctypes.byref(cap), ctypes.byref(h)))
+        self.h = h
+        self.cap = cap
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lp_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- Segmenter / Clusterer / Polygonizer -------------------------------
+    def segment(self, x, y, z, ring=None, cfg=None, cloud=None):
+        """Segmenter::segment -> (status, labels u32, RansacResult)."""
+        cfg = cfg or segmenter_config()
+        n = len(x) if cloud is None else cloud.num_points
+        labels = np.zeros(n, np.uint32)
+        rr = RansacResult()
+        if cloud is None:
+            x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+            ring = None if ring is None else np.ascontiguousarray(ring, np.uint16)
+            cloud = soa_cloud(x, y, z, ring)
+        st = lib().lp_segment(self.h, ctypes.byref(cfg), ctypes.byref(cloud), _p(labels), ctypes.byref(rr))
+        return st, labels, rr
+
+    def cluster(self, x, y, z, cfg=None):
+        """Clusterer::cluster -> (status, labels i32)."""
+        cfg = cfg or clusterer_config()
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        labels = np.zeros(len(x), np.int32)
+        cloud = soa_cloud(x, y, z)
+        st = lib().lp_cluster(self.h, ctypes.byref(cfg), ctypes.byref(cloud), _p(labels))
+        return st, labels
+
+    def convex_hulls(self, lists):
+        """Polygonizer::convexHull on each (x, y) double list -> (status, [indices])."""
+        offs = np.zeros(len(lists) + 1, np.int32)
+        for k, (lx, _) in enumerate(lists):
+            offs[k + 1] = offs[k] + len(lx)
+        xs = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64) for a, _ in lists]) if lists else
+                                  np.zeros(0), np.float64)
+        ys = np.ascontiguousarray(np.concatenate([np.asarray(b, np.float64) for _, b in lists]) if lists else
+                                  np.zeros(0), np.float64)
+        hoff = np.zeros(len(lists) + 1, np.int32)
+        cap = max(int(offs[-1]) + 4, 4)
+        hidx = np.zeros(cap, np.int32)
+        st = lib().lp_convex_hulls(self.h, _p(xs), _p(ys), _p(offs), len(lists), _p(hoff), _p(hidx), cap)
+        if st != OK:
+            return st, None
+        return st, [hidx[hoff[k]:hoff[k + 1]].copy() for k in range(len(lists))]
+
+    # -- fused frames --------------------------------------------------------
+    def _clouds(self, frames):
+        arr = (Cloud * len(frames))()
+        keep = []
+        for i, f in enumerate(frames):
+            x, y, z = (np.ascontiguousarray(a, np.float32) for a in (f[0], f[1], f[2]))
+            ring = None if f[3] is None else np.ascontiguousarray(f[3], np.uint16)
+            keep.append((x, y, z, ring))
+            arr[i] = soa_cloud(x, y, z, ring)
+        return arr, keep
+
+    def stage(self, frames, seg_cfgs, ccfg=None):
+        """frames: list of (x, y, z, ring); seg_cfgs: one config or one per frame."""
+        ccfg = ccfg or clusterer_config()
+        seg_cfgs = seg_cfgs if isinstance(seg_cfgs, (list, tuple)) else [seg_cfgs]
+        carr = (SegmenterConfig * len(seg_cfgs))(*seg_cfgs)
+        clouds, keep = self._clouds(frames)
+        self._staged = (frames, keep, carr, ccfg, clouds)
+        check(lib().lp_batch_stage(self.h, carr, len(seg_cfgs), ctypes.byref(ccfg), len(frames), clouds))
+
+    def run(self, use_graph=True):
+        check(lib().lp_batch_run(self.h, 1 if use_graph else 0))
+
+    def synchronize(self):
+        check(lib().lp_synchronize(self.h))
+
+    def fetch(self):
+        frames = self._staged[0]
+        B = len(frames)
+        outs = (FrameOutputs * B)()
+        res = (FrameResult * B)()
+        bufs = []
+        for i, f in enumerate(frames):
+            n = len(f[0])
+            b = dict(seg=np.zeros(n, np.uint32), obs=np.zeros(max(n, 1), np.int32),
+                     clab=np.zeros(max(n, 1), np.int32), hoff=np.zeros(n + 2, np.int32),
+                     hidx=np.zeros(2 * n + 16, np.int32))
+            bufs.append(b)
+            outs[i].seg_labels = b["seg"].ctypes.data
+            outs[i].obstacle_index = b["obs"].ctypes.data
+            outs[i].cluster_labels = b["clab"].ctypes.data
+            outs[i].hull_offsets = b["hoff"].ctypes.data
+            outs[i].hull_indices = b["hidx"].ctypes.data
+            outs[i].hull_capacity = len(b["hidx"])
+        lib().lp_batch_fetch(self.h, outs, res)
+        out = []
+        for i in range(B):
+            r = res[i]
+            b = bufs[i]
+            no, K, V = r.num_obstacles, r.num_clusters, r.num_hull_vertices
+            out.append(dict(status=r.status, seg_labels=b["seg"], obstacle_index=b["obs"][:no],
+                            cluster_labels=b["clab"][:no], num_clusters=K, hull_offsets=b["hoff"][:K + 1],
+                            hull_indices=b["hidx"][:V], num_voxels=r.num_voxels, jcp_queue=r.jcp_queue,
+                            ransac=dict(candidates=r.ransac.candidates, best_inliers=r.ransac.best_inliers,
+                                        applied=r.ransac.applied, plane=np.array(list(r.ransac.plane),
+                                                                                 np.float32))))
+        return out
+
+    def process(self, frames, seg_cfgs, ccfg=None, use_graph=False):
+        self.stage(frames, seg_cfgs, ccfg)
+        self.run(use_graph)
+        return self.fetch()
+
+    def debug(self, frame, which):
+        """Intermediate device state of the last run (see lp_debug_fetch)."""
+        need = lib().lp_debug_fetch(self.h, frame, which, ctypes.c_void_p(1), 0)
+        if need < 0:
+            check(need)
+        out = np.zeros(max(need, 1), _DEBUG_DTYPES[which])
+        got = lib().lp_debug_fetch(self.h, frame, which, _p(out), len(out))
+        if got < 0:
+            check(got)
+        return out[:got]
+
+    def profile(self, runs=5):
+        """Average per-stage milliseconds of the staged batch (HIP events)."""
+        ms = np.zeros(64, np.float32)
+        ns = ctypes.c_int32(0)
+        check(lib().lp_profile_run(self.h, runs, _p(ms), 64, ctypes.byref(ns)))
+        buf = ctypes.create_string_buffer(4096)
+        lib().lp_stage_names(self.h, buf, 4096)
+        names = buf.value.decode().split(",")
+        return list(zip(names, ms[: ns.value].tolist()))

@@ -1,0 +1,261 @@
+"""ctypes binding of the CPU oracle (oracle/liblp_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker.  The product path
+(lidar_amd / liblp_amd.so) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OK = 0
+ERR_INVALID = -1
+ERR_VOXEL_OVERFLOW = -4
+ERR_RANSAC_SINGLE = -5
+
+
+class SegConfig(ctypes.Structure):
+    """Mirror of SegmenterConfiguration (segmenter.hpp:87-112)."""
+    _fields_ = [
+        ("elevation_up_deg", ctypes.c_float),
+        ("elevation_down_deg", ctypes.c_float),
+        ("image_width", ctypes.c_int32),
+        ("image_height", ctypes.c_int32),
+        ("assume_unorganized_cloud", ctypes.c_int32),
+        ("grid_radial_spacing_m", ctypes.c_float),
+        ("grid_slice_resolution_deg", ctypes.c_float),
+        ("ground_height_threshold_m", ctypes.c_float),
+        ("road_maximum_slope_m_per_m", ctypes.c_float),
+        ("min_distance_m", ctypes.c_float),
+        ("max_distance_m", ctypes.c_float),
+        ("sensor_height_m", ctypes.c_float),
+        ("kernel_threshold_distance_m", ctypes.c_float),
+        ("amplification_factor", ctypes.c_float),
+        ("z_min_m", ctypes.c_float),
+        ("z_max_m", ctypes.c_float),
+    ]
+
+
+class ClusterConfig(ctypes.Structure):
+    """Mirror of ClustererConfiguration (clusterer.hpp:61-68)."""
+    _fields_ = [
+        ("voxel_grid_range_resolution_m", ctypes.c_float),
+        ("voxel_grid_azimuth_resolution_deg", ctypes.c_float),
+        ("voxel_grid_elevation_resolution_deg", ctypes.c_float),
+        ("min_cluster_size", ctypes.c_uint32),
+    ]
+
+
+class SegDebug(ctypes.Structure):
+    _fields_ = [
+        ("elevation_map", ctypes.c_void_p),
+        ("mapping", ctypes.c_void_p),
+        ("state_recm", ctypes.c_void_p),
+        ("state_final", ctypes.c_void_p),
+        ("queue", ctypes.c_void_p),
+        ("queue_len", ctypes.c_int32),
+        ("ransac_candidates", ctypes.c_uint32),
+        ("ransac_best_inliers", ctypes.c_uint32),
+        ("ransac_applied", ctypes.c_int32),
+        ("ransac_plane", ctypes.c_float * 4),
+        ("ransac_draws", ctypes.c_uint32 * 180),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liblp_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        L.lpo_seg_config_default.argtypes = [ctypes.POINTER(SegConfig)]
+        L.lpo_cluster_config_default.argtypes = [ctypes.POINTER(ClusterConfig)]
+        L.lpo_segment.argtypes = [ctypes.POINTER(SegConfig), P, P, P, P, ctypes.c_int32, P, ctypes.POINTER(SegDebug)]
+        L.lpo_cluster.argtypes = [ctypes.POINTER(ClusterConfig), P, P, P, ctypes.c_int32, ctypes.c_int32, P]
+        L.lpo_convex_hull.argtypes = [P, P, ctypes.c_int32, P]
+        L.lpo_convex_hull.restype = ctypes.c_int32
+        L.lpo_process_frame.argtypes = [ctypes.POINTER(SegConfig), ctypes.POINTER(ClusterConfig), P, P, P, P,
+                                        ctypes.c_int32, ctypes.c_int32, P, P, P, P, P, ctypes.c_int32, P,
+                                        ctypes.c_int32]
+        L.lpo_add_ring_info.argtypes = [P, P, ctypes.c_int32, P]
+        L.lpo_atan2_approx.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.lpo_atan2_approx.restype = ctypes.c_float
+        L.lpo_std_uniform_draws.argtypes = [ctypes.c_uint32, ctypes.c_int32, P]
+        L.lpo_mt19937_raw.argtypes = [ctypes.c_int32, P]
+        for fn in ("lpo_host_expf", "lpo_host_atanf"):
+            getattr(L, fn).argtypes = [ctypes.c_float]
+            getattr(L, fn).restype = ctypes.c_float
+        L.lpo_host_atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.lpo_host_atan2f.restype = ctypes.c_float
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def seg_config(**kw):
+    c = SegConfig()
+    lib().lpo_seg_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def cluster_config(**kw):
+    c = ClusterConfig()
+    lib().lpo_cluster_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def segment(x, y, z, ring=None, cfg=None, debug=False):
+    """Segmenter::segment.  Returns (status, labels[u32], debug dict or None)."""
+    cfg = cfg or seg_config()
+    n = len(x)
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    ring = None if ring is None else np.ascontiguousarray(ring, np.uint16)
+    labels = np.zeros(n, np.uint32)
+    dbg = None
+    out = None
+    if debug:
+        H, W = cfg.image_height, cfg.image_width
+        rings = int(np.float32(cfg.max_distance_m) / np.float32(cfg.grid_radial_spacing_m))
+        slices = 360 if cfg.grid_slice_resolution_deg == 1.0 else int(
+            np.float32(2 * np.pi) / (np.float32(cfg.grid_slice_resolution_deg) * np.float32(np.pi / 180)))
+        out = dict(elevation_map=np.zeros(rings * slices, np.float32), mapping=np.zeros(H * W, np.int32),
+                   state_recm=np.zeros(H * W, np.uint8), state_final=np.zeros(H * W, np.uint8),
+                   queue=np.zeros(H * W, np.int32))
+        dbg = SegDebug()
+        for k in ("elevation_map", "mapping", "state_recm", "state_final", "queue"):
+            setattr(dbg, k, out[k].ctypes.data)
+    st = lib().lpo_segment(ctypes.byref(cfg), _p(x), _p(y), _p(z), _p(ring), n, _p(labels),
+                           ctypes.byref(dbg) if dbg is not None else None)
+    if debug:
+        out["queue"] = out["queue"][: dbg.queue_len].copy()
+        out["ransac_candidates"] = dbg.ransac_candidates
+        out["ransac_best_inliers"] = dbg.ransac_best_inliers
+        out["ransac_applied"] = dbg.ransac_applied
+        out["ransac_plane"] = np.array(list(dbg.ransac_plane), np.float32)
+        out["ransac_draws"] = np.array(list(dbg.ransac_draws), np.uint32)[:120]
+    return st, labels, out
+
+
+def cluster(x, y, z, cfg=None, max_voxels=200000):
+    cfg = cfg or cluster_config()
+    n = len(x)
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    labels = np.zeros(n, np.int32)
+    st = lib().lpo_cluster(ctypes.byref(cfg), _p(x), _p(y), _p(z), n, max_voxels, _p(labels))
+    return st, labels
+
+
+def convex_hull(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    out = np.zeros(2 * len(x) + 1, np.int32)
+    k = lib().lpo_convex_hull(_p(x), _p(y), len(x), _p(out))
+    return out[:k].copy()
+
+
+def process_frame(x, y, z, ring=None, scfg=None, ccfg=None, max_voxels=200000):
+    """Processor glue: returns dict(status, seg_labels, obstacle_count,
+    cluster_labels, num_clusters, hull_offsets, hull_indices)."""
+    scfg = scfg or seg_config()
+    ccfg = ccfg or cluster_config()
+    n = len(x)
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    ring = None if ring is None else np.ascontiguousarray(ring, np.uint16)
+    seg = np.zeros(n, np.uint32)
+    clab = np.zeros(max(n, 1), np.int32)
+    nobs = ctypes.c_int32(0)
+    ncl = ctypes.c_int32(0)
+    hoff = np.zeros(n + 2, np.int32)
+    hidx = np.zeros(2 * n + 16, np.int32)
+    st = lib().lpo_process_frame(ctypes.byref(scfg), ctypes.byref(ccfg), _p(x), _p(y), _p(z), _p(ring), n,
+                                 max_voxels, _p(seg), ctypes.byref(nobs), _p(clab), ctypes.byref(ncl), _p(hoff),
+                                 len(hoff), _p(hidx), len(hidx))
+    k = ncl.value
+    return dict(status=st, seg_labels=seg, obstacle_count=nobs.value, cluster_labels=clab[: nobs.value].copy(),
+                num_clusters=k, hull_offsets=hoff[: k + 1].copy(), hull_indices=hidx[: hoff[k]].copy())
+
+
+def add_ring_info(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    ring = np.zeros(len(x), np.uint16)
+    lib().lpo_add_ring_info(_p(x), _p(y), len(x), _p(ring))
+    return ring
+
+
+def std_uniform_draws(n, count):
+    out = np.zeros(count, np.uint32)
+    lib().lpo_std_uniform_draws(n, count, _p(out))
+    return out
+
+
+def mt19937_raw(count):
+    out = np.zeros(count, np.uint32)
+    lib().lpo_mt19937_raw(count, _p(out))
+    return out
+
+
+class Processor:
+    """Persistent oracle processor: one Segmenter, Clusterer and Polygonizer
+    constructed once and reused per frame, as the reference's processor node
+    holds them (processor.hpp:248-258)."""
+
+    def __init__(self, scfg=None, ccfg=None, max_voxels=200000):
+        L = lib()
+        if not getattr(L, "_proc_bound", False):
+            P = ctypes.c_void_p
+            L.lpo_processor_create.argtypes = [ctypes.POINTER(SegConfig), ctypes.POINTER(ClusterConfig),
+                                               ctypes.c_int32]
+            L.lpo_processor_create.restype = ctypes.c_void_p
+            L.lpo_processor_destroy.argtypes = [P]
+            L.lpo_processor_segment.argtypes = [P, P, P, P, P, ctypes.c_int32, P, P]
+            L.lpo_processor_cluster.argtypes = [P, P, P, P, ctypes.c_int32, P]
+            L.lpo_processor_frame.argtypes = [P, P, P, P, P, ctypes.c_int32, P, P, P, P, P, ctypes.c_int32, P,
+                                              ctypes.c_int32]
+            L._proc_bound = True
+        self.scfg = scfg or seg_config()
+        self.ccfg = ccfg or cluster_config()
+        self.h = L.lpo_processor_create(ctypes.byref(self.scfg), ctypes.byref(self.ccfg), max_voxels)
+        if not self.h:
+            raise ValueError("invalid oracle configuration")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lpo_processor_destroy(self.h)
+            self.h = None
+
+    def segment(self, x, y, z, ring=None, labels=None):
+        n = len(x)
+        labels = np.zeros(n, np.uint32) if labels is None else labels
+        st = lib().lpo_processor_segment(self.h, _p(x), _p(y), _p(z), _p(ring), n, _p(labels), None)
+        return st, labels
+
+    def frame(self, x, y, z, ring=None, bufs=None):
+        """Full frame (segment -> split -> cluster -> gather -> hull).  `bufs`
+        lets a timing loop reuse output arrays."""
+        n = len(x)
+        if bufs is None:
+            bufs = dict(seg=np.zeros(n, np.uint32), clab=np.zeros(max(n, 1), np.int32),
+                        hoff=np.zeros(n + 2, np.int32), hidx=np.zeros(2 * n + 16, np.int32))
+        nobs = ctypes.c_int32(0)
+        ncl = ctypes.c_int32(0)
+        st = lib().lpo_processor_frame(self.h, _p(x), _p(y), _p(z), _p(ring), n, _p(bufs["seg"]),
+                                       ctypes.byref(nobs), _p(bufs["clab"]), ctypes.byref(ncl), _p(bufs["hoff"]),
+                                       len(bufs["hoff"]), _p(bufs["hidx"]), len(bufs["hidx"]))
+        k = ncl.value
+        return dict(status=st, seg_labels=bufs["seg"], obstacle_count=nobs.value,
+                    cluster_labels=bufs["clab"][: nobs.value], num_clusters=k,
+                    hull_offsets=bufs["hoff"][: k + 1], hull_indices=bufs["hidx"][: bufs["hoff"][k]])
