@@ -143,6 +143,11 @@ int lp_cluster(lp_context* ctx, const lp_clusterer_config* cfg, const lp_cloud* 
 int lp_convex_hulls(lp_context* ctx, const double* x, const double* y, const int32_t* offsets, int32_t num_lists,
                     int32_t* hull_offsets, int32_t* hull_indices, int32_t hull_capacity);
 
+/* Dataloader::addRingInfo (src/dataloader/src/dataloader.cpp:68-137): ring
+ * index per point of a scan-ordered cloud (63 down to 0, one step at each
+ * fourth-to-first azimuth quadrant transition). */
+int lp_add_ring_info(lp_context* ctx, const float* x, const float* y, int32_t n, uint16_t* ring);
+
 /* Fused per-frame path of the processor: segment -> obstacle split (input
  * order, processor.cpp:562-579) -> cluster -> per-cluster gather -> hull. */
 typedef struct lp_frame_outputs {

@@ -1,0 +1,240 @@
+// lidar_processing_lib_amd.hpp -- the reference's Segmenter / Clusterer /
+// Polygonizer interface (namespace lidar_processing_lib) implemented on the
+// MI355X C ABI of include/lp_amd.h.
+//
+// Same names, configuration fields, defaults and call signatures as
+//   lidar_processing_lib/include/lidar_processing_lib/segmenter.hpp:69-169
+//   lidar_processing_lib/include/lidar_processing_lib/clusterer.hpp:52-94
+//   lidar_processing_lib/include/lidar_processing_lib/polygonizer.hpp:62-106
+// so the processor's calls (src/processor/src/processor.cpp:556, 599, 663)
+// compile unchanged against a pcl::PointCloud<PointT>: the templates take any
+// cloud type exposing `points` (a contiguous vector of structs with float
+// x, y, z and, for organized clouds, an integral `ring`).  Errors throw the
+// reference's exception types: capacity overflow of the voxel map ->
+// std::overflow_error (static_unordered_map.hpp:79-81); anything else ->
+// std::runtime_error.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "lp_amd.h"
+
+namespace lidar_processing_lib {
+
+enum class Label : std::uint32_t { UNKNOWN = 0, GROUND, OBSTACLE };
+
+using ClusterLabel = std::int32_t;
+
+struct SegmenterConfiguration {
+    float elevation_up_deg = 2.0F;
+    float elevation_down_deg = -24.8F;
+    std::int32_t image_width = 2048;
+    std::int32_t image_height = 64;
+    bool assume_unorganized_cloud = false;
+    float grid_radial_spacing_m = 2.0F;
+    float grid_slice_resolution_deg = 1.0F;
+    float ground_height_threshold_m = 0.2F;
+    float road_maximum_slope_m_per_m = 0.2F;
+    float min_distance_m = 2.0F;
+    float max_distance_m = 100.0F;
+    float sensor_height_m = 1.73F;
+    float kernel_threshold_distance_m = 1.0F;
+    float amplification_factor = 5.0F;
+    float z_min_m = -3.0F;
+    float z_max_m = 4.0F;
+    bool display_recm_with_low_confidence_points = false;  // accepted, ignored (OpenCV window)
+};
+
+struct ClustererConfiguration {
+    float voxel_grid_range_resolution_m = 0.4F;
+    float voxel_grid_azimuth_resolution_deg = 1.0F;
+    float voxel_grid_elevation_resolution_deg = 1.5F;
+    std::uint32_t min_cluster_size = 3;
+};
+
+struct PointXY {
+    double x;
+    double y;
+};
+
+namespace detail {
+
+template <typename P, typename = void>
+struct has_ring : std::false_type {};
+template <typename P>
+struct has_ring<P, std::void_t<decltype(std::declval<P>().ring)>> : std::true_type {};
+
+[[noreturn]] inline void raise(int code) {
+    const std::string msg = std::string("lidar_processing_lib (MI355X): ") + lp_last_error();
+    if (code == LP_ERR_VOXEL_OVERFLOW) throw std::overflow_error(msg);
+    throw std::runtime_error(msg);
+}
+
+inline void check(int code) {
+    if (code != LP_OK) raise(code);
+}
+
+// One HIP context per object, like the reference's per-object scratch.
+class Context {
+  public:
+    explicit Context(std::int32_t max_points = 200000, std::int32_t max_h = 128, std::int32_t max_w = 2048) {
+        lp_capacity cap;
+        lp_capacity_default(&cap);
+        cap.max_points_per_frame = max_points;
+        cap.max_image_height = max_h;
+        cap.max_image_width = max_w;
+        check(lp_context_create(&cap, &ctx_));
+    }
+    ~Context() { lp_context_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    lp_context* get() const noexcept { return ctx_; }
+
+  private:
+    lp_context* ctx_ = nullptr;
+};
+
+template <typename CloudT>
+lp_cloud make_cloud(const CloudT& cloud, bool use_ring) {
+    using PointT = typename std::decay_t<decltype(cloud.points)>::value_type;
+    lp_cloud c{};
+    c.num_points = static_cast<std::int32_t>(cloud.points.size());
+    c.xyz_stride = static_cast<std::int32_t>(sizeof(PointT));
+    c.ring_stride = static_cast<std::int32_t>(sizeof(PointT));
+    c.on_device = 0;
+    if (c.num_points == 0) return c;
+    const PointT& p0 = cloud.points[0];
+    c.x = &p0.x;
+    c.y = &p0.y;
+    c.z = &p0.z;
+    if constexpr (has_ring<PointT>::value) {
+        static_assert(sizeof(p0.ring) == 2, "ring must be a 16-bit field");
+        c.ring = use_ring ? &p0.ring : nullptr;
+    } else {
+        c.ring = nullptr;
+    }
+    return c;
+}
+
+}  // namespace detail
+
+class Segmenter {
+  public:
+    Segmenter() : ctx_(new detail::Context()) {}
+
+    void config(const SegmenterConfiguration& config) { config_ = config; }
+
+    const SegmenterConfiguration& config() const noexcept { return config_; }
+
+    template <typename CloudT>
+    void segment(const CloudT& cloud, std::vector<Label>& labels) {
+        lp_segmenter_config c = to_c();
+        lp_cloud in = detail::make_cloud(cloud, !config_.assume_unorganized_cloud);
+        labels.assign(cloud.points.size(), Label::UNKNOWN);
+        detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
+    }
+
+    // The RANSAC plane the reference keeps in locals (segmenter.cpp:363-374).
+    const lp_ransac_result& ransac() const noexcept { return ransac_; }
+
+  private:
+    lp_segmenter_config to_c() const {
+        lp_segmenter_config c;
+        c.elevation_up_deg = config_.elevation_up_deg;
+        c.elevation_down_deg = config_.elevation_down_deg;
+        c.image_width = config_.image_width;
+        c.image_height = config_.image_height;
+        c.assume_unorganized_cloud = config_.assume_unorganized_cloud ? 1 : 0;
+        c.grid_radial_spacing_m = config_.grid_radial_spacing_m;
+        c.grid_slice_resolution_deg = config_.grid_slice_resolution_deg;
+        c.ground_height_threshold_m = config_.ground_height_threshold_m;
+        c.road_maximum_slope_m_per_m = config_.road_maximum_slope_m_per_m;
+        c.min_distance_m = config_.min_distance_m;
+        c.max_distance_m = config_.max_distance_m;
+        c.sensor_height_m = config_.sensor_height_m;
+        c.kernel_threshold_distance_m = config_.kernel_threshold_distance_m;
+        c.amplification_factor = config_.amplification_factor;
+        c.z_min_m = config_.z_min_m;
+        c.z_max_m = config_.z_max_m;
+        return c;
+    }
+
+    SegmenterConfiguration config_;
+    std::unique_ptr<detail::Context> ctx_;
+    lp_ransac_result ransac_{};
+};
+
+class Clusterer {
+  public:
+    static constexpr std::int32_t INVALID_LABEL = -1;
+
+    Clusterer() : ctx_(new detail::Context()) {}
+
+    void config(const ClustererConfiguration& config) { config_ = config; }
+    const ClustererConfiguration& config() const noexcept { return config_; }
+
+    template <typename CloudT>
+    void cluster(const CloudT& cloud, std::vector<ClusterLabel>& labels) {
+        lp_clusterer_config c;
+        c.voxel_grid_range_resolution_m = config_.voxel_grid_range_resolution_m;
+        c.voxel_grid_azimuth_resolution_deg = config_.voxel_grid_azimuth_resolution_deg;
+        c.voxel_grid_elevation_resolution_deg = config_.voxel_grid_elevation_resolution_deg;
+        c.min_cluster_size = config_.min_cluster_size;
+        lp_cloud in = detail::make_cloud(cloud, false);
+        labels.assign(cloud.points.size(), INVALID_LABEL);
+        detail::check(lp_cluster(ctx_->get(), &c, &in, labels.data()));
+    }
+
+  private:
+    ClustererConfiguration config_;
+    std::unique_ptr<detail::Context> ctx_;
+};
+
+class Polygonizer {
+  public:
+    Polygonizer() : ctx_(new detail::Context()) {}
+
+    // Counter-clockwise hull indices into `points` (polygonizer.cpp:33-91).
+    template <typename PointT>
+    void convexHull(const std::vector<PointT>& points, std::vector<std::int32_t>& indices) {
+        std::vector<std::vector<std::int32_t>> out;
+        convexHulls(std::vector<const std::vector<PointT>*>{&points}, out);
+        indices = std::move(out[0]);
+    }
+
+    // Many point lists in one call (one launch sequence for all clusters).
+    template <typename PointT>
+    void convexHulls(const std::vector<const std::vector<PointT>*>& lists,
+                     std::vector<std::vector<std::int32_t>>& indices) {
+        std::vector<double> x, y;
+        std::vector<std::int32_t> offsets(1, 0);
+        for (const auto* l : lists) {
+            for (const auto& p : *l) {
+                x.push_back(static_cast<double>(p.x));
+                y.push_back(static_cast<double>(p.y));
+            }
+            offsets.push_back(static_cast<std::int32_t>(x.size()));
+        }
+        std::vector<std::int32_t> hoff(lists.size() + 1);
+        std::vector<std::int32_t> hidx(x.size() + 4);
+        detail::check(lp_convex_hulls(ctx_->get(), x.data(), y.data(), offsets.data(),
+                                      static_cast<std::int32_t>(lists.size()), hoff.data(), hidx.data(),
+                                      static_cast<std::int32_t>(hidx.size())));
+        indices.resize(lists.size());
+        for (std::size_t k = 0; k < lists.size(); ++k)
+            indices[k].assign(hidx.begin() + hoff[k], hidx.begin() + hoff[k + 1]);
+    }
+
+  private:
+    std::unique_ptr<detail::Context> ctx_;
+};
+
+}  // namespace lidar_processing_lib

@@ -584,6 +584,25 @@ int lp_cluster(lp_context* c, const lp_clusterer_config* cfg, const lp_cloud* cl
     return status_of(fs[kFsErr]);
 }
 
+int lp_add_ring_info(lp_context* c, const float* x, const float* y, int32_t n, uint16_t* ring) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !ring))) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > c->L.N) return fail(LP_ERR_CAPACITY, "cloud has more points than max_points_per_frame");
+    if (n == 0) return LP_OK;
+    LP_HIP(hipSetDevice(c->device));
+    FrameDesc fd{};
+    fd.n = n;
+    c->frames.assign(1, fd);
+    c->num_frames = 0;
+    LP_HIP(hipMemcpyAsync(c->d.frames, &fd, sizeof(FrameDesc), hipMemcpyHostToDevice, c->stream));
+    LP_HIP(hipMemcpyAsync(c->d.x, x, n * 4, hipMemcpyHostToDevice, c->stream));
+    LP_HIP(hipMemcpyAsync(c->d.y, y, n * 4, hipMemcpyHostToDevice, c->stream));
+    launch_ring_info(c, n);
+    LP_HIP(hipGetLastError());
+    LP_HIP(hipMemcpyAsync(ring, c->d.ring, n * 2, hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    return LP_OK;
+}
+
 int lp_convex_hulls(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_lists,
                     int32_t* hull_offsets, int32_t* hull_indices, int32_t hull_capacity) {
     if (!c || !offsets || !hull_offsets || num_lists < 0) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");

@@ -138,7 +138,7 @@ __device__ int chain_hull(const double* X, const double* Y, const int32_t* I, co
     if (n < 3) {
         // polygonizer.cpp:36-41: fewer than three points are returned as
         // they are (in obstacle-index order, the reference's list order).
-        for (int i = 0; i < n; ++i) S[kk++] = static_cast<uint16_t>(i);
+        for (int i = 0; i < n; ++i) S[kk++] = P[i];
         for (int a = 1; a < kk; ++a)
             for (int b = a; b > 0 && I[S[b - 1]] > I[S[b]]; --b) {
                 const uint16_t tmp = S[b];
@@ -239,17 +239,56 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
         const bool final_chunk = cnt <= static_cast<uint32_t>(kHullChunk);
         const uint32_t lo = ci * kHullChunk;
         const int n = static_cast<int>(min(cnt - lo, static_cast<uint32_t>(kHullChunk)));
-        int m = 1;
-        while (m < n) m <<= 1;
-        for (int i = lane; i < m; i += kWave) {
-            if (i < n) {
-                const size_t src = in_b + ob + base + lo + i;
-                X[i] = d.hcx[src];
-                Y[i] = d.hcy[src];
-                I[i] = d.hci[src];
-            }
-            P[i] = static_cast<uint16_t>(i);
+        for (int i = lane; i < n; i += kWave) {
+            const size_t src = in_b + ob + base + lo + i;
+            X[i] = d.hcx[src];
+            Y[i] = d.hcy[src];
+            I[i] = d.hci[src];
         }
+        wave_sync();
+        // Akl-Toussaint cull: points strictly inside the quadrilateral of the
+        // leftmost, lowest, rightmost and highest points (counter-clockwise
+        // along the hull boundary) are no hull vertices; exact tests only.
+        int ex[4] = {0, 0, 0, 0};
+        for (int i = lane; i < n; i += kWave) {
+            if (X[i] < X[ex[0]] || (X[i] == X[ex[0]] && i < ex[0])) ex[0] = i;
+            if (Y[i] < Y[ex[1]] || (Y[i] == Y[ex[1]] && i < ex[1])) ex[1] = i;
+            if (X[i] > X[ex[2]] || (X[i] == X[ex[2]] && i < ex[2])) ex[2] = i;
+            if (Y[i] > Y[ex[3]] || (Y[i] == Y[ex[3]] && i < ex[3])) ex[3] = i;
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            int c0 = __shfl_xor(ex[0], o, kWave), c1 = __shfl_xor(ex[1], o, kWave);
+            int c2 = __shfl_xor(ex[2], o, kWave), c3 = __shfl_xor(ex[3], o, kWave);
+            if (X[c0] < X[ex[0]] || (X[c0] == X[ex[0]] && c0 < ex[0])) ex[0] = c0;
+            if (Y[c1] < Y[ex[1]] || (Y[c1] == Y[ex[1]] && c1 < ex[1])) ex[1] = c1;
+            if (X[c2] > X[ex[2]] || (X[c2] == X[ex[2]] && c2 < ex[2])) ex[2] = c2;
+            if (Y[c3] > Y[ex[3]] || (Y[c3] == Y[ex[3]] && c3 < ex[3])) ex[3] = c3;
+        }
+        int ns = 0;
+        for (int i0 = 0; i0 < n; i0 += kWave) {
+            const int i = i0 + lane;
+            bool keep = false;
+            if (i < n) {
+                int dummy;
+                keep = true;
+                if (n >= 4) {
+                    bool inside = true;
+#pragma unroll
+                    for (int e = 0; e < 4 && inside; ++e) {
+                        const int a = ex[e], b = ex[(e + 1) & 3];
+                        inside = orient_exact(X[a], Y[a], X[b], Y[b], X[i], Y[i], &dummy) > 0;
+                    }
+                    keep = !inside;
+                }
+            }
+            const uint64_t mask = __ballot(keep);
+            if (keep) P[ns + lanes_below(mask)] = static_cast<uint16_t>(i);
+            ns += static_cast<int>(__popcll(mask));
+        }
+        int m = 1;
+        while (m < ns) m <<= 1;
+        for (int i = ns + lane; i < m; i += kWave) P[i] = static_cast<uint16_t>(n + i);  // padding sorts last
         wave_sync();
         // Bitonic sort of the permutation; padding entries (>= n) sort last.
         for (int k = 2; k <= m; k <<= 1) {
@@ -270,7 +309,7 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
         }
         if (lane == 0) {
             uint32_t inexact = 0;
-            nout[w] = chain_hull(X, Y, I, P, n, S, &inexact);
+            nout[w] = chain_hull(X, Y, I, P, ns, S, &inexact);
             if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
         }
         wave_sync();

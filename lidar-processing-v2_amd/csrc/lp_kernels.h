@@ -22,6 +22,8 @@ void launch_mt19937(uint32_t* out, int count, hipStream_t st);
 void launch_segment_front(lp_context* c, int B);
 void launch_jcp(lp_context* c, int B);
 void launch_cluster(lp_context* c, int B, bool from_split);
+// Ring indices of the scan-ordered cloud staged as frame 0 (x, y).
+void launch_ring_info(lp_context* c, int n);
 // Fill kernel of the per-batch device state (first stage of every run).
 void launch_clear(lp_context* c, int B);
 void launch_hull(lp_context* c, int B);

@@ -446,6 +446,7 @@ __global__ void k_mt19937(uint32_t* out, int count) {
 // (segmenter.cpp:389-418).
 __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
     __shared__ uint32_t idx2[kRansacIters], idx3[kRansacIters];
+    __shared__ uint32_t raw[kMtRawCount];
     __shared__ uint32_t ok;
     const int f = blockIdx.x;
     const FrameDesc& fd = d.frames[f];
@@ -457,6 +458,8 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
         if (lane == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrRansacSingle);
         return;
     }
+    for (int i = lane; i < kMtRawCount; i += 64) raw[i] = d.mt_raw[i];
+    __syncthreads();
     if (lane == 0) {
         int pos = 0;
         bool good = true;
@@ -465,7 +468,7 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
                 good = false;
                 return 0;
             }
-            uint64_t prod = static_cast<uint64_t>(d.mt_raw[pos++]) * n;
+            uint64_t prod = static_cast<uint64_t>(raw[pos++]) * n;
             uint32_t low = static_cast<uint32_t>(prod);
             if (low < n) {
                 const uint32_t thresh = (0u - n) % n;
@@ -474,7 +477,7 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
                         good = false;
                         break;
                     }
-                    prod = static_cast<uint64_t>(d.mt_raw[pos++]) * n;
+                    prod = static_cast<uint64_t>(raw[pos++]) * n;
                     low = static_cast<uint32_t>(prod);
                 }
             }

@@ -26,7 +26,7 @@ EXPORTED = [
     "lp_capacity_default", "lp_context_create", "lp_context_destroy", "lp_last_error", "lp_segment",
     "lp_cluster", "lp_convex_hulls", "lp_process_frames", "lp_batch_stage", "lp_batch_run", "lp_batch_fetch",
     "lp_synchronize", "lp_stage_names", "lp_profile_run", "lp_context_stream", "lp_debug_fetch",
-    "lp_math_eval", "lp_math_check_range",
+    "lp_math_eval", "lp_math_check_range", "lp_add_ring_info",
 ]
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
@@ -183,6 +183,7 @@ def lib():
         L.lp_context_stream.argtypes = [P]
         L.lp_context_stream.restype = P
         L.lp_debug_fetch.argtypes = [P, I, I, P, I]
+        L.lp_add_ring_info.argtypes = [P, P, P, I, P]
         L.lp_math_eval.argtypes = [I, P, P, P, I, I]
         L.lp_math_check_range.argtypes = [I, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.POINTER(ctypes.c_uint64)]
         _LIB = L
@@ -301,6 +302,14 @@ class Context:
         cloud = soa_cloud(x, y, z)
         st = lib().lp_cluster(self.h, ctypes.byref(cfg), ctypes.byref(cloud), _p(labels))
         return st, labels
+
+    def add_ring_info(self, x, y):
+        """Dataloader::addRingInfo -> (status, ring u16)."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        ring = np.zeros(len(x), np.uint16)
+        st = lib().lp_add_ring_info(self.h, _p(x), _p(y), len(x), _p(ring))
+        return st, ring
 
     def convex_hulls(self, lists):
         """Polygonizer::convexHull on each (x, y) double list -> (status, [indices])."""

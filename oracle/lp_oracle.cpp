@@ -853,6 +853,17 @@ void lpo_mt19937_raw(int32_t count, uint32_t* out) {
     for (int32_t i = 0; i < count; ++i) out[i] = static_cast<uint32_t>(gen());
 }
 
+// Host libm over float bit patterns [lo, lo+count): which 0 expf, 1 atanf.
+void lpo_host_math_range(int32_t which, uint32_t lo, uint32_t count, uint32_t* out_bits) {
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t b = lo + i;
+        float x;
+        std::memcpy(&x, &b, 4);
+        const float r = which == 0 ? std::exp(x) : std::atan(x);
+        std::memcpy(&out_bits[i], &r, 4);
+    }
+}
+
 float lpo_host_expf(float x) { return std::exp(x); }
 float lpo_host_atanf(float x) { return std::atan(x); }
 float lpo_host_atan2f(float y, float x) { return std::atan2(y, x); }

@@ -130,6 +130,8 @@ void lpo_std_uniform_draws(uint32_t n, int32_t count, uint32_t* out);
 /* Raw mt19937{42} outputs. */
 void lpo_mt19937_raw(int32_t count, uint32_t* out);
 float lpo_host_expf(float x);
+/* Host libm over float bit patterns [lo, lo+count): which 0 expf, 1 atanf. */
+void lpo_host_math_range(int32_t which, uint32_t lo, uint32_t count, uint32_t* out_bits);
 float lpo_host_atanf(float x);
 float lpo_host_atan2f(float y, float x);
 

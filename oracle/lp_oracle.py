@@ -202,6 +202,18 @@ def std_uniform_draws(n, count):
     return out
 
 
+def host_math_range(which, lo, count):
+    """Host C library results (bit patterns) over float bits [lo, lo+count):
+    which 0 = expf, 1 = atanf."""
+    L = lib()
+    if not getattr(L, "_range_bound", False):
+        L.lpo_host_math_range.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+        L._range_bound = True
+    out = np.zeros(count, np.uint32)
+    L.lpo_host_math_range(which, lo, count, _p(out))
+    return out
+
+
 def mt19937_raw(count):
     out = np.zeros(count, np.uint32)
     lib().lpo_mt19937_raw(count, _p(out))

@@ -1,0 +1,261 @@
+"""GPU path vs the oracle, through the C ABI (GPU).
+
+Integer and label outputs must be bit-exact; the RANSAC plane (float) is
+required to be bit-exact too, which is stricter than the 1e-5 relative bound
+the plane is specified with.  Hull vertex lists are compared as coordinate
+sequences: among points with identical (x, y) the reference keeps whichever
+index its unstable std::sort happens to place (polygonizer.cpp:52-59).
+"""
+import numpy as np
+import pytest
+
+from lidar_amd import native as N
+from lidar_amd import synth
+
+from tests.test_oracle_segmenter import build
+
+pytestmark = pytest.mark.gpu
+
+UNKNOWN, GROUND, OBSTACLE = 0, 1, 2
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(max_frames=6, max_points=262144, max_height=128, max_width=2048)
+    yield c
+    c.close()
+
+
+def cfgs(oracle, f):
+    kw = dict(image_height=f.height, image_width=f.width, elevation_up_deg=f.fov_up_deg,
+              elevation_down_deg=f.fov_down_deg)
+    return oracle.seg_config(**kw), N.segmenter_config(**kw)
+
+
+_FRAMES = {}
+
+
+def frame(name):
+    if name not in _FRAMES:
+        _FRAMES[name] = {"c1": lambda: synth.workload_c1(0), "c2": lambda: synth.workload_c2(0),
+                         "c4": lambda: synth.workload_c4(0), "c1b": lambda: synth.workload_c1(7)}[name]()
+    return _FRAMES[name]
+
+
+def hull_coords(x, y, labels, K, hoff, hidx):
+    out = []
+    order = np.argsort(labels, kind="stable")
+    starts = np.searchsorted(labels[order], np.arange(K + 1))
+    for c in range(K):
+        m = order[starts[c]:starts[c + 1]]
+        idx = hidx[hoff[c]:hoff[c + 1]]
+        out.append(np.stack([x[m][idx], y[m][idx]], 1))
+    return out
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c4"])
+def test_segment_stages_bit_exact(ctx, oracle, name):
+    f = frame(name)
+    sco, scg = cfgs(oracle, f)
+    st, lab_o, dbg = oracle.segment(f.x, f.y, f.z, f.ring, cfg=sco, debug=True)
+    stg, lab_g, rr = ctx.segment(f.x, f.y, f.z, f.ring, cfg=scg)
+    assert st == 0 and stg == 0
+    assert np.array_equal(ctx.debug(0, N.DEBUG_ELEVATION_MAP).view(np.uint32), dbg["elevation_map"].view(np.uint32))
+    assert np.array_equal(ctx.debug(0, N.DEBUG_MAPPING), dbg["mapping"])
+    assert np.array_equal(ctx.debug(0, N.DEBUG_STATE_RECM), dbg["state_recm"])
+    assert np.array_equal(ctx.debug(0, N.DEBUG_JCP_QUEUE), dbg["queue"])
+    assert np.array_equal(ctx.debug(0, N.DEBUG_STATE_FINAL), dbg["state_final"])
+    assert np.array_equal(lab_g, lab_o)
+    assert rr.candidates == dbg["ransac_candidates"] and rr.best_inliers == dbg["ransac_best_inliers"]
+    assert rr.applied == dbg["ransac_applied"]
+    assert np.array_equal(np.array(list(rr.plane), np.float32).view(np.uint32), dbg["ransac_plane"].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c4", "c1b"])
+def test_full_frame_bit_exact(ctx, oracle, name):
+    f = frame(name)
+    sco, scg = cfgs(oracle, f)
+    ref = oracle.process_frame(f.x, f.y, f.z, f.ring, scfg=sco)
+    out = ctx.process([(f.x, f.y, f.z, f.ring)], scg)[0]
+    assert ref["status"] == 0 and out["status"] == 0
+    assert np.array_equal(out["seg_labels"], ref["seg_labels"])
+    assert np.array_equal(out["obstacle_index"], np.flatnonzero(ref["seg_labels"] == OBSTACLE))
+    assert out["num_clusters"] == ref["num_clusters"]
+    assert np.array_equal(out["cluster_labels"], ref["cluster_labels"])
+    obs = out["obstacle_index"]
+    ox, oy = f.x[obs].astype(np.float64), f.y[obs].astype(np.float64)
+    K = ref["num_clusters"]
+    hg = hull_coords(ox, oy, out["cluster_labels"], K, out["hull_offsets"], out["hull_indices"])
+    hr = hull_coords(ox, oy, ref["cluster_labels"], K, ref["hull_offsets"], ref["hull_indices"])
+    bad = [c for c in range(K) if not np.array_equal(hg[c], hr[c])]
+    assert not bad, f"{len(bad)} hulls differ, first {bad[:5]}"
+
+
+def _known_answer_cases():
+    return [
+        # in-order updates (Gauss-Seidel), expected labels hand-derived
+        ([(10, 600, 20.0, -1.73), (9, 602, 20.0, -1.73), (11, 602, 20.0, -1.33), (10, 598, 20.0, -1.73)],
+         [OBSTACLE, OBSTACLE, OBSTACLE, GROUND]),
+        # stale border slots
+        ([(20, 5, 8.2, -1.73), (20, 3, 8.2, -1.52), (30, 0, 50.0, -1.73), (28, 2, 50.0, -1.58), (32, 2, 50.0, 1.5)],
+         [OBSTACLE, OBSTACLE, OBSTACLE, GROUND, OBSTACLE]),
+        # unresolved queue pixel stays UNKNOWN
+        ([(40, 1000, 20.0, -1.73), (41, 1000, 20.0, 1.27), (42, 1000, 20.0, 1.30)], [UNKNOWN, OBSTACLE, OBSTACLE]),
+    ]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_jcp_known_answers(ctx, case):
+    pts, expect = _known_answer_cases()[case]
+    x, y, z, r = build(pts)
+    st, lab, _ = ctx.segment(x, y, z, r)
+    assert st == 0
+    assert list(lab) == expect
+
+
+def test_ransac_single_candidate_is_an_error(ctx):
+    x, y, z, r = build([(2, 300, 5.0, -1.73), (30, 900, 20.0, -1.73), (31, 900, 20.0, -1.70)])
+    st, lab, _ = ctx.segment(x, y, z, r)
+    assert st == N.ERR_RANSAC_SINGLE_CANDIDATE
+
+
+def test_empty_and_tiny_clouds(ctx, oracle):
+    e = np.zeros(0, np.float32)
+    st, lab, rr = ctx.segment(e, e, e, np.zeros(0, np.uint16))
+    assert st == 0 and len(lab) == 0 and rr.applied == 0
+    x, y, z, r = build([(10, 700, 20.0, -1.73), (10, 700, 20.5, -1.73)])
+    st, lab, _ = ctx.segment(x, y, z, r)
+    assert list(lab) == [GROUND, UNKNOWN]
+
+
+def test_batch_with_uneven_frames(ctx, oracle):
+    frames = [synth.workload_c3(i) for i in range(5)]
+    assert len({f.width for f in frames}) >= 2
+    pairs = [cfgs(oracle, f) for f in frames]
+    outs = ctx.process([(f.x, f.y, f.z, f.ring) for f in frames], [p[1] for p in pairs])
+    for f, (sco, _), out in zip(frames, pairs, outs):
+        ref = oracle.process_frame(f.x, f.y, f.z, f.ring, scfg=sco)
+        assert out["status"] == 0
+        assert np.array_equal(out["seg_labels"], ref["seg_labels"])
+        assert np.array_equal(out["cluster_labels"], ref["cluster_labels"])
+        assert out["num_clusters"] == ref["num_clusters"]
+
+
+def test_graph_replay_matches_eager(ctx, oracle):
+    frames = [frame("c1"), frame("c1b")]
+    scg = cfgs(oracle, frames[0])[1]
+    ctx.stage([(f.x, f.y, f.z, f.ring) for f in frames], scg)
+    ctx.run(use_graph=False)
+    eager = ctx.fetch()
+    for _ in range(3):
+        ctx.run(use_graph=True)
+        got = ctx.fetch()
+        for a, b in zip(eager, got):
+            for k in ("seg_labels", "cluster_labels", "hull_offsets", "hull_indices"):
+                assert np.array_equal(a[k], b[k]), k
+        # eager work between replays
+        ctx.run(use_graph=False)
+        ctx.synchronize()
+
+
+def test_repeatable(ctx, oracle):
+    f = frame("c2")
+    scg = cfgs(oracle, f)[1]
+    a = ctx.process([(f.x, f.y, f.z, f.ring)], scg)[0]
+    b = ctx.process([(f.x, f.y, f.z, f.ring)], scg)[0]
+    for k in ("seg_labels", "cluster_labels", "hull_offsets", "hull_indices"):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_aos_point_records(ctx, oracle):
+    """pcl::PointXYZIR layout: x, y, z, pad, intensity, ring (u16), 32-byte records."""
+    f = frame("c1")
+    rec = np.zeros(f.n, dtype=np.dtype({"names": ["x", "y", "z", "intensity", "ring"],
+                                        "formats": ["<f4", "<f4", "<f4", "<f4", "<u2"],
+                                        "offsets": [0, 4, 8, 16, 20], "itemsize": 32}))
+    rec["x"], rec["y"], rec["z"], rec["intensity"], rec["ring"] = f.x, f.y, f.z, f.intensity, f.ring
+    cloud = N.aos_cloud(rec)
+    st, lab, _ = ctx.segment(None, None, None, cfg=cfgs(oracle, f)[1], cloud=cloud)
+    st2, lab2, _ = ctx.segment(f.x, f.y, f.z, f.ring, cfg=cfgs(oracle, f)[1])
+    assert st == 0 and np.array_equal(lab, lab2)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_standalone_cluster(ctx, oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = 5000
+    blobs = rng.uniform(-40, 40, (30, 2))
+    idx = rng.integers(0, 30, n)
+    x = (blobs[idx, 0] + rng.normal(0, 1.0, n)).astype(np.float32)
+    y = (blobs[idx, 1] + rng.normal(0, 1.0, n)).astype(np.float32)
+    z = rng.uniform(-1.5, 2.0, n).astype(np.float32)
+    st, lab = ctx.cluster(x, y, z)
+    sto, labo = oracle.cluster(x, y, z)
+    assert st == 0 and sto == 0
+    assert np.array_equal(lab, labo)
+
+
+def test_standalone_cluster_wrap_and_overflow(oracle):
+    x = np.array([-10.0, -10.0, -10.0, 10.0, 10.0, 10.0], np.float32)
+    y = np.array([0.0, 0.0, 0.0, 0.0873, 0.0873, 0.0873], np.float32)
+    z = np.array([0.0, 0.05, 0.1, 0.0, 0.05, 0.1], np.float32)
+    with N.Context(max_points=4096, max_voxels=10) as c:
+        st, lab = c.cluster(x, y, z)
+        assert st == 0 and list(lab) == [0] * 6
+        xs = np.linspace(5, 60, 40).astype(np.float32)
+        st, _ = c.cluster(xs, np.ones(40, np.float32), np.zeros(40, np.float32))
+        assert st == N.ERR_VOXEL_OVERFLOW
+
+
+def test_standalone_hulls(ctx, oracle):
+    rng = np.random.default_rng(11)
+    lists = []
+    for k in range(40):
+        n = int(rng.integers(1, 3000))
+        x = rng.normal(0, 3, n).astype(np.float32).astype(np.float64)
+        y = rng.normal(0, 1, n).astype(np.float32).astype(np.float64)
+        if k % 5 == 0:
+            y = x * 0.5  # collinear
+        if k % 7 == 0:
+            x[:] = x[0]
+            y[:] = y[0]  # identical
+        lists.append((x, y))
+    st, hulls = ctx.convex_hulls(lists)
+    assert st == 0
+    for (x, y), h in zip(lists, hulls):
+        r = oracle.convex_hull(x, y)
+        assert np.array_equal(np.stack([x[h], y[h]], 1), np.stack([x[r], y[r]], 1))
+
+
+def test_cpp_dropin_processor(tmp_path, oracle):
+    """The reference processor's calls compiled against the drop-in C++
+    classes (cpp/processor_example.cpp) give the oracle's results."""
+    import os
+    import subprocess
+    f = frame("c1b")
+    rec = np.zeros(f.n, dtype=np.dtype({"names": ["x", "y", "z", "intensity", "ring"],
+                                        "formats": ["<f4", "<f4", "<f4", "<f4", "<u2"],
+                                        "offsets": [0, 4, 8, 16, 20], "itemsize": 32}))
+    rec["x"], rec["y"], rec["z"], rec["intensity"], rec["ring"] = f.x, f.y, f.z, f.intensity, f.ring
+    src = tmp_path / "in.bin"
+    dst = tmp_path / "out.bin"
+    with open(src, "wb") as fh:
+        fh.write(np.int32(f.n).tobytes())
+        fh.write(rec.tobytes())
+    exe = os.path.join(os.path.dirname(N.library_path()), "processor_example")
+    subprocess.run([exe, str(src), str(dst)], check=True, timeout=120)
+    buf = np.fromfile(dst, np.int32)
+    seg = buf[:f.n].view(np.uint32)
+    no = buf[f.n]
+    clab = buf[f.n + 1:f.n + 1 + no]
+    k = buf[f.n + 1 + no]
+    hoff = buf[f.n + 2 + no:f.n + 3 + no + k]
+    hidx = buf[f.n + 3 + no + k:]
+    ref = oracle.process_frame(f.x, f.y, f.z, f.ring)
+    assert np.array_equal(seg, ref["seg_labels"])
+    assert np.array_equal(clab, ref["cluster_labels"])
+    obs = np.flatnonzero(ref["seg_labels"] == OBSTACLE)
+    ox, oy = f.x[obs].astype(np.float64), f.y[obs].astype(np.float64)
+    hg = hull_coords(ox, oy, clab, k, hoff, hidx)
+    hr = hull_coords(ox, oy, ref["cluster_labels"], k, ref["hull_offsets"], ref["hull_indices"])
+    assert all(np.array_equal(a, b) for a, b in zip(hg, hr))
