@@ -18,6 +18,12 @@
  *                      lidar_processing_lib/src/polygonizer.cpp:33-91)
  *   lp_process_frames replaces the per-frame body of Processor::run from the
  *                     segment call to the last hull (src/processor/src/processor.cpp:552-670)
+ *   lp_remove_noise   replaces NoiseRemover::filter
+ *                     (lidar_processing_lib/include/lidar_processing_lib/noise_remover.hpp:64,
+ *                      lidar_processing_lib/src/noise_remover.cpp:38-68)
+ *   lp_oriented_boxes replaces a loop of Polygonizer::boundingBoxRotatingCalipers calls
+ *                     (lidar_processing_lib/include/lidar_processing_lib/polygonizer.hpp:115,
+ *                      lidar_processing_lib/src/polygonizer.cpp:93-278)
  *
  * Conventions (mirroring the reference):
  *   - the caller owns inputs and outputs; every output entry is written
@@ -143,6 +149,35 @@ int lp_cluster(lp_context* ctx, const lp_clusterer_config* cfg, const lp_cloud* 
 int lp_convex_hulls(lp_context* ctx, const double* x, const double* y, const int32_t* offsets, int32_t num_lists,
                     int32_t* hull_offsets, int32_t* hull_indices, int32_t hull_capacity);
 
+/* NoiseRemoverConfiguration (noise_remover.hpp:41-54): same fields, same defaults. */
+typedef struct lp_noise_config {
+    float radius_multiplier_m_per_m;
+    float min_search_radius_m;
+    uint32_t min_neighbours;
+} lp_noise_config;
+void lp_noise_config_default(lp_noise_config* cfg);
+
+/* NoiseRemover::filter: labels[num_points] receives 0 (VALID) or 1 (NOISE).
+ * A point is NOISE when fewer than min_neighbours points, itself included,
+ * lie within its search radius.  (The reference's KD-tree search keeps
+ * unvisited stack entries between queries, kdtree.hpp:351-371, which can
+ * count a point twice; this implements the radius rule itself.) */
+int lp_remove_noise(lp_context* ctx, const lp_noise_config* cfg, const lp_cloud* cloud, uint8_t* labels);
+
+/* BoundingBox (polygonizer.hpp:75-81): corners as x0,y0,x1,y1,x2,y2,x3,y3. */
+typedef struct lp_bounding_box {
+    double corners[8];
+    float area;
+    float angle_rad;
+    int32_t is_valid;
+    int32_t reserved;
+} lp_bounding_box;
+
+/* Polygonizer::boundingBoxRotatingCalipers on num_polygons convex polygons
+ * (vertices counter-clockwise, as convexHull returns them) packed by offsets. */
+int lp_oriented_boxes(lp_context* ctx, const double* x, const double* y, const int32_t* offsets,
+                      int32_t num_polygons, lp_bounding_box* boxes);
+
 /* Dataloader::addRingInfo (src/dataloader/src/dataloader.cpp:68-137): ring
  * index per point of a scan-ordered cloud (63 down to 0, one step at each
  * fourth-to-first azimuth quadrant transition). */
@@ -157,6 +192,8 @@ typedef struct lp_frame_outputs {
     int32_t* hull_offsets;      /* [num_points + 1] (first num_clusters+1 used) */
     int32_t* hull_indices;      /* cluster-local hull vertex indices */
     int32_t hull_capacity;
+    uint8_t* noise_labels;      /* [num_points] with the noise filter on (else unused, may be NULL) */
+    lp_bounding_box* boxes;     /* [num_points] (first num_clusters used) with boxes on, else may be NULL */
 } lp_frame_outputs;
 
 typedef struct lp_frame_result {
@@ -168,7 +205,17 @@ typedef struct lp_frame_result {
     int32_t num_hull_vertices;
     int32_t jcp_queue;          /* pixels resolved by JCP */
     lp_ransac_result ransac;
+    int32_t num_noise;          /* points removed by the noise filter */
+    int32_t num_boxes;          /* clusters with an oriented box computed */
 } lp_frame_result;
+
+/* Optional stages of the fused path, kept by the context until changed:
+ * the noise filter ahead of the segmenter (NULL switches it off; the kept
+ * points, in input order, are the cloud every later output indexes, like the
+ * filter -> keep VALID -> segment order of the full pipeline), and one oriented
+ * box per cluster hull (boundingBoxRotatingCalipers over the hull vertices). */
+int lp_set_noise_filter(lp_context* ctx, const lp_noise_config* cfg);
+int lp_set_oriented_boxes(lp_context* ctx, int32_t enable);
 
 /* Processes num_frames frames as one batch (CSR-packed on the device, one
  * launch sequence).  cfgs may hold one config (shared) or num_frames. */
@@ -184,6 +231,9 @@ int lp_batch_stage(lp_context* ctx, const lp_segmenter_config* seg_cfgs, int32_t
 int lp_batch_run(lp_context* ctx, int32_t use_graph);
 int lp_batch_fetch(lp_context* ctx, const lp_frame_outputs* outputs, lp_frame_result* results);
 int lp_synchronize(lp_context* ctx);
+/* Non-blocking: 1 when all work queued on the context has finished, 0 while
+ * it is still running, negative on error. */
+int lp_query(lp_context* ctx);
 
 /* Timing of the dominant kernel for roofline reporting: average duration (ms)
  * of the last `runs` launches of the named stage, from HIP events recorded on

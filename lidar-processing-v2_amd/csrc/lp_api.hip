@@ -146,6 +146,13 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.y, B * N);
     go(x.z, B * N);
     go(x.ring, B * N);
+    go(x.rx, B * N);
+    go(x.ry, B * N);
+    go(x.rz, B * N);
+    go(x.rring, B * N);
+    go(x.noise, B * N);
+    go(x.dsort, B * N);
+    go(x.didx, B * N);
     go(x.pt_cell, B * N);
     go(x.pt_pix, B * N);
     go(x.labels, B * N);
@@ -214,11 +221,21 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.fs, B * kFsCount);
     go(x.frames, B);
     go(x.cc, 1);
+    go(x.nc, 1);
+    go(x.boxes, B * static_cast<size_t>(L.box_stride));
     return static_cast<size_t>(p - base);
 }
 
-int stage_cloud(lp_context* c, const lp_cloud& cl, int32_t off, uint8_t* raw_dev) {
-    DevBuf& d = c->d;
+// Copies a cloud into the packed inputs (x, y, z, ring at offset `off`), or
+// into the noise filter's raw inputs when `raw_inputs`.
+int stage_cloud(lp_context* c, const lp_cloud& cl, int32_t off, uint8_t* raw_dev, bool raw_inputs = false) {
+    DevBuf d = c->d;
+    if (raw_inputs) {
+        d.x = d.rx;
+        d.y = d.ry;
+        d.z = d.rz;
+        d.ring = d.rring;
+    }
     const int n = cl.num_points;
     if (n == 0) return LP_OK;
     const hipMemcpyKind kind = cl.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -252,22 +269,33 @@ int stage_cloud(lp_context* c, const lp_cloud& cl, int32_t off, uint8_t* raw_dev
 
 int run_pipeline(lp_context* c, int B, bool segment_only) {
     c->L.B_active = B;
+    if (c->staged_filtered && !segment_only) launch_noise(c, B, true);
     launch_segment_front(c, B);
     launch_jcp(c, B);
     if (!segment_only) {
         launch_cluster(c, B, true);
         launch_hull(c, B);
+        if (c->boxes_on) launch_boxes(c, B, true);
     }
     LP_HIP(hipGetLastError());
     return LP_OK;
+}
+
+// NoiseRemover::filter constants (noise_remover.cpp:44-48).
+NoiseConst noise_const(const lp_noise_config& cfg) {
+    NoiseConst nc{};
+    nc.scaling = std::pow(static_cast<double>(cfg.radius_multiplier_m_per_m), 2.0);
+    nc.min_sqr = cfg.min_search_radius_m * cfg.min_search_radius_m;
+    nc.k = cfg.min_neighbours;
+    return nc;
 }
 
 int status_of(uint32_t err) {
     if (err & kErrVoxelOverflow) return fail(LP_ERR_VOXEL_OVERFLOW, "more occupied voxels than max_voxels");
     if (err & kErrRansacSingle)
         return fail(LP_ERR_RANSAC_SINGLE_CANDIDATE, "exactly one RANSAC candidate (reference never terminates)");
-    if (err & (kErrCellCapacity | kErrQueueCapacity | kErrHullCapacity | kErrRngExhausted))
-        return fail(LP_ERR_CAPACITY, "a device capacity was exceeded (cell, JCP queue, hull or RNG stream)");
+    if (err & (kErrCellCapacity | kErrQueueCapacity | kErrHullCapacity | kErrRngExhausted | kErrBoxCapacity))
+        return fail(LP_ERR_CAPACITY, "a device capacity was exceeded (cell, JCP queue, hull, box or RNG stream)");
     return LP_OK;
 }
 
@@ -345,7 +373,8 @@ int lp_context_create(const lp_capacity* cap, lp_context** out) {
     L.cell_stride = 32768;
     L.cand_stride = 4 * 4096;
     L.hash_cap = 1 << ceil_log2(2 * static_cast<int64_t>(L.N));
-    L.blk_stride = (std::max(L.N, L.P) + 2047) / 2048 + 1;
+    L.blk_stride = (std::max({L.N, L.P, L.hash_cap}) + 2047) / 2048 + 1;
+    L.box_stride = L.N / 3 + 1024;
     const size_t bytes = layout_bytes(L, nullptr, nullptr);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->arena, bytes);
@@ -382,8 +411,10 @@ int lp_context_destroy(lp_context* c) {
 
 void* lp_context_stream(lp_context* c) { return c ? c->stream : nullptr; }
 
-int lp_batch_stage(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
-                   const lp_clusterer_config* ccfg, int32_t num_frames, const lp_cloud* clouds) {
+namespace {
+
+int stage_batch(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
+                const lp_clusterer_config* ccfg, int32_t num_frames, const lp_cloud* clouds, bool filter) {
     if (!c || !seg_cfgs || !ccfg || !clouds) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
     if (num_frames < 1 || num_frames > c->L.B) return fail(LP_ERR_CAPACITY, "batch larger than max_frames");
     if (num_seg_cfgs != 1 && num_seg_cfgs != num_frames)
@@ -402,18 +433,23 @@ int lp_batch_stage(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t n
             return fail(LP_ERR_INVALID_ARGUMENT, why);
         fd.pt_off = off;
         fd.n = cl.num_points;
+        fd.n_raw = cl.num_points;
         fd.pix_off = f * c->L.P;
         fd.has_ring = cl.ring != nullptr ? 1 : 0;
         off += cl.num_points;
     }
     // Inputs: SoA copies straight in; AoS records through a device staging
     // area (the slot-weight arrays are free at this point of the pipeline).
+    // With the noise filter on they land in its raw inputs and the filter
+    // writes the kept points where the segmenter reads them.
     uint8_t* raw = reinterpret_cast<uint8_t*>(c->d.slot_w);
     for (int f = 0; f < num_frames; ++f) {
-        const int st = stage_cloud(c, clouds[f], c->frames[f].pt_off, raw);
+        const int st = stage_cloud(c, clouds[f], c->frames[f].pt_off, raw, filter);
         if (st != LP_OK) return st;
     }
     derive_clu(*ccfg, c->cap.max_voxels, c->cc);
+    if (filter) LP_HIP(hipMemcpyAsync(c->d.nc, &c->nc, sizeof(NoiseConst), hipMemcpyHostToDevice, c->stream));
+    c->staged_filtered = filter;
     LP_HIP(hipMemcpyAsync(c->d.frames, c->frames.data(), sizeof(FrameDesc) * num_frames, hipMemcpyHostToDevice,
                           c->stream));
     LP_HIP(hipMemcpyAsync(c->d.cc, &c->cc, sizeof(CluConst), hipMemcpyHostToDevice, c->stream));
@@ -422,12 +458,21 @@ int lp_batch_stage(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t n
     return LP_OK;
 }
 
+}  // namespace
+
+int lp_batch_stage(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
+                   const lp_clusterer_config* ccfg, int32_t num_frames, const lp_cloud* clouds) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    return stage_batch(c, seg_cfgs, num_seg_cfgs, ccfg, num_frames, clouds, c->noise_on);
+}
+
 int lp_batch_run(lp_context* c, int32_t use_graph) {
     if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
     LP_HIP(hipSetDevice(c->device));
     const int B = c->num_frames;
     if (!use_graph) return run_pipeline(c, B, false);
-    if (!c->graph_exec || c->graph_frames != B) {
+    const int options = (c->staged_filtered ? 1 : 0) | (c->boxes_on ? 2 : 0);
+    if (!c->graph_exec || c->graph_frames != B || c->graph_options != options) {
         if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
         if (c->graph) (void)hipGraphDestroy(c->graph);
         c->graph_exec = nullptr;
@@ -441,6 +486,7 @@ int lp_batch_run(lp_context* c, int32_t use_graph) {
         c->graph = g;
         LP_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
         c->graph_frames = B;
+        c->graph_options = options;
     }
     LP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     return LP_OK;
@@ -449,6 +495,15 @@ int lp_batch_run(lp_context* c, int32_t use_graph) {
 int lp_synchronize(lp_context* c) {
     if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
     LP_HIP(hipStreamSynchronize(c->stream));
+    return LP_OK;
+}
+
+int lp_query(lp_context* c) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    LP_HIP(e);
     return LP_OK;
 }
 
@@ -466,7 +521,11 @@ int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_resu
         const FrameDesc& fd = c->frames[f];
         lp_frame_result r{};
         r.status = status_of(s[kFsErr]);
-        r.num_points = fd.n;
+        // Points the segmenter saw: the noise filter's kept count when it ran.
+        const int n = c->staged_filtered ? static_cast<int>(s[kFsKept]) : fd.n;
+        r.num_points = n;
+        r.num_noise = c->staged_filtered ? static_cast<int32_t>(s[kFsNoise]) : 0;
+        r.num_boxes = c->boxes_on ? static_cast<int32_t>(s[kFsBoxes]) : 0;
         r.num_obstacles = static_cast<int32_t>(s[kFsObstacles]);
         r.num_voxels = static_cast<int32_t>(s[kFsVoxels]);
         r.num_clusters = static_cast<int32_t>(s[kFsClusters]);
@@ -486,8 +545,10 @@ int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_resu
         if (!outputs) continue;
         const lp_frame_outputs& o = outputs[f];
         const size_t ob = static_cast<size_t>(f) * L.N;
-        if (o.seg_labels && fd.n)
-            LP_HIP(hipMemcpyAsync(o.seg_labels, c->d.labels + fd.pt_off, fd.n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (c->staged_filtered && o.noise_labels && fd.n_raw)
+            LP_HIP(hipMemcpyAsync(o.noise_labels, c->d.noise + ob, fd.n_raw, hipMemcpyDeviceToHost, c->stream));
+        if (o.seg_labels && n)
+            LP_HIP(hipMemcpyAsync(o.seg_labels, c->d.labels + fd.pt_off, n * 4, hipMemcpyDeviceToHost, c->stream));
         if (r.status != LP_OK) continue;
         const int no = r.num_obstacles, K = r.num_clusters, V = r.num_hull_vertices;
         if (o.obstacle_index && no)
@@ -506,6 +567,9 @@ int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_resu
             }
             LP_HIP(hipMemcpyAsync(o.hull_indices, c->d.hull_local + ob, V * 4, hipMemcpyDeviceToHost, c->stream));
         }
+        if (c->boxes_on && o.boxes && K)
+            LP_HIP(hipMemcpyAsync(o.boxes, c->d.boxes + static_cast<size_t>(f) * L.box_stride,
+                                  K * sizeof(lp_bounding_box), hipMemcpyDeviceToHost, c->stream));
     }
     LP_HIP(hipStreamSynchronize(c->stream));
     return overall;
@@ -527,7 +591,7 @@ int lp_segment(lp_context* c, const lp_segmenter_config* cfg, const lp_cloud* cl
         return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
     lp_clusterer_config cc;
     lp_clusterer_config_shipped(&cc);
-    int st = lp_batch_stage(c, cfg, 1, &cc, 1, cloud);
+    int st = stage_batch(c, cfg, 1, &cc, 1, cloud, false);
     if (st != LP_OK) return st;
     st = run_pipeline(c, 1, true);
     if (st != LP_OK) return st;
@@ -560,6 +624,7 @@ int lp_cluster(lp_context* c, const lp_clusterer_config* cfg, const lp_cloud* cl
     FrameDesc fd{};
     fd.pt_off = 0;
     fd.n = cloud->num_points;
+    fd.n_raw = cloud->num_points;
     fd.pix_off = 0;
     fd.has_ring = 0;  // no polar grid, no image: the clear stage only resets the voxel state
     c->frames.assign(1, fd);
@@ -582,6 +647,92 @@ int lp_cluster(lp_context* c, const lp_clusterer_config* cfg, const lp_cloud* cl
     LP_HIP(hipStreamSynchronize(c->stream));
     c->num_frames = 0;  // the staged frame is not a full-pipeline batch
     return status_of(fs[kFsErr]);
+}
+
+void lp_noise_config_default(lp_noise_config* cfg) {
+    cfg->radius_multiplier_m_per_m = 0.02F;
+    cfg->min_search_radius_m = 0.1F;
+    cfg->min_neighbours = 4U;
+}
+
+int lp_set_noise_filter(lp_context* c, const lp_noise_config* cfg) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    if (!cfg) {
+        c->noise_on = false;
+        return LP_OK;
+    }
+    c->nc = noise_const(*cfg);
+    c->noise_on = true;
+    return LP_OK;
+}
+
+int lp_set_oriented_boxes(lp_context* c, int32_t enable) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    c->boxes_on = enable != 0;
+    return LP_OK;
+}
+
+int lp_remove_noise(lp_context* c, const lp_noise_config* cfg, const lp_cloud* cloud, uint8_t* labels) {
+    if (!c || !cfg || !cloud || (!labels && cloud->num_points > 0))
+        return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (cloud->num_points < 0 || cloud->num_points > c->L.N)
+        return fail(LP_ERR_CAPACITY, "cloud has more points than max_points_per_frame");
+    if (cloud->num_points > 0 && (!cloud->x || !cloud->y || !cloud->z))
+        return fail(LP_ERR_INVALID_ARGUMENT, "null coordinates");
+    if (cloud->num_points == 0) return LP_OK;
+    LP_HIP(hipSetDevice(c->device));
+    FrameDesc fd{};
+    fd.n = cloud->num_points;
+    fd.n_raw = cloud->num_points;
+    c->frames.assign(1, fd);
+    lp_cloud xyz = *cloud;
+    xyz.ring = nullptr;
+    int st = stage_cloud(c, xyz, 0, reinterpret_cast<uint8_t*>(c->d.slot_w), true);
+    if (st != LP_OK) return st;
+    const NoiseConst nc = noise_const(*cfg);
+    LP_HIP(hipMemcpyAsync(c->d.frames, c->frames.data(), sizeof(FrameDesc), hipMemcpyHostToDevice, c->stream));
+    LP_HIP(hipMemcpyAsync(c->d.nc, &nc, sizeof(NoiseConst), hipMemcpyHostToDevice, c->stream));
+    c->num_frames = 0;
+    c->L.B_active = 1;
+    launch_noise(c, 1, false);
+    LP_HIP(hipGetLastError());
+    LP_HIP(hipMemcpyAsync(labels, c->d.noise, cloud->num_points, hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    return LP_OK;
+}
+
+int lp_oriented_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,
+                      lp_bounding_box* boxes) {
+    if (!c || !offsets || num_polygons < 0 || (num_polygons > 0 && !boxes))
+        return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_polygons == 0) return LP_OK;
+    const int32_t total = offsets[num_polygons];
+    if (offsets[0] != 0 || total < 0 || total > c->L.N || num_polygons > c->L.box_stride || num_polygons > c->L.N)
+        return fail(LP_ERR_CAPACITY, "polygons exceed the context capacity");
+    for (int k = 0; k < num_polygons; ++k)
+        if (offsets[k + 1] < offsets[k]) return fail(LP_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+    if (total > 0 && (!x || !y)) return fail(LP_ERR_INVALID_ARGUMENT, "null coordinates");
+    LP_HIP(hipSetDevice(c->device));
+    const size_t BN = static_cast<size_t>(c->L.B) * c->L.N;
+    std::vector<uint32_t> fs(kFsCount, 0u);
+    fs[kFsClusters] = static_cast<uint32_t>(num_polygons);
+    fs[kFsHullTotal] = static_cast<uint32_t>(total);
+    LP_HIP(hipMemcpyAsync(c->d.fs, fs.data(), kFsCount * 4, hipMemcpyHostToDevice, c->stream));
+    LP_HIP(hipMemcpyAsync(c->d.hull_off, offsets, num_polygons * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (total) {
+        LP_HIP(hipMemcpyAsync(c->d.hcx + BN, x, total * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        LP_HIP(hipMemcpyAsync(c->d.hcy + BN, y, total * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    c->num_frames = 0;
+    c->L.B_active = 1;
+    launch_boxes(c, 1, false);
+    LP_HIP(hipGetLastError());
+    LP_HIP(hipMemcpyAsync(fs.data(), c->d.fs, kFsCount * 4, hipMemcpyDeviceToHost, c->stream));
+    LP_HIP(hipMemcpyAsync(boxes, c->d.boxes, num_polygons * sizeof(lp_bounding_box), hipMemcpyDeviceToHost,
+                          c->stream));
+    LP_HIP(hipStreamSynchronize(c->stream));
+    if (fs[kFsErr] & kErrBoxCapacity) return fail(LP_ERR_CAPACITY, "a polygon exceeds the box kernel's capacity");
+    return LP_OK;
 }
 
 int lp_add_ring_info(lp_context* c, const float* x, const float* y, int32_t n, uint16_t* ring) {
@@ -623,6 +774,10 @@ int lp_convex_hulls(lp_context* c, const double* x, const double* y, const int32
     if (total) {
         LP_HIP(hipMemcpyAsync(c->d.hcx, x, total * sizeof(double), hipMemcpyHostToDevice, c->stream));
         LP_HIP(hipMemcpyAsync(c->d.hcy, y, total * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        // (x, y) pairs kept for the canonical index among identical points.
+        LP_HIP(hipMemcpy2DAsync(c->d.dsort, 16, x, 8, 8, total, hipMemcpyHostToDevice, c->stream));
+        LP_HIP(hipMemcpy2DAsync(reinterpret_cast<char*>(c->d.dsort) + 8, 16, y, 8, 8, total, hipMemcpyHostToDevice,
+                                c->stream));
     }
     launch_hull_lists(c, num_lists, total);
     LP_HIP(hipGetLastError());

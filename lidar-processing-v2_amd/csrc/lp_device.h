@@ -203,6 +203,13 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(__lane_id()); }
 
+// Orders LDS accesses between the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Number of set bits of `mask` in lanes below this one.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),

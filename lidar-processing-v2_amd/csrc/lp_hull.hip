@@ -172,12 +172,6 @@ __device__ int chain_hull(const double* X, const double* Y, const int32_t* I, co
     return kk - 1;
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ int upper_bound_u32(const uint32_t* a, int n, uint32_t v) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -407,8 +401,14 @@ __global__ void k_hull_check(DevBuf d, Layout L) {
 
 // Hull vertices as cluster-local indices: rank of the obstacle index among
 // the cluster's members (the gather order of processor.cpp:640-658).  One
-// wavefront per output vertex.
-__global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L) {
+// wavefront per output vertex.  Among members with exactly the vertex's
+// coordinates the smallest index is reported: which of several identical
+// points a chain keeps depends on how the rounds split the cluster (and the
+// reference's own choice follows its unstable std::sort), so the coordinates
+// are the defined result and the index is made canonical.  Coordinates come
+// from the obstacle points, or for standalone lists from the copy staged in
+// `list_xy` (list offset + list-local index).
+__global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const double2* list_xy) {
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const uint32_t total = d.fs[f * kFsCount + kFsHullTotal];
@@ -421,8 +421,33 @@ __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L) {
         const uint32_t base = d.cl_off[ob + c];
         const uint32_t size = d.cl_size[ob + c];
         const int32_t idx = d.hull_idx[ob + base + i];
+        double vx, vy;
+        if (list_xy) {
+            vx = list_xy[base + idx].x;
+            vy = list_xy[base + idx].y;
+        } else {
+            vx = d.ox[ob + idx];
+            vy = d.oy[ob + idx];
+        }
+        int32_t canon = idx;
+        for (uint32_t q = lane; q < size; q += kWave) {
+            const int32_t m = d.cl_members[ob + base + q];
+            if (m >= canon) continue;
+            double mx, my;
+            if (list_xy) {
+                mx = list_xy[base + m].x;
+                my = list_xy[base + m].y;
+            } else {
+                mx = d.ox[ob + m];
+                my = d.oy[ob + m];
+            }
+            if (mx == vx && my == vy) canon = m;
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) canon = min(canon, __shfl_xor(canon, o, kWave));
+        d.hull_idx[ob + base + i] = canon;
         uint32_t cnt = 0;
-        for (uint32_t q = lane; q < size; q += kWave) cnt += d.cl_members[ob + base + q] < idx ? 1u : 0u;
+        for (uint32_t q = lane; q < size; q += kWave) cnt += d.cl_members[ob + base + q] < canon ? 1u : 0u;
         cnt = wave_sum(cnt);
         if (lane == 0) d.hull_local[ob + v] = static_cast<int32_t>(cnt);
     }
@@ -461,10 +486,10 @@ void launch_hull_lists(lp_context* c, int num_lists, int total_points) {
     const int32_t* offs = reinterpret_cast<const int32_t*>(c->d.hc_choff);  // staged by the caller
     hipLaunchKernelGGL(k_hull_lists_init, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d, c->L, offs,
                        num_lists, total_points);
-    launch_hull(c, 1);
+    launch_hull(c, 1, reinterpret_cast<const double2*>(c->d.dsort));
 }
 
-void launch_hull(lp_context* c, int B) {
+void launch_hull(lp_context* c, int B, const double2* list_xy) {
     DevBuf& d = c->d;
     const Layout& L = c->L;
     hipStream_t st = c->stream;
@@ -478,7 +503,7 @@ void launch_hull(lp_context* c, int B) {
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
     hipLaunchKernelGGL(k_hull_check, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
-    hipLaunchKernelGGL(k_hull_local, dim3(512, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_hull_local, dim3(512, B), dim3(256), 0, st, d, L, list_xy);
 }
 
 }  // namespace lp

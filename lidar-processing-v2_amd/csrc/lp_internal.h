@@ -31,6 +31,7 @@ enum : uint32_t {
     kErrQueueCapacity = 1u << 3,
     kErrVoxelOverflow = 1u << 4,
     kErrHullCapacity = 1u << 5,
+    kErrBoxCapacity = 1u << 6,
 };
 
 // Segmenter constants derived on the host exactly as the reference derives
@@ -54,6 +55,15 @@ struct SegConst {
     float neg_amp;            // -amplification_factor
 };
 
+// Noise filter constants (noise_remover.cpp:44-48): the squared radius
+// multiplier (computed in double by std::pow on the host), the squared
+// minimum radius (float) and the neighbour threshold.
+struct NoiseConst {
+    double scaling;
+    float min_sqr;
+    uint32_t k;
+};
+
 // Clusterer constants (clusterer.hpp:178-185).
 struct CluConst {
     float res_r, res_az, res_el;
@@ -64,7 +74,8 @@ struct CluConst {
 // Per-frame descriptor of a batch (device copy).
 struct FrameDesc {
     int32_t pt_off;   // CSR offset of the frame's points in the packed inputs
-    int32_t n;        // points
+    int32_t n;        // points seen by the segmenter (after the noise filter, set on the device)
+    int32_t n_raw;    // points staged
     int32_t pix_off;  // CSR offset of the frame's pixels
     int32_t has_ring;
     SegConst sc;
@@ -75,6 +86,13 @@ struct DevBuf {
     // packed inputs (CSR by FrameDesc::pt_off)
     float *x, *y, *z;
     uint16_t* ring;
+    // staged inputs ahead of the noise filter, its labels and the cell-sorted
+    // points of its grid (frame stride N)
+    float *rx, *ry, *rz;
+    uint16_t* rring;
+    uint8_t* noise;
+    float4* dsort;
+    int32_t* didx;
     // per point
     int32_t* pt_cell;
     int32_t* pt_pix;
@@ -148,6 +166,8 @@ struct DevBuf {
     uint32_t* fs;            // [B][kFsCount]
     FrameDesc* frames;       // [B]
     CluConst* cc;            // [1]
+    NoiseConst* nc;          // [1]
+    lp_bounding_box* boxes;  // [B][box_stride]
 };
 
 // Per-frame scalar slots in DevBuf::fs.
@@ -167,6 +187,10 @@ enum FsSlot : int {
     kFsHullInexact,    // predicates whose double evaluation had the wrong sign
     kFsCcGlobal,       // 1: too many voxels for the LDS union-find
     kFsErr,
+    kFsBoxes,          // clusters with a box
+    // kept across the per-run clear: written by the noise filter before it
+    kFsKept,           // points kept by the noise filter
+    kFsNoise,          // points labelled NOISE
     kFsCount
 };
 
@@ -178,6 +202,7 @@ struct Layout {
     int32_t cand_stride;    // slices*4 capacity
     int32_t hash_cap;       // power of two
     int32_t blk_stride;     // scan block sums per frame
+    int32_t box_stride;     // oriented boxes per frame (capacity)
 };
 
 }  // namespace lp
@@ -195,6 +220,11 @@ struct lp_context {
     int32_t num_frames = 0;
     lp::CluConst cc{};
     int32_t staged_points = 0;
+    // optional stages of the fused path
+    bool noise_on = false;
+    lp::NoiseConst nc{};
+    bool boxes_on = false;
+    bool staged_filtered = false;   // the staged batch went through the noise filter
     // pinned host buffers for inputs / outputs
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -202,6 +232,7 @@ struct lp_context {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int32_t graph_frames = -1;
+    int32_t graph_options = -1;     // noise/boxes switches the graph was captured with
     // stage timing
     std::vector<hipEvent_t> events;
     std::vector<std::string> stage_names;

@@ -253,8 +253,11 @@ __device__ __forceinline__ uint64_t pixel_map(const DevBuf& d, size_t qo, int j,
             l = fin[sv];
         lab[i] = l;
     }
-    const int sel10 = src[10] == j - 2 ? 1 : (src[10] == j - 1 ? 2 : 0);  // 1: state(j-2), 2: state(j-1)
-    const bool sel11 = src[11] == j - 1;
+    // Only sources inside this row come from the scan's pair state; a stale
+    // column slot at column 0 or 1 can name the last queue pixels of the
+    // previous row, whose labels are already final in `fin`.
+    const int sel10 = src[10] < rs ? 0 : (src[10] == j - 2 ? 1 : (src[10] == j - 1 ? 2 : 0));  // 1: j-2, 2: j-1
+    const bool sel11 = src[11] >= rs && src[11] == j - 1;
     if (!ok) {
         uint64_t m = 0;
 #pragma unroll

@@ -10,7 +10,7 @@ namespace lp {
 void stage_mark(lp_context* c, const char* name);
 
 // Which per-frame count bounds a scan.
-enum ScanCount : int { kCountPoints = 0, kCountPixels = 1, kCountSlot = 2 };
+enum ScanCount : int { kCountPoints = 0, kCountPixels = 1, kCountSlot = 2, kCountHash = 3, kCountRawPoints = 4 };
 
 // Exclusive scan of `in` into `out` for every frame of the batch; frame f
 // covers in[f*stride .. f*stride + count_f).  The total goes to
@@ -26,7 +26,15 @@ void launch_cluster(lp_context* c, int B, bool from_split);
 void launch_ring_info(lp_context* c, int n);
 // Fill kernel of the per-batch device state (first stage of every run).
 void launch_clear(lp_context* c, int B);
-void launch_hull(lp_context* c, int B);
+// Per-cluster hulls; `list_xy` (standalone lists only) holds the lists'
+// coordinates as (x, y) pairs for the canonical choice among identical points.
+void launch_hull(lp_context* c, int B, const double2* list_xy = nullptr);
+// Noise filter over the staged raw inputs; with `compact`, the kept points
+// become the segmenter's inputs.
+void launch_noise(lp_context* c, int B, bool compact);
+// Oriented boxes of the hulls of the last hull stage (gathered from the
+// obstacle coordinates) or, for lists, of polygons staged by the caller.
+void launch_boxes(lp_context* c, int B, bool from_hulls);
 
 // Standalone hull entry (lp_convex_hulls): `num_lists` lists staged in the
 // round-0 hull buffers of frame 0 with offsets in cl_off.

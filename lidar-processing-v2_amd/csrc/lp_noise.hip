@@ -1,0 +1,215 @@
+// lp_noise.hip -- dynamic radius outlier removal (DROR) on the GPU.
+//
+// NoiseRemover::filter (lidar_processing_lib/src/noise_remover.cpp:38-68):
+// a point is NOISE when fewer than min_neighbours points, itself included,
+// satisfy dist_sqr(point, other) <= r^2, with
+//   r^2 = max((float)(pow((double)multiplier, 2) * (double)(x*x + y*y)), min_r*min_r)
+//   dist_sqr = (t0-p0)^2 + ((t1-p1)^2 + ((t2-p2)^2 + 0))   (kdtree.hpp:131-143)
+// all in float except the widened product.
+//
+// Layout: the frame's points are bucketed into 1 m grid cells through an
+// open-addressing hash of cell keys (one slot per occupied cell, load <= 0.5),
+// counted, scanned and scattered into a cell-sorted float4 array.  One thread
+// per point (in cell-sorted order, so a wavefront reads neighbouring cells)
+// visits its own cell and then the cube of cells within ceil(r) + 1 and stops
+// at the min_neighbours-th hit.
+#include "lp_device.h"
+#include "lp_internal.h"
+#include "lp_kernels.h"
+
+namespace lp {
+
+namespace {
+
+constexpr int kNoiseMaxExtent = 4;        // cells each way; larger radii scan the whole frame
+constexpr float kNoiseCellSlack = 0.015625f;
+
+// 1 m cells; x and y clamped to [-512, 511] m and z to [-1024, 1023] m so the
+// key packs into 31 bits (never -1, the empty marker).  Clamping is monotone
+// and never moves two cells further apart, so the cube search still covers
+// every cell that can hold a point within the radius.
+__device__ __forceinline__ void noise_cell(float x, float y, float z, int& cx, int& cy, int& cz) {
+    cx = static_cast<int>(fminf(fmaxf(floorf(x), -512.0f), 511.0f)) + 512;
+    cy = static_cast<int>(fminf(fmaxf(floorf(y), -512.0f), 511.0f)) + 512;
+    cz = static_cast<int>(fminf(fmaxf(floorf(z), -1024.0f), 1023.0f)) + 1024;
+}
+
+__device__ __forceinline__ int32_t noise_key(int cx, int cy, int cz) { return cx | (cy << 10) | (cz << 20); }
+
+__device__ __forceinline__ uint32_t noise_slot(int32_t key, uint32_t mask) {
+    uint32_t h = static_cast<uint32_t>(key) * 0x9E3779B1u;
+    h ^= h >> 16;
+    return h & mask;
+}
+
+__device__ __forceinline__ int32_t noise_find(const int32_t* hkey, uint32_t mask, int32_t key) {
+    uint32_t h = noise_slot(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t k = hkey[h];
+        if (k == key) return static_cast<int32_t>(h);
+        if (k == -1) return -1;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+// noise_remover.cpp:46-59; std::max(a, b) keeps a unless a < b (a NaN stays).
+__device__ __forceinline__ float noise_radius_sqr(const NoiseConst& nc, float x, float y) {
+    const double range_sqr = static_cast<double>(x * x + y * y);
+    const float r2 = static_cast<float>(nc.scaling * range_sqr);
+    return (r2 < nc.min_sqr) ? nc.min_sqr : r2;
+}
+
+__device__ __forceinline__ float noise_dist(const float4& t, const float4& p) {
+    const float d0 = t.x - p.x, d1 = t.y - p.y, d2 = t.z - p.z;
+    return d0 * d0 + (d1 * d1 + (d2 * d2 + 0.0f));
+}
+
+__global__ void k_noise_clear(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int h = t0; h < L.hash_cap; h += stride) {
+        const size_t o = (size_t)f * L.hash_cap + h;
+        d.hkey[o] = -1;
+        d.hmin[o] = 0;
+        d.hcount[o] = 0;
+    }
+    if (t0 == 0) {
+        d.fs[f * kFsCount + kFsKept] = 0;
+        d.fs[f * kFsCount + kFsNoise] = 0;
+    }
+}
+
+__global__ void k_noise_insert(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n_raw) return;
+    const int gi = fd.pt_off + i;
+    int cx, cy, cz;
+    noise_cell(d.rx[gi], d.ry[gi], d.rz[gi], cx, cy, cz);
+    const int32_t key = noise_key(cx, cy, cz);
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    const size_t ho = (size_t)f * L.hash_cap;
+    uint32_t h = noise_slot(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+        if (prev == -1 || prev == key) break;
+        h = (h + 1) & mask;
+    }
+    atomicAdd(&d.hcount[ho + h], 1u);
+    d.vslot[(size_t)f * L.N + i] = static_cast<int32_t>(h);
+}
+
+// hdense holds the exclusive scan of hcount (cell starts) on entry.
+__global__ void k_noise_scatter(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n_raw) return;
+    const int gi = fd.pt_off + i;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const int32_t h = d.vslot[(size_t)f * L.N + i];
+    const int pos = d.hdense[ho + h] + atomicAdd(&d.hmin[ho + h], 1);
+    const size_t po = (size_t)f * L.N;
+    d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], 0.0f);
+    d.didx[po + pos] = i;
+}
+
+__global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int n = fd.n_raw;
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= n) return;
+    const NoiseConst nc = *d.nc;
+    const size_t po = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    const float4 t = d.dsort[po + pos];
+    const int i = d.didx[po + pos];
+    const float r2 = noise_radius_sqr(nc, t.x, t.y);
+    const uint32_t k = nc.k;
+    uint32_t cnt = 0;
+    if (k > 0 && r2 >= 0.0f) {
+        const float r = sqrtf(r2);
+        if (r < static_cast<float>(kNoiseMaxExtent - 1)) {
+            const int e = static_cast<int>(floorf(r + kNoiseCellSlack)) + 1;
+            int cx, cy, cz;
+            noise_cell(t.x, t.y, t.z, cx, cy, cz);
+            bool done = false;
+            // Own cell first: most points find their neighbours there.
+            for (int q = 0; q < (2 * e + 1) * (2 * e + 1) * (2 * e + 1) + 1 && !done; ++q) {
+                int ax = cx, ay = cy, az = cz;
+                if (q > 0) {
+                    const int w = 2 * e + 1;
+                    const int m = q - 1;
+                    const int dx = m % w - e, dy = (m / w) % w - e, dz = m / (w * w) - e;
+                    if (dx == 0 && dy == 0 && dz == 0) continue;
+                    ax += dx;
+                    ay += dy;
+                    az += dz;
+                    if (ax < 0 || ax > 1023 || ay < 0 || ay > 1023 || az < 0 || az > 2047) continue;
+                }
+                const int32_t h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
+                if (h < 0) continue;
+                const int s0 = d.hdense[ho + h];
+                const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
+                for (int s = s0; s < s1; ++s) {
+                    if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+        } else {
+            // Radius beyond the cube (range > ~150 m) or infinite: scan all.
+            for (int s = 0; s < n; ++s)
+                if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) break;
+        }
+    }
+    const bool noise = cnt < k;
+    d.noise[po + i] = noise ? 1 : 0;
+    d.obs_flag[po + i] = noise ? 0u : 1u;  // keep flag for the compaction scan
+    if (noise) atomicAdd(&d.fs[f * kFsCount + kFsNoise], 1u);
+}
+
+// Stable compaction of the kept points into the segmenter's inputs; the
+// frame's point count becomes the kept count.
+__global__ void k_noise_compact(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    FrameDesc& fd = d.frames[f];
+    const int n = fd.n_raw;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t po = (size_t)f * L.N;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fd.n = static_cast<int32_t>(d.fs[f * kFsCount + kFsKept]);
+    if (i >= n || !d.obs_flag[po + i]) return;
+    const int gi = fd.pt_off + i;
+    const int go = fd.pt_off + static_cast<int>(reinterpret_cast<const uint32_t*>(d.obs_src)[po + i]);
+    d.x[go] = d.rx[gi];
+    d.y[go] = d.ry[gi];
+    d.z[go] = d.rz[gi];
+    if (fd.has_ring) d.ring[go] = d.rring[gi];
+}
+
+}  // namespace
+
+void launch_noise(lp_context* c, int B, bool compact) {
+    DevBuf& d = c->d;
+    const Layout& L = c->L;
+    hipStream_t st = c->stream;
+    const int nb = (L.N + 255) / 256;
+    stage_mark(c, "noise");
+    hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_noise_insert, dim3(nb, B), dim3(256), 0, st, d, L);
+    launch_scan(c, B, d.hcount, reinterpret_cast<uint32_t*>(d.hdense), L.hash_cap, kCountHash, 0, -1);
+    hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_noise_count, dim3(nb, B), dim3(256), 0, st, d, L);
+    if (compact) {
+        launch_scan(c, B, d.obs_flag, reinterpret_cast<uint32_t*>(d.obs_src), L.N, kCountRawPoints, 0, kFsKept);
+        hipLaunchKernelGGL(k_noise_compact, dim3(nb, B), dim3(256), 0, st, d, L);
+    }
+}
+
+}  // namespace lp

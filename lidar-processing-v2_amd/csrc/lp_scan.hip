@@ -11,10 +11,12 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__device__ __forceinline__ int scan_count(const DevBuf& d, int f, ScanCount kind, int slot) {
+__device__ __forceinline__ int scan_count(const DevBuf& d, const Layout& L, int f, ScanCount kind, int slot) {
     const FrameDesc& fd = d.frames[f];
     if (kind == kCountPoints) return fd.n;
     if (kind == kCountPixels) return fd.sc.H * fd.sc.W;
+    if (kind == kCountHash) return L.hash_cap;
+    if (kind == kCountRawPoints) return fd.n_raw;
     return static_cast<int>(d.fs[f * kFsCount + slot]);
 }
 
@@ -22,7 +24,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L
                                                               ScanCount kind, int slot) {
     __shared__ uint32_t lds[kScanThreads / kWave + 1];
     const int f = blockIdx.y;
-    const int n = scan_count(d, f, kind, slot);
+    const int n = scan_count(d, L, f, kind, slot);
     const int base = blockIdx.x * kScanTile;
     if (base >= n) return;
     const uint32_t* a = in + (size_t)f * stride;
@@ -39,7 +41,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L
 __global__ __launch_bounds__(1024) void k_scan_blocks(DevBuf d, Layout L, ScanCount kind, int slot, int total_slot) {
     __shared__ uint32_t lds[1024 / kWave + 1];
     const int f = blockIdx.x;
-    const int n = scan_count(d, f, kind, slot);
+    const int n = scan_count(d, L, f, kind, slot);
     const int nb = (n + kScanTile - 1) / kScanTile;
     uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
     uint32_t carry = 0;
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, 
                                                             size_t stride, ScanCount kind, int slot) {
     __shared__ uint32_t lds[kScanThreads / kWave + 1];
     const int f = blockIdx.y;
-    const int n = scan_count(d, f, kind, slot);
+    const int n = scan_count(d, L, f, kind, slot);
     const int base = blockIdx.x * kScanTile;
     if (base >= n) return;
     const uint32_t* a = in + (size_t)f * stride;

@@ -42,7 +42,7 @@ __global__ void k_clear(DevBuf d, Layout L) {
         d.hcount[o] = 0;
     }
     for (int k = t0; k < kRansacIters; k += stride) d.hyp_count[f * kRansacIters + k] = 0;
-    if (t0 < kFsCount) d.fs[f * kFsCount + t0] = 0;
+    if (t0 < kFsKept) d.fs[f * kFsCount + t0] = 0;  // the noise filter's counts run ahead of this clear
 }
 
 // ---------------------------------------------------------------------------

@@ -26,7 +26,8 @@ EXPORTED = [
     "lp_capacity_default", "lp_context_create", "lp_context_destroy", "lp_last_error", "lp_segment",
     "lp_cluster", "lp_convex_hulls", "lp_process_frames", "lp_batch_stage", "lp_batch_run", "lp_batch_fetch",
     "lp_synchronize", "lp_stage_names", "lp_profile_run", "lp_context_stream", "lp_debug_fetch",
-    "lp_math_eval", "lp_math_check_range", "lp_add_ring_info",
+    "lp_math_eval", "lp_math_check_range", "lp_add_ring_info", "lp_query", "lp_noise_config_default",
+    "lp_remove_noise", "lp_oriented_boxes", "lp_set_noise_filter", "lp_set_oriented_boxes",
 ]
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
@@ -118,6 +119,26 @@ class RansacResult(ctypes.Structure):
     ]
 
 
+class NoiseConfig(ctypes.Structure):
+    """lp_noise_config = NoiseRemoverConfiguration (noise_remover.hpp:41-54)."""
+    _fields_ = [
+        ("radius_multiplier_m_per_m", ctypes.c_float),
+        ("min_search_radius_m", ctypes.c_float),
+        ("min_neighbours", ctypes.c_uint32),
+    ]
+
+
+class BoundingBox(ctypes.Structure):
+    """lp_bounding_box = BoundingBox (polygonizer.hpp:75-81)."""
+    _fields_ = [
+        ("corners", ctypes.c_double * 8),
+        ("area", ctypes.c_float),
+        ("angle_rad", ctypes.c_float),
+        ("is_valid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class FrameOutputs(ctypes.Structure):
     _fields_ = [
         ("seg_labels", ctypes.c_void_p),
@@ -126,6 +147,8 @@ class FrameOutputs(ctypes.Structure):
         ("hull_offsets", ctypes.c_void_p),
         ("hull_indices", ctypes.c_void_p),
         ("hull_capacity", ctypes.c_int32),
+        ("noise_labels", ctypes.c_void_p),
+        ("boxes", ctypes.c_void_p),
     ]
 
 
@@ -139,6 +162,8 @@ class FrameResult(ctypes.Structure):
         ("num_hull_vertices", ctypes.c_int32),
         ("jcp_queue", ctypes.c_int32),
         ("ransac", RansacResult),
+        ("num_noise", ctypes.c_int32),
+        ("num_boxes", ctypes.c_int32),
     ]
 
 
@@ -178,6 +203,13 @@ def lib():
         L.lp_batch_run.argtypes = [P, I]
         L.lp_batch_fetch.argtypes = [P, ctypes.POINTER(FrameOutputs), ctypes.POINTER(FrameResult)]
         L.lp_synchronize.argtypes = [P]
+        L.lp_query.argtypes = [P]
+        L.lp_noise_config_default.argtypes = [ctypes.POINTER(NoiseConfig)]
+        L.lp_noise_config_default.restype = None
+        L.lp_remove_noise.argtypes = [P, ctypes.POINTER(NoiseConfig), ctypes.POINTER(Cloud), P]
+        L.lp_oriented_boxes.argtypes = [P, P, P, P, I, ctypes.POINTER(BoundingBox)]
+        L.lp_set_noise_filter.argtypes = [P, ctypes.POINTER(NoiseConfig)]
+        L.lp_set_oriented_boxes.argtypes = [P, I]
         L.lp_stage_names.argtypes = [P, ctypes.c_char_p, I]
         L.lp_profile_run.argtypes = [P, I, P, I, ctypes.POINTER(I)]
         L.lp_context_stream.argtypes = [P]
@@ -223,6 +255,19 @@ def clusterer_config(shipped=True, **kw):
     return c
 
 
+def noise_config(**kw):
+    c = NoiseConfig()
+    lib().lp_noise_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def box_dict(b):
+    return dict(corners=np.array(b.corners[:], np.float64).reshape(4, 2), area=np.float32(b.area),
+                angle_rad=np.float32(b.angle_rad), is_valid=bool(b.is_valid))
+
+
 def soa_cloud(x, y, z, ring=None):
     c = Cloud()
     c.x, c.y, c.z = (a.ctypes.data for a in (x, y, z))
@@ -265,6 +310,7 @@ class Context:
         check(lib().lp_context_create(ctypes.byref(cap), ctypes.byref(h)))
         self.h = h
         self.cap = cap
+        self._boxes_on = False
 
     def close(self):
         if getattr(self, "h", None):
@@ -328,6 +374,39 @@ class Context:
             return st, None
         return st, [hidx[hoff[k]:hoff[k + 1]].copy() for k in range(len(lists))]
 
+    def remove_noise(self, x, y, z, cfg=None):
+        """NoiseRemover::filter -> (status, labels u8: 0 VALID, 1 NOISE)."""
+        cfg = cfg or noise_config()
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        labels = np.zeros(len(x), np.uint8)
+        cloud = soa_cloud(x, y, z)
+        st = lib().lp_remove_noise(self.h, ctypes.byref(cfg), ctypes.byref(cloud), _p(labels))
+        return st, labels
+
+    def oriented_boxes(self, polygons):
+        """boundingBoxRotatingCalipers on each (x, y) CCW polygon -> (status, [box dict])."""
+        offs = np.zeros(len(polygons) + 1, np.int32)
+        for k, (px, _) in enumerate(polygons):
+            offs[k + 1] = offs[k] + len(px)
+        xs = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64) for a, _ in polygons]) if polygons
+                                  else np.zeros(0), np.float64)
+        ys = np.ascontiguousarray(np.concatenate([np.asarray(b, np.float64) for _, b in polygons]) if polygons
+                                  else np.zeros(0), np.float64)
+        boxes = (BoundingBox * max(len(polygons), 1))()
+        st = lib().lp_oriented_boxes(self.h, _p(xs), _p(ys), _p(offs), len(polygons), boxes)
+        if st != OK:
+            return st, None
+        return st, [box_dict(boxes[k]) for k in range(len(polygons))]
+
+    def set_noise_filter(self, cfg):
+        """Noise filter ahead of the segmenter in the fused path (None: off)."""
+        check(lib().lp_set_noise_filter(self.h, None if cfg is None else ctypes.byref(cfg)))
+        self._noise_cfg = cfg
+
+    def set_oriented_boxes(self, enable):
+        check(lib().lp_set_oriented_boxes(self.h, 1 if enable else 0))
+        self._boxes_on = bool(enable)
+
     # -- fused frames --------------------------------------------------------
     def _clouds(self, frames):
         arr = (Cloud * len(frames))()
@@ -354,6 +433,12 @@ class Context:
     def synchronize(self):
         check(lib().lp_synchronize(self.h))
 
+    def query(self):
+        """True once all queued work of this context has finished."""
+        r = lib().lp_query(self.h)
+        check(min(r, 0))
+        return r == 1
+
     def fetch(self):
         frames = self._staged[0]
         B = len(frames)
@@ -364,8 +449,11 @@ class Context:
             n = len(f[0])
             b = dict(seg=np.zeros(n, np.uint32), obs=np.zeros(max(n, 1), np.int32),
                      clab=np.zeros(max(n, 1), np.int32), hoff=np.zeros(n + 2, np.int32),
-                     hidx=np.zeros(2 * n + 16, np.int32))
+                     hidx=np.zeros(2 * n + 16, np.int32), noise=np.zeros(max(n, 1), np.uint8),
+                     boxes=(BoundingBox * (max(n, 1) if self._boxes_on else 1))())
             bufs.append(b)
+            outs[i].noise_labels = b["noise"].ctypes.data
+            outs[i].boxes = ctypes.addressof(b["boxes"])
             outs[i].seg_labels = b["seg"].ctypes.data
             outs[i].obstacle_index = b["obs"].ctypes.data
             outs[i].cluster_labels = b["clab"].ctypes.data
@@ -378,7 +466,9 @@ class Context:
             r = res[i]
             b = bufs[i]
             no, K, V = r.num_obstacles, r.num_clusters, r.num_hull_vertices
-            out.append(dict(status=r.status, seg_labels=b["seg"], obstacle_index=b["obs"][:no],
+            out.append(dict(status=r.status, seg_labels=b["seg"][:r.num_points], obstacle_index=b["obs"][:no],
+                            num_points=r.num_points, num_noise=r.num_noise, noise_labels=b["noise"][:len(frames[i][0])],
+                            boxes=[box_dict(b["boxes"][k]) for k in range(K)] if self._boxes_on else [],
                             cluster_labels=b["clab"][:no], num_clusters=K, hull_offsets=b["hoff"][:K + 1],
                             hull_indices=b["hidx"][:V], num_voxels=r.num_voxels, jcp_queue=r.jcp_queue,
                             ransac=dict(candidates=r.ransac.candidates, best_inliers=r.ransac.best_inliers,

@@ -123,6 +123,41 @@ int lpo_processor_frame(lpo_processor* p, const float* x, const float* y, const 
 /* Dataloader::addRingInfo for a scan-ordered cloud. */
 void lpo_add_ring_info(const float* x, const float* y, int32_t n, uint16_t* ring);
 
+/* NoiseRemoverConfiguration (noise_remover.hpp:41-54). */
+typedef struct lpo_noise_config {
+    float radius_multiplier_m_per_m;
+    float min_search_radius_m;
+    uint32_t min_neighbours;
+} lpo_noise_config;
+void lpo_noise_config_default(lpo_noise_config* c);
+/* Search radius squared of a point (noise_remover.cpp:46-59). */
+float lpo_noise_radius_sqr(const lpo_noise_config* cfg, float x, float y);
+/* The filter's rule: labels[i] = 1 (NOISE) when fewer than min_neighbours
+ * points (i included) have dist_sqr <= r_i^2, else 0 (VALID).  counts
+ * (optional) receives the full in-radius count. */
+int lpo_noise_definition(const lpo_noise_config* cfg, const float* x, const float* y, const float* z, int32_t n,
+                         uint8_t* labels, uint32_t* counts);
+/* A persistent NoiseRemover with the KD-tree's search stack carried between
+ * queries and frames (kdtree.hpp:339-397). */
+typedef struct lpo_noise_remover lpo_noise_remover;
+lpo_noise_remover* lpo_noise_remover_create(void);
+void lpo_noise_remover_destroy(lpo_noise_remover* p);
+int lpo_noise_remover_filter(lpo_noise_remover* p, const lpo_noise_config* cfg, const float* x, const float* y,
+                             const float* z, int32_t n, uint8_t* labels);
+int32_t lpo_noise_remover_stack_depth(const lpo_noise_remover* p);
+
+/* BoundingBox (polygonizer.hpp:75-81): corners as x0,y0,..,x3,y3. */
+typedef struct lpo_bounding_box {
+    double corners[8];
+    float area;
+    float angle_rad;
+    int32_t is_valid;
+} lpo_bounding_box;
+/* findAntipodalPairsOfConvexHull: pair count (pairs[2k], pairs[2k+1]), -1 if the walk does not end. */
+int32_t lpo_antipodal_pairs(const double* x, const double* y, int32_t n, int32_t* pairs, int32_t cap);
+/* boundingBoxRotatingCalipers over the hull vertices in CCW order. */
+int lpo_obb_rotating_calipers(const double* x, const double* y, int32_t n, lpo_bounding_box* out);
+
 /* Library math helpers exposed for pinning tests. */
 float lpo_atan2_approx(float y, float x);
 /* First `count` draws of mt19937{42} + uniform_int_distribution<uint32_t>{0,n-1}. */

@@ -271,3 +271,113 @@ class Processor:
         return dict(status=st, seg_labels=bufs["seg"], obstacle_count=nobs.value,
                     cluster_labels=bufs["clab"][: nobs.value], num_clusters=k,
                     hull_offsets=bufs["hoff"][: k + 1], hull_indices=bufs["hidx"][: bufs["hoff"][k]])
+
+
+# ----------------------------------------------------------- DROR and OBB
+class NoiseConfig(ctypes.Structure):
+    """NoiseRemoverConfiguration (noise_remover.hpp:41-54)."""
+    _fields_ = [("radius_multiplier_m_per_m", ctypes.c_float), ("min_search_radius_m", ctypes.c_float),
+                ("min_neighbours", ctypes.c_uint32)]
+
+
+class BoundingBox(ctypes.Structure):
+    """BoundingBox (polygonizer.hpp:75-81), corners x0,y0,..,x3,y3."""
+    _fields_ = [("corners", ctypes.c_double * 8), ("area", ctypes.c_float), ("angle_rad", ctypes.c_float),
+                ("is_valid", ctypes.c_int32)]
+
+
+def _bind_ext():
+    L = lib()
+    if getattr(L, "_ext_bound", False):
+        return L
+    P = ctypes.c_void_p
+    L.lpo_noise_config_default.argtypes = [ctypes.POINTER(NoiseConfig)]
+    L.lpo_noise_radius_sqr.argtypes = [ctypes.POINTER(NoiseConfig), ctypes.c_float, ctypes.c_float]
+    L.lpo_noise_radius_sqr.restype = ctypes.c_float
+    L.lpo_noise_definition.argtypes = [ctypes.POINTER(NoiseConfig), P, P, P, ctypes.c_int32, P, P]
+    L.lpo_noise_remover_create.restype = P
+    L.lpo_noise_remover_destroy.argtypes = [P]
+    L.lpo_noise_remover_filter.argtypes = [P, ctypes.POINTER(NoiseConfig), P, P, P, ctypes.c_int32, P]
+    L.lpo_noise_remover_stack_depth.argtypes = [P]
+    L.lpo_antipodal_pairs.argtypes = [P, P, ctypes.c_int32, P, ctypes.c_int32]
+    L.lpo_antipodal_pairs.restype = ctypes.c_int32
+    L.lpo_obb_rotating_calipers.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(BoundingBox)]
+    L._ext_bound = True
+    return L
+
+
+def noise_config(**kw):
+    c = NoiseConfig()
+    _bind_ext().lpo_noise_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def noise_radius_sqr(x, y, cfg=None):
+    return _bind_ext().lpo_noise_radius_sqr(ctypes.byref(cfg or noise_config()), float(x), float(y))
+
+
+def noise_definition(x, y, z, cfg=None):
+    """The DROR rule: (labels u8 1 = NOISE, in-radius counts u32)."""
+    L = _bind_ext()
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    n = len(x)
+    labels = np.zeros(n, np.uint8)
+    counts = np.zeros(n, np.uint32)
+    st = L.lpo_noise_definition(ctypes.byref(cfg or noise_config()), _p(x), _p(y), _p(z), n, _p(labels),
+                                _p(counts))
+    if st != OK:
+        raise ValueError("noise_definition failed")
+    return labels, counts
+
+
+class NoiseRemover:
+    """NoiseRemover::filter as the reference object behaves, search stack
+    kept between queries and frames."""
+
+    def __init__(self):
+        self.L = _bind_ext()
+        self.h = self.L.lpo_noise_remover_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.lpo_noise_remover_destroy(self.h)
+            self.h = None
+
+    def filter(self, x, y, z, cfg=None):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        labels = np.zeros(len(x), np.uint8)
+        st = self.L.lpo_noise_remover_filter(self.h, ctypes.byref(cfg or noise_config()), _p(x), _p(y), _p(z),
+                                             len(x), _p(labels))
+        if st != OK:
+            raise ValueError("noise remover rejected the frame")
+        return labels
+
+    def stack_depth(self):
+        return self.L.lpo_noise_remover_stack_depth(self.h)
+
+
+def antipodal_pairs(x, y):
+    L = _bind_ext()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    cap = 4 * len(x) + 16
+    out = np.zeros(2 * cap, np.int32)
+    k = L.lpo_antipodal_pairs(_p(x), _p(y), len(x), _p(out), cap)
+    if k < 0:
+        raise ValueError("antipodal walk did not terminate")
+    return out[: 2 * k].reshape(-1, 2)
+
+
+def obb_rotating_calipers(x, y):
+    """boundingBoxRotatingCalipers over hull vertices (CCW).  Returns dict."""
+    L = _bind_ext()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    b = BoundingBox()
+    st = L.lpo_obb_rotating_calipers(_p(x), _p(y), len(x), ctypes.byref(b))
+    if st != OK:
+        raise ValueError("rotating calipers failed")
+    return dict(corners=np.array(b.corners[:], np.float64).reshape(4, 2), area=np.float32(b.area),
+                angle_rad=np.float32(b.angle_rad), is_valid=bool(b.is_valid))

@@ -1,0 +1,137 @@
+"""Noise filter (DROR) and oriented boxes on the GPU vs the oracle (GPU).
+
+Noise labels are bit-exact against the radius rule of NoiseRemover::filter
+(noise_remover.cpp:38-68); boxes are bit-exact (corners, float area, float
+angle, validity) against the restatement of boundingBoxRotatingCalipers
+(polygonizer.cpp:163-278).  The fused path (filter -> keep VALID in input
+order -> segment -> cluster -> hull -> box) is compared stage by stage.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from lidar_amd import native as N
+from lidar_amd import synth
+
+from tests.test_gpu_parity import cfgs, hull_coords
+from tests.test_oracle_dror_obb import cloud, convex_polygon, strictly_convex_ccw
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(max_frames=4, max_points=262144, max_height=128, max_width=2048)
+    yield c
+    c.close()
+
+
+def box_equal(a, b):
+    return (a["is_valid"] == b["is_valid"] and np.array_equal(a["corners"].view(np.uint64), b["corners"].view(np.uint64))
+            and a["area"].view(np.uint32) == b["area"].view(np.uint32)
+            and a["angle_rad"].view(np.uint32) == b["angle_rad"].view(np.uint32))
+
+
+@pytest.mark.parametrize("fi", [0, 1])
+def test_noise_labels_bit_exact_on_full_frames(ctx, oracle, fi):
+    f = synth.workload_c2(fi)
+    print(f"noise c2 frame {fi} n={f.n}", flush=True)
+    st, lab = ctx.remove_noise(f.x, f.y, f.z)
+    assert st == 0
+    ref, _ = oracle.noise_definition(f.x, f.y, f.z)
+    assert np.array_equal(lab, ref)
+    assert 1500 < lab.sum() < 4000
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_noise_labels_on_sparse_clouds_and_configs(ctx, oracle, seed):
+    x, y, z = cloud(seed)
+    for cfgkw in ({}, dict(min_neighbours=2), dict(min_neighbours=0), dict(min_search_radius_m=0.5),
+                  dict(radius_multiplier_m_per_m=0.05)):
+        st, lab = ctx.remove_noise(x, y, z, N.noise_config(**cfgkw))
+        assert st == 0
+        ref, _ = oracle.noise_definition(x, y, z, oracle.noise_config(**cfgkw))
+        assert np.array_equal(lab, ref), cfgkw
+
+
+def test_noise_edge_cases(ctx, oracle):
+    st, lab = ctx.remove_noise(np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(0, np.float32))
+    assert st == 0 and len(lab) == 0
+    one = np.array([3.0], np.float32)
+    st, lab = ctx.remove_noise(one, one, one)
+    assert st == 0 and lab.tolist() == [1]
+    # Far points (radius beyond the cell cube), duplicates and a NaN.
+    x = np.array([400.0, 400.0, 400.5, 401.0, 1e6, 0.0, 0.0, 0.0, 0.0, np.nan], np.float32)
+    y = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0], np.float32)
+    z = np.zeros(len(x), np.float32)
+    st, lab = ctx.remove_noise(x, y, z)
+    assert st == 0
+    ref, _ = oracle.noise_definition(x, y, z)
+    assert np.array_equal(lab, ref)
+
+
+def test_boxes_bit_exact_on_random_polygons(ctx, oracle):
+    rng = np.random.default_rng(5)
+    polys = [convex_polygon(rng, int(rng.integers(3, 80))) for _ in range(300)]
+    w, h, t = 4.5, 1.8, 0.7
+    loc = np.array([[-w / 2, -h / 2], [w / 2, -h / 2], [w / 2, h / 2], [-w / 2, h / 2]])
+    R = np.array([[math.cos(t), -math.sin(t)], [math.sin(t), math.cos(t)]])
+    P = loc @ R.T + [3.0, 4.0]
+    polys.append(strictly_convex_ccw(P[:, 0], P[:, 1]))
+    polys.append((np.array([0.0, 1.0]), np.array([0.0, 0.0])))  # fewer than 3 vertices: invalid
+    st, boxes = ctx.oriented_boxes(polys)
+    assert st == 0
+    for k, (px, py) in enumerate(polys):
+        ref = oracle.obb_rotating_calipers(px, py)
+        assert box_equal(boxes[k], ref), k
+
+
+def full_frame_reference(oracle, f, sco):
+    noise, _ = oracle.noise_definition(f.x, f.y, f.z)
+    keep = noise == 0
+    x, y, z, ring = f.x[keep], f.y[keep], f.z[keep], f.ring[keep]
+    ref = oracle.process_frame(x, y, z, ring, scfg=sco)
+    obs = np.nonzero(ref["seg_labels"] == 2)[0]
+    ox, oy = x[obs].astype(np.float64), y[obs].astype(np.float64)
+    hulls = hull_coords(ox, oy, ref["cluster_labels"], ref["num_clusters"], ref["hull_offsets"],
+                        ref["hull_indices"])
+    boxes = [oracle.obb_rotating_calipers(h[:, 0].copy(), h[:, 1].copy()) for h in hulls]
+    return noise, (x, y), ref, hulls, boxes
+
+
+def test_fused_filter_segment_cluster_hull_boxes(ctx, oracle):
+    frames = [synth.workload_c2(0), synth.workload_c2(3)]
+    ctx.set_noise_filter(N.noise_config())
+    ctx.set_oriented_boxes(True)
+    try:
+        for use_graph in (False, True, True):
+            outs = ctx.process([(f.x, f.y, f.z, f.ring) for f in frames], [cfgs(oracle, f)[1] for f in frames],
+                               use_graph=use_graph)
+            for f, out in zip(frames, outs):
+                sco, _ = cfgs(oracle, f)
+                noise, (x, y), ref, hulls, boxes = full_frame_reference(oracle, f, sco)
+                assert out["status"] == 0
+                assert np.array_equal(out["noise_labels"], noise)
+                assert out["num_points"] == len(x) and out["num_noise"] == int(noise.sum())
+                assert np.array_equal(out["seg_labels"], ref["seg_labels"])
+                assert np.array_equal(out["cluster_labels"], ref["cluster_labels"])
+                obs = out["obstacle_index"]
+                gh = hull_coords(x[obs].astype(np.float64), y[obs].astype(np.float64), out["cluster_labels"],
+                                 out["num_clusters"], out["hull_offsets"], out["hull_indices"])
+                assert len(gh) == len(hulls)
+                assert all(np.array_equal(a, b) for a, b in zip(gh, hulls))
+                assert len(out["boxes"]) == len(boxes)
+                assert all(box_equal(a, b) for a, b in zip(out["boxes"], boxes))
+    finally:
+        ctx.set_noise_filter(None)
+        ctx.set_oriented_boxes(False)
+
+
+def test_filter_off_again_restores_plain_path(ctx, oracle):
+    f = synth.workload_c1(0)
+    sco, scg = cfgs(oracle, f)
+    out = ctx.process([(f.x, f.y, f.z, f.ring)], [scg])[0]
+    ref = oracle.process_frame(f.x, f.y, f.z, f.ring, scfg=sco)
+    assert out["num_points"] == f.n and out["num_noise"] == 0 and out["boxes"] == []
+    assert np.array_equal(out["seg_labels"], ref["seg_labels"])
