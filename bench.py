@@ -2,8 +2,9 @@
 
 Workload: synthetic 64x2048 HDL-64E-like scans of the full single-frame
 pipeline configuration (40 cars, 60 poles, 2000 rain points, ~128k points),
-each frame through segmentation (projection, RECM, RANSAC, JCP), the
-obstacle split, curved-voxel clustering and per-cluster convex hulls.
+each frame through the DROR noise filter, segmentation (projection, RECM,
+RANSAC, JCP) of the kept points, the obstacle split, curved-voxel
+clustering, per-cluster convex hulls and a rotating-calipers box per hull.
 A step is one batch of B frames already resident in HBM, run as one
 captured hipGraph.  `value` is frames/s of the batched form; the plain
 per-frame path (B = 1, one graph per frame) and the host-CPU oracle are
@@ -33,18 +34,22 @@ def frames_c2(count, offset=0):
     return [synth.workload_c2(offset + i) for i in range(count)]
 
 
-def algorithmic_bytes(n, p, n_obs, voxels, hull_vertices):
-    """Per-frame algorithmic HBM bytes of the whole path: read xyz+ring (14 B
-    per point), write labels (4), range image write+read (10 B per pixel),
-    read obstacle xyz (12) + write cluster labels (4), voxel key+value (8 per
-    voxel), read hull input x,y (8 per obstacle point), write hull vertices
-    (16 per vertex)."""
-    return 14 * n + 4 * n + 10 * p + 12 * n_obs + 4 * n_obs + 8 * voxels + 8 * n_obs + 16 * hull_vertices
+def algorithmic_bytes(n_raw, n, p, n_obs, voxels, hull_vertices, clusters):
+    """Per-frame algorithmic HBM bytes of the whole path: noise filter (read
+    xyz 12 B + write a label 1 B per raw point, write the kept xyz+ring 14 B),
+    read xyz+ring (14 B per point), write labels (4), range image write+read
+    (10 B per pixel), read obstacle xyz (12) + write cluster labels (4), voxel
+    key+value (8 per voxel), read hull input x,y (8 per obstacle point), write
+    hull vertices (16 per vertex), read them for the box (16) and write a box
+    (80 per cluster)."""
+    return (13 * n_raw + 14 * n + 14 * n + 4 * n + 10 * p + 12 * n_obs + 4 * n_obs + 8 * voxels + 8 * n_obs
+            + 16 * hull_vertices + 16 * hull_vertices + 80 * clusters)
 
 
-def stage_bytes(stage, n, p, q, n_obs, voxels, hull_vertices, n_cand):
+def stage_bytes(stage, n, p, q, n_obs, voxels, hull_vertices, n_cand, n_raw=0, clusters=0):
     """Algorithmic bytes one launch of a stage moves (per frame)."""
     return {
+        "noise": 13 * n_raw + 14 * n,                # xyz in, label out, kept points out
         "project": 14 * n + 8 * n,                  # xyz+ring in, cell+pixel out
         "cells": 4 * n * 3 + 4 * 18000 * 2,          # scatter + sort of point indices, zmin out
         "recm": 4 * 18000 * 2,
@@ -58,6 +63,7 @@ def stage_bytes(stage, n, p, q, n_obs, voxels, hull_vertices, n_cand):
         "voxels": 12 * n_obs + 12 * n_obs + 8 * voxels,
         "components": 8 * voxels * 14 + 12 * n_obs,
         "hull": 8 * n_obs + 16 * hull_vertices,
+        "boxes": 16 * hull_vertices + 80 * clusters,
     }.get(stage, 0)
 
 
@@ -66,7 +72,10 @@ def run_gpu(args, frames):
     B = args.batch
     maxn = max(f.n for f in frames)
     scfg = N.segmenter_config(image_height=64, image_width=2048)
+    ncfg = N.noise_config()
     ctx = N.Context(max_frames=B, max_points=maxn, max_height=64, max_width=2048)
+    ctx.set_noise_filter(ncfg)
+    ctx.set_oriented_boxes(True)
     batch = [(f.x, f.y, f.z, f.ring) for f in frames[:B]]
     ctx.stage(batch, scfg)
     ctx.run(use_graph=False)
@@ -91,6 +100,8 @@ def run_gpu(args, frames):
 
     # Plain per-frame path: one frame, one graph launch, synchronous.
     ctx1 = N.Context(max_frames=1, max_points=maxn, max_height=64, max_width=2048)
+    ctx1.set_noise_filter(ncfg)
+    ctx1.set_oriented_boxes(True)
     lat = []
     lat_e2e = []
     for i in range(args.plain_frames):
@@ -120,20 +131,78 @@ def run_gpu(args, frames):
                                note="host arrays in, labels/clusters/hulls out (PCIe included)"))
 
 
+def run_batched(frames, steps, noise=False, boxes=False):
+    """Frames/s of one batch of `frames` (any sizes) replayed as a graph."""
+    from lidar_amd import native as N
+    maxn = max(f.n for f in frames)
+    H = max(f.height for f in frames)
+    ctx = N.Context(max_frames=len(frames), max_points=maxn, max_height=H, max_width=2048)
+    if noise:
+        ctx.set_noise_filter(N.noise_config())
+    ctx.set_oriented_boxes(boxes)
+    cfgs = [N.segmenter_config(image_height=f.height, image_width=f.width, elevation_up_deg=f.fov_up_deg,
+                               elevation_down_deg=f.fov_down_deg) for f in frames]
+    ctx.stage([(f.x, f.y, f.z, f.ring) for f in frames], cfgs)
+    ctx.run(use_graph=True)
+    outs = ctx.fetch()
+    bad = [o["status"] for o in outs if o["status"] != 0]
+    if bad:
+        raise SystemExit(f"batched frames failed: {bad}")
+    for _ in range(2):
+        ctx.run(use_graph=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run(use_graph=True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return dict(frames=len(frames), frames_per_s=round(len(frames) * steps / dt, 2),
+                points_per_s=round(sum(f.n for f in frames) * steps / dt, 1),
+                ms_per_batch=round(dt / steps * 1e3, 4))
+
+
+def run_serving(seconds):
+    """Eight simulated sensors (4x 32x1024, 3x 64x2048, 1x 128x2048) at 10 Hz
+    with +-5 ms jitter, served with continuous batching."""
+    from lidar_amd import native as N
+    from lidar_amd import serving
+    pools = serving.sensor_pools(pool_frames=3)
+    schedule = serving.arrival_schedule(seconds)
+    srv = N.Server(slots=3, max_batch_frames=8)
+    # Warm the per-batch-size graphs outside the measured schedule.
+    serving.serve(srv, pools, serving.arrival_schedule(0.3, rate_hz=200.0, seed=11))
+    before = srv.stats()
+    run = serving.serve(srv, pools, schedule)
+    st = srv.stats()
+    srv.close()
+    out = serving.summarize(schedule, run["latency_s"])
+    batches = st["batches"] - before["batches"]
+    out.update(wall_s=round(run["wall_s"], 3), frames_per_s=round(len(schedule) / run["wall_s"], 2),
+               slots=st["slots"], batches=int(batches), mean_batch=round(len(schedule) / max(batches, 1), 3),
+               slot_bytes=int(st["slot_bytes"]))
+    return out
+
+
 def run_cpu(frames, budget_s):
+    """The oracle's full frame with the reference NoiseRemover's KD-tree
+    search (its data structures and its stack behaviour) in front."""
     from oracle import lp_oracle as O
     proc = O.Processor(O.seg_config(), O.cluster_config())
+    nr = O.NoiseRemover()
     bufs = None
     times = []
     t_end = time.perf_counter() + budget_s
     k = 0
     while time.perf_counter() < t_end or k < 3:
         f = frames[k % len(frames)]
-        if bufs is None or len(bufs["seg"]) != f.n:
-            bufs = dict(seg=np.zeros(f.n, np.uint32), clab=np.zeros(f.n, np.int32),
-                        hoff=np.zeros(f.n + 2, np.int32), hidx=np.zeros(2 * f.n + 16, np.int32))
+        if bufs is None or len(bufs["seg"]) < f.n:
+            n = max(ff.n for ff in frames)
+            bufs = dict(noise=np.zeros(n, np.uint8), seg=np.zeros(n, np.uint32), clab=np.zeros(n, np.int32),
+                        hoff=np.zeros(n + 2, np.int32), hidx=np.zeros(2 * n + 16, np.int32),
+                        boxes=(O.BoundingBox * (n + 2))())
         t = time.perf_counter()
-        r = proc.frame(f.x, f.y, f.z, f.ring, bufs)
+        r = O.full_frame(proc, f.x, f.y, f.z, f.ring, noise_remover=nr, bufs=bufs)
         times.append(time.perf_counter() - t)
         assert r["status"] == 0
         k += 1
@@ -151,6 +220,8 @@ def main():
     ap.add_argument("--plain-frames", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the c3/c4 batches and the serving run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard across GPUs "
@@ -160,6 +231,16 @@ def main():
     frames = frames_c2(args.batch)
     log("gpu")
     g = run_gpu(args, frames)
+    extra = {}
+    if not args.no_extra:
+        from lidar_amd import synth
+        log("batched c3 (mixed widths)")
+        extra["batched_c3"] = run_batched([synth.workload_c3(i) for i in range(32)], 5)
+        log("batched c4 (128x2048)")
+        extra["batched_c4"] = run_batched([synth.workload_c4(i) for i in range(16)], 5)
+        if args.serve_seconds > 0:
+            log("serving")
+            extra["serving"] = run_serving(args.serve_seconds)
     cpu = None
     if not args.no_cpu:
         log("cpu oracle")
@@ -169,8 +250,10 @@ def main():
     outs = g["outs"]
     B = args.batch
     mean = lambda key: float(np.mean([o[key] if not isinstance(o[key], np.ndarray) else len(o[key]) for o in outs]))
-    n = float(np.mean([f.n for f in frames[:B]]))
+    n_raw = float(np.mean([f.n for f in frames[:B]]))
+    n = float(np.mean([o["num_points"] for o in outs]))
     n_obs = float(np.mean([len(o["obstacle_index"]) for o in outs]))
+    ncl = mean("num_clusters")
     vox = mean("num_voxels")
     hv = float(np.mean([len(o["hull_indices"]) for o in outs]))
     q = mean("jcp_queue")
@@ -178,7 +261,7 @@ def main():
     P = 64 * 2048
     stages = [(s, ms) for s, ms in g["stages"] if s not in ("clear",)]
     dom, dom_ms = max(stages, key=lambda t: t[1])
-    dom_bytes = stage_bytes(dom, n, P, q, n_obs, vox, hv, ncand) * B
+    dom_bytes = stage_bytes(dom, n, P, q, n_obs, vox, hv, ncand, n_raw, ncl) * B
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -187,7 +270,7 @@ def main():
             traffic = json.load(open(tpath)).get(dom)
         except (OSError, ValueError):
             traffic = None
-    alg_frame = algorithmic_bytes(n, P, n_obs, vox, hv)
+    alg_frame = algorithmic_bytes(n_raw, n, P, n_obs, vox, hv, ncl)
     line = {
         "metric": METRIC,
         "value": round(g["fps"], 2),
@@ -203,9 +286,10 @@ def main():
         "data": "synthetic (seeded 64x2048 scans generated on the host, resident in HBM)",
         "config": {
             "workload": "64x2048 full single-frame pipeline: 40 cars, 60 poles, 2000 rain points "
-                        "(~128k points), segment + split + cluster + hull, batch of %d frames per step" % B,
+                        "(~128k points), DROR filter + segment + split + cluster + hull + rotating-calipers box, "
+                        "batch of %d frames per step" % B,
             "batch_frames": B,
-            "points_per_frame": round(n),
+            "points_per_frame": round(n_raw),
             "form": "batched (CSR-packed frames, one hipGraph per step)",
         },
         "points_per_s": round(g["pts"], 1),
@@ -230,10 +314,12 @@ def main():
             "frac": round(alg_frame * g["fps"] / 1e9 / HBM_PEAK_GBS, 6),
         },
     }
+    line.update(extra)
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["fps"], 3), "unit": "frames/s", "cores": 1, "kind": "port",
                                 "sample": f"{cpu['frames']} frames of the same workload (8 distinct seeds) through "
-                                          "the oracle's segment+split+cluster+gather+hull, single thread",
+                                          "the oracle's KD-tree DROR + segment+split+cluster+gather+hull+box, "
+                                          "single thread",
                                 "points_per_s": round(cpu["pts"], 1), "p50_ms": round(cpu["p50_ms"], 3)}
         line["speedup_vs_cpu"] = {"batched": round(g["fps"] / cpu["fps"], 2),
                                   "plain": round(g["plain"]["frames_per_s"] / cpu["fps"], 2)}

@@ -235,6 +235,50 @@ int lp_synchronize(lp_context* ctx);
  * it is still running, negative on error. */
 int lp_query(lp_context* ctx);
 
+/* Serving front end for several sensors.  No reference counterpart: the
+ * reference processor handles one frame per point-cloud callback on a
+ * single-threaded executor (src/processor/src/processor.cpp:878-894); the
+ * server feeds that per-frame path from many sensors with continuous
+ * batching: frames queue as they arrive and, whenever one of `slots`
+ * contexts (one HIP stream each) is free, up to max_batch_frames queued
+ * frames of any sizes run on it as one batch, overlapping the batches of the
+ * other slots.  The slots created are those whose preallocated buffers fit
+ * hbm_budget_bytes.  A frame's outputs are the same alone and in any mix. */
+typedef struct lp_server_config {
+    int32_t slots;                 /* contexts (batches in flight at once) */
+    int32_t max_batch_frames;      /* frames one slot runs as one batch */
+    lp_capacity capacity;          /* per-frame limits of every slot (max_frames is ignored) */
+    uint64_t hbm_budget_bytes;     /* device memory the slots may hold; 0 = no limit */
+    lp_clusterer_config cluster;
+    int32_t noise_filter;          /* 1: noise filter (below) ahead of the segmenter */
+    lp_noise_config noise;
+    int32_t oriented_boxes;        /* 1: a rotating-calipers box per cluster hull */
+} lp_server_config;
+void lp_server_config_default(lp_server_config* cfg);
+
+typedef struct lp_server lp_server;
+int lp_server_create(const lp_server_config* cfg, lp_server** out);
+int lp_server_destroy(lp_server* s);
+/* Queues one frame.  cfg is copied; the cloud's memory, the outputs and the
+ * result must stay valid until lp_server_poll returns the frame's tag. */
+int lp_server_submit(lp_server* s, const lp_segmenter_config* cfg, const lp_cloud* cloud,
+                     const lp_frame_outputs* outputs, lp_frame_result* result, uint64_t tag);
+/* Non-blocking: collects finished batches (writing outputs and results),
+ * starts queued frames on free slots, and writes up to max_done tags of
+ * completed frames to done.  Returns the number written, or an error. */
+int lp_server_poll(lp_server* s, uint64_t* done, int32_t max_done);
+
+typedef struct lp_server_stats {
+    int32_t slots;
+    int32_t busy_slots;
+    int32_t queued;
+    int32_t in_flight;
+    uint64_t slot_bytes;           /* device bytes held by each slot */
+    uint64_t batches;
+    uint64_t frames;
+} lp_server_stats;
+int lp_server_stats_get(lp_server* s, lp_server_stats* st);
+
 /* Timing of the dominant kernel for roofline reporting: average duration (ms)
  * of the last `runs` launches of the named stage, from HIP events recorded on
  * the context stream. */

@@ -398,8 +398,10 @@ int lp_context_create(const lp_capacity* cap, lp_context** out) {
 int lp_context_destroy(lp_context* c) {
     if (!c) return LP_OK;
     (void)hipSetDevice(c->device);
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto& g : c->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->arena) (void)hipFree(c->arena);
@@ -472,23 +474,30 @@ int lp_batch_run(lp_context* c, int32_t use_graph) {
     const int B = c->num_frames;
     if (!use_graph) return run_pipeline(c, B, false);
     const int options = (c->staged_filtered ? 1 : 0) | (c->boxes_on ? 2 : 0);
-    if (!c->graph_exec || c->graph_frames != B || c->graph_options != options) {
-        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-        if (c->graph) (void)hipGraphDestroy(c->graph);
-        c->graph_exec = nullptr;
-        c->graph = nullptr;
+    lp_context::GraphEntry* ge = nullptr;
+    for (auto& g : c->graphs)
+        if (g.frames == B && g.options == options) ge = &g;
+    if (!ge) {
         LP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int st = run_pipeline(c, B, false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (st != LP_OK) return st;
+        if (st != LP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
         if (e != hipSuccess) return fail(LP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        c->graph = g;
-        LP_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
-        c->graph_frames = B;
-        c->graph_options = options;
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(LP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        c->graphs.push_back({B, options, g, ex});
+        ge = &c->graphs.back();
     }
-    LP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    c->L.B_active = B;
+    LP_HIP(hipGraphLaunch(ge->exec, c->stream));
     return LP_OK;
 }
 

@@ -228,11 +228,17 @@ struct lp_context {
     // pinned host buffers for inputs / outputs
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
-    // graph per batch shape
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int32_t graph_frames = -1;
-    int32_t graph_options = -1;     // noise/boxes switches the graph was captured with
+    // One captured graph per (batch size, optional-stage switches): every
+    // kernel reads its frame descriptors from device memory and is launched
+    // with capacity-sized grids, so a graph replays for any frames of that
+    // batch size.
+    struct GraphEntry {
+        int32_t frames;
+        int32_t options;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
     // stage timing
     std::vector<hipEvent_t> events;
     std::vector<std::string> stage_names;

@@ -209,11 +209,13 @@ constexpr int kSolveThreads = 256;
 constexpr int kSolveWaves = kSolveThreads / kWave;
 constexpr int kFinLds = 65536;
 
+// wg += w for a GROUND slot, wo += w for an OBSTACLE slot (segmenter.cpp:
+// 613-626).  Written as two selects: every weight is +0 or positive and the
+// sums start at +0, so adding +0.0f to the other sum leaves it bit-identical,
+// and the select form keeps both sums in registers.
 __device__ __forceinline__ void add_slot(uint32_t l, float w, float& wg, float& wo) {
-    if (l == kSG)
-        wg += w;
-    else if (l == kSO)
-        wo += w;
+    wg += (l == kSG) ? w : 0.0f;
+    wo += (l == kSO) ? w : 0.0f;
 }
 
 // Transfer map of queue pixel j (pair state (state(j-2), state(j-1)) ->

@@ -28,6 +28,8 @@ EXPORTED = [
     "lp_synchronize", "lp_stage_names", "lp_profile_run", "lp_context_stream", "lp_debug_fetch",
     "lp_math_eval", "lp_math_check_range", "lp_add_ring_info", "lp_query", "lp_noise_config_default",
     "lp_remove_noise", "lp_oriented_boxes", "lp_set_noise_filter", "lp_set_oriented_boxes",
+    "lp_server_config_default", "lp_server_create", "lp_server_destroy", "lp_server_submit", "lp_server_poll",
+    "lp_server_stats_get",
 ]
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
@@ -167,6 +169,31 @@ class FrameResult(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32),
+        ("max_batch_frames", ctypes.c_int32),
+        ("capacity", Capacity),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("cluster", ClustererConfig),
+        ("noise_filter", ctypes.c_int32),
+        ("noise", NoiseConfig),
+        ("oriented_boxes", ctypes.c_int32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32),
+        ("busy_slots", ctypes.c_int32),
+        ("queued", ctypes.c_int32),
+        ("in_flight", ctypes.c_int32),
+        ("slot_bytes", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("frames", ctypes.c_uint64),
+    ]
+
+
 def library_path():
     return os.path.join(_HERE, "liblp_amd.so")
 
@@ -210,6 +237,14 @@ def lib():
         L.lp_oriented_boxes.argtypes = [P, P, P, P, I, ctypes.POINTER(BoundingBox)]
         L.lp_set_noise_filter.argtypes = [P, ctypes.POINTER(NoiseConfig)]
         L.lp_set_oriented_boxes.argtypes = [P, I]
+        L.lp_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+        L.lp_server_config_default.restype = None
+        L.lp_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
+        L.lp_server_destroy.argtypes = [P]
+        L.lp_server_submit.argtypes = [P, ctypes.POINTER(SegmenterConfig), ctypes.POINTER(Cloud),
+                                       ctypes.POINTER(FrameOutputs), ctypes.POINTER(FrameResult), ctypes.c_uint64]
+        L.lp_server_poll.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), I]
+        L.lp_server_stats_get.argtypes = [P, ctypes.POINTER(ServerStats)]
         L.lp_stage_names.argtypes = [P, ctypes.c_char_p, I]
         L.lp_profile_run.argtypes = [P, I, P, I, ctypes.POINTER(I)]
         L.lp_context_stream.argtypes = [P]
@@ -501,3 +536,94 @@ class Context:
         lib().lp_stage_names(self.h, buf, 4096)
         names = buf.value.decode().split(",")
         return list(zip(names, ms[: ns.value].tolist()))
+
+
+class FrameSlot:
+    """Host buffers of one in-flight frame (inputs kept alive, outputs written
+    by the server)."""
+
+    def __init__(self, max_points, boxes=False):
+        n = max(max_points, 1)
+        self.seg = np.zeros(n, np.uint32)
+        self.obs = np.zeros(n, np.int32)
+        self.clab = np.zeros(n, np.int32)
+        self.hoff = np.zeros(n + 2, np.int32)
+        self.hidx = np.zeros(2 * n + 16, np.int32)
+        self.noise = np.zeros(n, np.uint8)
+        self.boxes = (BoundingBox * (n if boxes else 1))()
+        self.out = FrameOutputs()
+        self.out.seg_labels = self.seg.ctypes.data
+        self.out.obstacle_index = self.obs.ctypes.data
+        self.out.cluster_labels = self.clab.ctypes.data
+        self.out.hull_offsets = self.hoff.ctypes.data
+        self.out.hull_indices = self.hidx.ctypes.data
+        self.out.hull_capacity = len(self.hidx)
+        self.out.noise_labels = self.noise.ctypes.data
+        self.out.boxes = ctypes.addressof(self.boxes)
+        self.res = FrameResult()
+        self.cloud = None
+        self.cfg = None
+        self.inputs = None
+
+    def result(self, boxes=False):
+        r = self.res
+        no, K, V = r.num_obstacles, r.num_clusters, r.num_hull_vertices
+        return dict(status=r.status, num_points=r.num_points, num_noise=r.num_noise,
+                    seg_labels=self.seg[:r.num_points].copy(), obstacle_index=self.obs[:no].copy(),
+                    cluster_labels=self.clab[:no].copy(), num_clusters=K, hull_offsets=self.hoff[:K + 1].copy(),
+                    hull_indices=self.hidx[:V].copy(), boxes=[box_dict(self.boxes[k]) for k in range(K)] if boxes
+                    else [], num_voxels=r.num_voxels)
+
+
+class Server:
+    """lp_server: continuous batching over several contexts."""
+
+    def __init__(self, slots=3, max_batch_frames=8, max_points=262144, max_height=128, max_width=2048,
+                 hbm_budget_bytes=0, noise_cfg=None, boxes=False, ccfg=None):
+        cfg = ServerConfig()
+        lib().lp_server_config_default(ctypes.byref(cfg))
+        cfg.slots = slots
+        cfg.max_batch_frames = max_batch_frames
+        cfg.capacity.max_points_per_frame = max_points
+        cfg.capacity.max_image_height = max_height
+        cfg.capacity.max_image_width = max_width
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        if ccfg is not None:
+            cfg.cluster = ccfg
+        if noise_cfg is not None:
+            cfg.noise_filter = 1
+            cfg.noise = noise_cfg
+        cfg.oriented_boxes = 1 if boxes else 0
+        h = ctypes.c_void_p()
+        check(lib().lp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.boxes = boxes
+        self._done = (ctypes.c_uint64 * 1024)()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, slot, x, y, z, ring, seg_cfg, tag):
+        """Queue one frame; `slot` (a FrameSlot) keeps the inputs alive and
+        receives the outputs."""
+        slot.inputs = (x, y, z, ring)
+        slot.cloud = soa_cloud(x, y, z, ring)
+        slot.cfg = seg_cfg
+        check(lib().lp_server_submit(self.h, ctypes.byref(seg_cfg), ctypes.byref(slot.cloud),
+                                     ctypes.byref(slot.out), ctypes.byref(slot.res), tag))
+
+    def poll(self):
+        n = lib().lp_server_poll(self.h, self._done, len(self._done))
+        if n < 0:
+            check(n)
+        return [self._done[k] for k in range(n)]
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().lp_server_stats_get(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

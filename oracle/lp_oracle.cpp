@@ -700,8 +700,10 @@ struct lpo_processor {
     Seg seg;
     Clu clu;
     std::vector<float> ox, oy, oz;
-    std::vector<double> gx, gy;
+    std::vector<double> gx, gy, px, py;
     std::vector<int32_t> sorted, out;
+    std::vector<float> kx, ky, kz;
+    std::vector<uint16_t> kring;
     lpo_processor(const lpo_seg_config& s, const lpo_cluster_config& c, int32_t max_voxels)
         : seg(s), clu(c, max_voxels) {}
 };
@@ -731,10 +733,10 @@ int lpo_processor_cluster(lpo_processor* p, const float* x, const float* y, cons
     return p->clu.cluster(x, y, z, n, labels);
 }
 
-int lpo_processor_frame(lpo_processor* p, const float* x, const float* y, const float* z,
-                        const uint16_t* ring, int32_t n, uint32_t* seg_labels, int32_t* obstacle_count,
-                        int32_t* cluster_labels, int32_t* num_clusters, int32_t* hull_offsets,
-                        int32_t hull_offsets_cap, int32_t* hull_indices, int32_t hull_cap) {
+static int frame_impl(lpo_processor* p, const float* x, const float* y, const float* z, const uint16_t* ring,
+                      int32_t n, uint32_t* seg_labels, int32_t* obstacle_count, int32_t* cluster_labels,
+                      int32_t* num_clusters, int32_t* hull_offsets, int32_t hull_offsets_cap, int32_t* hull_indices,
+                      int32_t hull_cap, lpo_bounding_box* boxes) {
     if (p == nullptr || n < 0) return LPO_ERR_INVALID;
     int st = p->seg.segment(x, y, z, ring, n, seg_labels, nullptr);
     if (st != LPO_OK) return st;
@@ -777,8 +779,54 @@ int lpo_processor_frame(lpo_processor* p, const float* x, const float* y, const 
         std::memcpy(hull_indices + total, p->out.data(), sizeof(int32_t) * k);
         total += k;
         hull_offsets[l + 1] = total;
+        if (boxes) {
+            // processor.cpp:663-704: the polygon of the hull vertices, then
+            // boundingBoxRotatingCalipers on it.
+            p->px.clear();
+            p->py.clear();
+            for (int32_t v = 0; v < k; ++v) {
+                p->px.push_back(p->gx[p->out[v]]);
+                p->py.push_back(p->gy[p->out[v]]);
+            }
+            const int bs = lpo_obb_rotating_calipers(p->px.data(), p->py.data(), k, &boxes[l]);
+            if (bs != LPO_OK) return bs;
+        }
     }
     return LPO_OK;
+}
+
+int lpo_processor_frame(lpo_processor* p, const float* x, const float* y, const float* z,
+                        const uint16_t* ring, int32_t n, uint32_t* seg_labels, int32_t* obstacle_count,
+                        int32_t* cluster_labels, int32_t* num_clusters, int32_t* hull_offsets,
+                        int32_t hull_offsets_cap, int32_t* hull_indices, int32_t hull_cap) {
+    return frame_impl(p, x, y, z, ring, n, seg_labels, obstacle_count, cluster_labels, num_clusters, hull_offsets,
+                      hull_offsets_cap, hull_indices, hull_cap, nullptr);
+}
+
+int lpo_full_frame(lpo_processor* p, lpo_noise_remover* nr, const lpo_noise_config* ncfg, const float* x,
+                   const float* y, const float* z, const uint16_t* ring, int32_t n, uint8_t* noise_labels,
+                   int32_t* kept, uint32_t* seg_labels, int32_t* obstacle_count, int32_t* cluster_labels,
+                   int32_t* num_clusters, int32_t* hull_offsets, int32_t hull_offsets_cap, int32_t* hull_indices,
+                   int32_t hull_cap, lpo_bounding_box* boxes) {
+    if (p == nullptr || ncfg == nullptr || n < 0) return LPO_ERR_INVALID;
+    const int st = nr ? lpo_noise_remover_filter(nr, ncfg, x, y, z, n, noise_labels)
+                      : lpo_noise_definition(ncfg, x, y, z, n, noise_labels, nullptr);
+    if (st != LPO_OK) return st;
+    p->kx.clear();
+    p->ky.clear();
+    p->kz.clear();
+    p->kring.clear();
+    for (int32_t i = 0; i < n; ++i)
+        if (noise_labels[i] == 0) {
+            p->kx.push_back(x[i]);
+            p->ky.push_back(y[i]);
+            p->kz.push_back(z[i]);
+            if (ring) p->kring.push_back(ring[i]);
+        }
+    *kept = static_cast<int32_t>(p->kx.size());
+    return frame_impl(p, p->kx.data(), p->ky.data(), p->kz.data(), ring ? p->kring.data() : nullptr, *kept,
+                      seg_labels, obstacle_count, cluster_labels, num_clusters, hull_offsets, hull_offsets_cap,
+                      hull_indices, hull_cap, boxes);
 }
 
 int lpo_segment(const lpo_seg_config* cfg, const float* x, const float* y, const float* z,

@@ -14,6 +14,8 @@
  *   lidar_processing_lib/src/polygonizer.cpp:33-91     (convexHull)
  *   src/processor/src/processor.cpp:552-670            (split / gather / hull glue)
  *   src/dataloader/src/dataloader.cpp:68-137           (addRingInfo)
+ *   lidar_processing_lib/src/noise_remover.cpp:38-68 + kdtree.hpp:131-397 (NoiseRemover, DROR)
+ *   lidar_processing_lib/src/polygonizer.cpp:93-278    (antipodal pairs, rotating-calipers box)
  * and mirrors the reference's data-structure choices (vector-of-vector polar
  * grid, per-cell std::sort, FIFO JCP queue, linear-probing voxel map, BFS with
  * a visited set, std::sort hulls) so that its speed is representative.
@@ -157,6 +159,16 @@ typedef struct lpo_bounding_box {
 int32_t lpo_antipodal_pairs(const double* x, const double* y, int32_t n, int32_t* pairs, int32_t cap);
 /* boundingBoxRotatingCalipers over the hull vertices in CCW order. */
 int lpo_obb_rotating_calipers(const double* x, const double* y, int32_t n, lpo_bounding_box* out);
+
+/* The full single-frame pipeline: noise filter (the reference object when nr
+ * is given, else the radius rule) -> keep VALID in input order -> segment ->
+ * split -> cluster -> gather -> hull -> boundingBoxRotatingCalipers on each
+ * hull polygon (boxes[num_clusters], may be NULL).  *kept = points kept. */
+int lpo_full_frame(lpo_processor* p, lpo_noise_remover* nr, const lpo_noise_config* ncfg, const float* x,
+                   const float* y, const float* z, const uint16_t* ring, int32_t n, uint8_t* noise_labels,
+                   int32_t* kept, uint32_t* seg_labels, int32_t* obstacle_count, int32_t* cluster_labels,
+                   int32_t* num_clusters, int32_t* hull_offsets, int32_t hull_offsets_cap, int32_t* hull_indices,
+                   int32_t hull_cap, lpo_bounding_box* boxes);
 
 /* Library math helpers exposed for pinning tests. */
 float lpo_atan2_approx(float y, float x);

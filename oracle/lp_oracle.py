@@ -381,3 +381,39 @@ def obb_rotating_calipers(x, y):
         raise ValueError("rotating calipers failed")
     return dict(corners=np.array(b.corners[:], np.float64).reshape(4, 2), area=np.float32(b.area),
                 angle_rad=np.float32(b.angle_rad), is_valid=bool(b.is_valid))
+
+
+def _bind_full():
+    L = _bind_ext()
+    if not getattr(L, "_full_bound", False):
+        P = ctypes.c_void_p
+        L.lpo_full_frame.argtypes = [P, P, ctypes.POINTER(NoiseConfig), P, P, P, P, ctypes.c_int32, P,
+                                     ctypes.POINTER(ctypes.c_int32), P, ctypes.POINTER(ctypes.c_int32), P,
+                                     ctypes.POINTER(ctypes.c_int32), P, ctypes.c_int32, P, ctypes.c_int32, P]
+        L._full_bound = True
+    return L
+
+
+def full_frame(proc, x, y, z, ring=None, noise_remover=None, ncfg=None, bufs=None):
+    """Noise filter -> keep VALID -> segment -> split -> cluster -> gather ->
+    hull -> box per hull, on a Processor.  noise_remover: a NoiseRemover (the
+    reference object's behaviour) or None (the radius rule)."""
+    L = _bind_full()
+    n = len(x)
+    if bufs is None:
+        bufs = dict(noise=np.zeros(max(n, 1), np.uint8), seg=np.zeros(max(n, 1), np.uint32),
+                    clab=np.zeros(max(n, 1), np.int32), hoff=np.zeros(n + 2, np.int32),
+                    hidx=np.zeros(2 * n + 16, np.int32), boxes=(BoundingBox * (n + 2))())
+    kept = ctypes.c_int32(0)
+    nobs = ctypes.c_int32(0)
+    ncl = ctypes.c_int32(0)
+    st = L.lpo_full_frame(proc.h, noise_remover.h if noise_remover is not None else None,
+                          ctypes.byref(ncfg or noise_config()), _p(x), _p(y), _p(z), _p(ring), n, _p(bufs["noise"]),
+                          ctypes.byref(kept), _p(bufs["seg"]), ctypes.byref(nobs), _p(bufs["clab"]),
+                          ctypes.byref(ncl), _p(bufs["hoff"]), len(bufs["hoff"]), _p(bufs["hidx"]),
+                          len(bufs["hidx"]), bufs["boxes"])
+    k = ncl.value
+    return dict(status=st, noise_labels=bufs["noise"][:n], num_points=kept.value,
+                seg_labels=bufs["seg"][:kept.value], obstacle_count=nobs.value,
+                cluster_labels=bufs["clab"][: nobs.value], num_clusters=k, hull_offsets=bufs["hoff"][: k + 1],
+                hull_indices=bufs["hidx"][: bufs["hoff"][k]], boxes=bufs["boxes"])

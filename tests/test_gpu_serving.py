@@ -1,0 +1,72 @@
+"""Continuous-batching server (GPU): every frame's outputs are the same alone
+and in the mix, HBM-budget admission, capacity checks."""
+import numpy as np
+import pytest
+
+from lidar_amd import native as N
+from lidar_amd import serving
+
+from tests.test_gpu_noise_boxes import box_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(pools, noise, boxes):
+    """Each pool frame processed on its own context, one frame per batch."""
+    ref = {}
+    with N.Context(max_frames=1, max_points=262144, max_height=128, max_width=2048) as ctx:
+        if noise:
+            ctx.set_noise_filter(N.noise_config())
+        ctx.set_oriented_boxes(boxes)
+        for s, p in enumerate(pools):
+            for k, f in enumerate(p["frames"]):
+                ref[s, k] = ctx.process([(f.x, f.y, f.z, f.ring)], p["cfg"])[0]
+    return ref
+
+
+@pytest.mark.parametrize("noise,boxes", [(False, False), (True, True)])
+def test_alone_equals_in_mix(noise, boxes):
+    pools = serving.sensor_pools(pool_frames=2)
+    ref = alone(pools, noise, boxes)
+    # 200 Hz per sensor for 0.4 s: frames pile up while batches run, so the
+    # slots take mixed batches of all three sensor geometries.
+    schedule = serving.arrival_schedule(0.4, rate_hz=200.0, jitter_ms=2.0, seed=3)
+    srv = N.Server(slots=3, max_batch_frames=8, noise_cfg=N.noise_config() if noise else None, boxes=boxes)
+    print(f"serving {len(schedule)} frames noise={noise} boxes={boxes}", flush=True)
+    out = serving.serve(srv, pools, schedule, keep_results=True)
+    st = srv.stats()
+    srv.close()
+    assert st["frames"] == len(schedule) and st["batches"] < len(schedule)  # some batches held several frames
+    for tag, (_, s, k) in enumerate(schedule):
+        got = out["results"][tag]
+        want = ref[s, k % len(pools[s]["frames"])]
+        assert got["status"] == 0
+        for key in ("seg_labels", "obstacle_index", "cluster_labels", "hull_offsets", "hull_indices"):
+            assert np.array_equal(got[key], want[key]), (tag, key)
+        assert got["num_points"] == want["num_points"]
+        if boxes:
+            assert len(got["boxes"]) == len(want["boxes"])
+            assert all(box_equal(a, b) for a, b in zip(got["boxes"], want["boxes"]))
+
+
+def test_hbm_budget_admission():
+    srv = N.Server(slots=4, max_batch_frames=2)
+    per_slot = srv.stats()["slot_bytes"]
+    assert srv.stats()["slots"] == 4
+    srv.close()
+    srv = N.Server(slots=4, max_batch_frames=2, hbm_budget_bytes=int(per_slot * 2.5))
+    assert srv.stats()["slots"] == 2
+    srv.close()
+    with pytest.raises(N.LidarError):
+        N.Server(slots=4, max_batch_frames=2, hbm_budget_bytes=per_slot // 2)
+
+
+def test_oversized_frame_rejected_at_submit():
+    srv = N.Server(slots=1, max_batch_frames=1, max_points=1000, max_height=64, max_width=2048)
+    f = serving.sensor_pools(pool_frames=1, sensors=[64])[0]
+    x = f["frames"][0]
+    slot = N.FrameSlot(x.n)
+    with pytest.raises(N.LidarError):
+        srv.submit(slot, x.x, x.y, x.z, x.ring, f["cfg"], 0)
+    assert srv.poll() == []
+    srv.close()
