@@ -16,6 +16,7 @@
 #pragma once
 
 #include <cstddef>
+#include <array>
 #include <cstdint>
 #include <memory>
 #include <numeric>
@@ -63,6 +64,23 @@ struct ClustererConfiguration {
 struct PointXY {
     double x;
     double y;
+};
+
+// BoundingBox (polygonizer.hpp:75-81).
+struct BoundingBox {
+    std::array<PointXY, 4> corners;
+    float area;
+    float angle_rad;
+    bool is_valid;
+};
+
+// NoiseRemoverConfiguration / NoiseRemoverLabel (noise_remover.hpp:35-54).
+enum class NoiseRemoverLabel : std::uint8_t { VALID = 0, NOISE = 1 };
+
+struct NoiseRemoverConfiguration {
+    float radius_multiplier_m_per_m = 0.02F;
+    float min_search_radius_m = 0.1F;
+    std::uint32_t min_neighbours = 4U;
 };
 
 namespace detail {
@@ -233,8 +251,79 @@ class Polygonizer {
             indices[k].assign(hidx.begin() + hoff[k], hidx.begin() + hoff[k + 1]);
     }
 
+    // Oriented box of a convex polygon, vertices counter-clockwise
+    // (polygonizer.cpp:163-278).
+    BoundingBox boundingBoxRotatingCalipers(const std::vector<PointXY>& convex_hull_points) {
+        std::vector<BoundingBox> out;
+        boundingBoxesRotatingCalipers(std::vector<const std::vector<PointXY>*>{&convex_hull_points}, out);
+        return out[0];
+    }
+
+    // Many polygons in one call.
+    void boundingBoxesRotatingCalipers(const std::vector<const std::vector<PointXY>*>& polygons,
+                                       std::vector<BoundingBox>& boxes) {
+        std::vector<double> x, y;
+        std::vector<std::int32_t> offsets(1, 0);
+        for (const auto* poly : polygons) {
+            for (const auto& p : *poly) {
+                x.push_back(p.x);
+                y.push_back(p.y);
+            }
+            offsets.push_back(static_cast<std::int32_t>(x.size()));
+        }
+        std::vector<lp_bounding_box> raw(polygons.size());
+        detail::check(lp_oriented_boxes(ctx_->get(), x.data(), y.data(), offsets.data(),
+                                        static_cast<std::int32_t>(polygons.size()), raw.data()));
+        boxes.resize(polygons.size());
+        for (std::size_t k = 0; k < polygons.size(); ++k) {
+            for (int c = 0; c < 4; ++c) boxes[k].corners[c] = {raw[k].corners[2 * c], raw[k].corners[2 * c + 1]};
+            boxes[k].area = raw[k].area;
+            boxes[k].angle_rad = raw[k].angle_rad;
+            boxes[k].is_valid = raw[k].is_valid != 0;
+        }
+    }
+
   private:
     std::unique_ptr<detail::Context> ctx_;
+};
+
+// NoiseRemover::filter (noise_remover.cpp:38-68) over an array of xyz points
+// (KDTree<float, 3>::PointT, i.e. std::array<float, 3>).
+class NoiseRemover {
+  public:
+    using PointT = std::array<float, 3>;
+
+    NoiseRemover() : ctx_(new detail::Context()) {}
+
+    void reserve(std::uint32_t max_pts) {
+        if (max_pts > capacity_) {
+            ctx_.reset(new detail::Context(static_cast<std::int32_t>(max_pts)));
+            capacity_ = max_pts;
+        }
+    }
+
+    void config(const NoiseRemoverConfiguration& config) { config_ = config; }
+    const NoiseRemoverConfiguration& config() const noexcept { return config_; }
+
+    void filter(const std::vector<PointT>& points, std::vector<NoiseRemoverLabel>& labels) {
+        labels.assign(points.size(), NoiseRemoverLabel::VALID);
+        if (points.empty()) return;
+        reserve(static_cast<std::uint32_t>(points.size()));
+        lp_noise_config c{config_.radius_multiplier_m_per_m, config_.min_search_radius_m, config_.min_neighbours};
+        lp_cloud cloud{};
+        cloud.x = &points[0][0];
+        cloud.y = &points[0][1];
+        cloud.z = &points[0][2];
+        cloud.xyz_stride = static_cast<std::int32_t>(sizeof(PointT));
+        cloud.num_points = static_cast<std::int32_t>(points.size());
+        static_assert(sizeof(NoiseRemoverLabel) == 1, "labels are bytes");
+        detail::check(lp_remove_noise(ctx_->get(), &c, &cloud, reinterpret_cast<std::uint8_t*>(labels.data())));
+    }
+
+  private:
+    NoiseRemoverConfiguration config_;
+    std::unique_ptr<detail::Context> ctx_;
+    std::uint32_t capacity_ = 200000;
 };
 
 }  // namespace lidar_processing_lib

@@ -2,13 +2,19 @@
 // (src/processor/src/processor.cpp:556-670) written against the drop-in
 // lidar_processing_lib classes of lidar_processing_lib_amd.hpp.
 //
-// Usage: processor_example <in.bin> <out.bin>
+// Usage: processor_example <in.bin> <out.bin> [full]
 //   in.bin : int32 n, then n records {float x, y, z, pad, intensity; uint16 ring; 10 bytes padding}
 //            (the 32-byte pcl::PointXYZIR layout)
 //   out.bin: uint32 seg_labels[n], int32 n_obs, int32 cluster_labels[n_obs],
 //            int32 k, int32 hull_offsets[k+1], int32 hull_indices[...]
-#include <cstdio>
+//   full   : the noise filter runs first (NoiseRemover::filter, VALID points
+//            kept in input order; seg_labels then covers the n_kept points) and
+//            every hull polygon gets a rotating-calipers box; out.bin then
+//            starts with int32 n_kept, uint8 noise[n] and ends with k boxes of
+//            {double corners[8]; float area; float angle; int32 valid}.
 #include <cstdint>
+#include <cstdio>
+#include <string>
 #include <vector>
 
 #include "lidar_processing_lib_amd.hpp"
@@ -36,18 +42,34 @@ struct Cloud {  // the part of pcl::PointCloud<P> the library uses
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+    if (argc != 3 && !(argc == 4 && std::string(argv[3]) == "full")) {
+        std::fprintf(stderr, "usage: %s in.bin out.bin [full]\n", argv[0]);
         return 2;
     }
+    const bool full = argc == 4;
     std::FILE* fi = std::fopen(argv[1], "rb");
     if (!fi) return 2;
-    std::int32_t n = 0;
-    if (std::fread(&n, 4, 1, fi) != 1) return 2;
-    Cloud<PointXYZIR> cloud;
-    cloud.points.resize(n);
-    if (std::fread(cloud.points.data(), sizeof(PointXYZIR), n, fi) != static_cast<std::size_t>(n)) return 2;
+    std::int32_t n_in = 0;
+    if (std::fread(&n_in, 4, 1, fi) != 1) return 2;
+    Cloud<PointXYZIR> input;
+    input.points.resize(n_in);
+    if (std::fread(input.points.data(), sizeof(PointXYZIR), n_in, fi) != static_cast<std::size_t>(n_in)) return 2;
     std::fclose(fi);
+
+    // Noise filter ahead of the segmenter: keep VALID points in input order.
+    std::vector<lidar_processing_lib::NoiseRemoverLabel> noise;
+    Cloud<PointXYZIR> cloud;
+    if (full) {
+        lidar_processing_lib::NoiseRemover remover;
+        std::vector<lidar_processing_lib::NoiseRemover::PointT> pts(n_in);
+        for (std::int32_t i = 0; i < n_in; ++i) pts[i] = {input.points[i].x, input.points[i].y, input.points[i].z};
+        remover.filter(pts, noise);
+        for (std::int32_t i = 0; i < n_in; ++i)
+            if (noise[i] == lidar_processing_lib::NoiseRemoverLabel::VALID) cloud.points.push_back(input.points[i]);
+    } else {
+        cloud.points = input.points;
+    }
+    const std::int32_t n = static_cast<std::int32_t>(cloud.points.size());
 
     lidar_processing_lib::Segmenter segmenter;
     lidar_processing_lib::Clusterer clusterer;
@@ -79,8 +101,24 @@ int main(int argc, char** argv) {
     std::vector<std::vector<std::int32_t>> hulls;
     polygonizer.convexHulls(lists, hulls);  // processor.cpp:663, all clusters at once
 
+    // processor.cpp:665-704: the hull polygon of each cluster, then its box.
+    std::vector<lidar_processing_lib::BoundingBox> boxes;
+    if (full) {
+        std::vector<std::vector<lidar_processing_lib::PointXY>> polys(hulls.size());
+        std::vector<const std::vector<lidar_processing_lib::PointXY>*> plist;
+        for (std::size_t c = 0; c < hulls.size(); ++c) {
+            for (auto v : hulls[c]) polys[c].push_back(groups[c][v]);
+            plist.push_back(&polys[c]);
+        }
+        polygonizer.boundingBoxesRotatingCalipers(plist, boxes);
+    }
+
     std::FILE* fo = std::fopen(argv[2], "wb");
     if (!fo) return 2;
+    if (full) {
+        std::fwrite(&n, 4, 1, fo);
+        std::fwrite(noise.data(), 1, noise.size(), fo);
+    }
     std::fwrite(labels.data(), 4, labels.size(), fo);
     const std::int32_t no = static_cast<std::int32_t>(clabels.size());
     std::fwrite(&no, 4, 1, fo);
@@ -94,6 +132,16 @@ int main(int argc, char** argv) {
         std::fwrite(&off, 4, 1, fo);
     }
     for (auto& h : hulls) std::fwrite(h.data(), 4, h.size(), fo);
+    for (const auto& b : boxes) {
+        for (const auto& c : b.corners) {
+            std::fwrite(&c.x, 8, 1, fo);
+            std::fwrite(&c.y, 8, 1, fo);
+        }
+        const std::int32_t valid = b.is_valid ? 1 : 0;
+        std::fwrite(&b.area, 4, 1, fo);
+        std::fwrite(&b.angle_rad, 4, 1, fo);
+        std::fwrite(&valid, 4, 1, fo);
+    }
     std::fclose(fo);
     std::printf("points %d obstacles %d clusters %d\n", n, no, k);
     return 0;

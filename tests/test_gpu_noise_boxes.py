@@ -135,3 +135,59 @@ def test_filter_off_again_restores_plain_path(ctx, oracle):
     ref = oracle.process_frame(f.x, f.y, f.z, f.ring, scfg=sco)
     assert out["num_points"] == f.n and out["num_noise"] == 0 and out["boxes"] == []
     assert np.array_equal(out["seg_labels"], ref["seg_labels"])
+
+
+def test_cpp_dropin_full_pipeline(tmp_path, oracle):
+    """NoiseRemover -> Segmenter -> Clusterer -> Polygonizer (hulls and boxes)
+    through the drop-in C++ classes (cpp/processor_example.cpp, mode full)
+    against the oracle's full frame with the radius rule."""
+    import os
+    import subprocess
+    f = synth.workload_c2(5)
+    rec = np.zeros(f.n, dtype=np.dtype({"names": ["x", "y", "z", "intensity", "ring"],
+                                        "formats": ["<f4", "<f4", "<f4", "<f4", "<u2"],
+                                        "offsets": [0, 4, 8, 16, 20], "itemsize": 32}))
+    rec["x"], rec["y"], rec["z"], rec["intensity"], rec["ring"] = f.x, f.y, f.z, f.intensity, f.ring
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(src, "wb") as fh:
+        fh.write(np.int32(f.n).tobytes())
+        fh.write(rec.tobytes())
+    exe = os.path.join(os.path.dirname(N.library_path()), "processor_example")
+    subprocess.run([exe, str(src), str(dst), "full"], check=True, timeout=120)
+    raw = open(dst, "rb").read()
+    pos = 0
+
+    def take(dtype, count):
+        nonlocal pos
+        a = np.frombuffer(raw, dtype, count, pos)
+        pos += a.nbytes
+        return a
+
+    n = int(take(np.int32, 1)[0])
+    noise = take(np.uint8, f.n)
+    seg = take(np.uint32, n)
+    no = int(take(np.int32, 1)[0])
+    clab = take(np.int32, no)
+    k = int(take(np.int32, 1)[0])
+    hoff = take(np.int32, k + 1)
+    hidx = take(np.int32, int(hoff[-1]))
+    boxes_raw = take(np.dtype([("c", "<f8", 8), ("a", "<f4"), ("g", "<f4"), ("v", "<i4")]), k)
+    assert pos == len(raw)
+    proc = oracle.Processor()
+    ref = oracle.full_frame(proc, f.x, f.y, f.z, f.ring)
+    assert np.array_equal(noise, ref["noise_labels"])
+    assert n == ref["num_points"]
+    assert np.array_equal(seg, ref["seg_labels"])
+    assert np.array_equal(clab, ref["cluster_labels"])
+    keep = noise == 0
+    x, y = f.x[keep], f.y[keep]
+    obs = np.flatnonzero(seg == 2)
+    ox, oy = x[obs].astype(np.float64), y[obs].astype(np.float64)
+    hg = hull_coords(ox, oy, clab, k, hoff, hidx)
+    hr = hull_coords(ox, oy, ref["cluster_labels"], k, ref["hull_offsets"], ref["hull_indices"])
+    assert all(np.array_equal(a, b) for a, b in zip(hg, hr))
+    for c in range(k):
+        rb = ref["boxes"][c]
+        assert boxes_raw[c]["v"] == rb.is_valid
+        assert np.array_equal(boxes_raw[c]["c"].view(np.uint64), np.array(rb.corners[:], np.float64).view(np.uint64))
+        assert boxes_raw[c]["a"] == np.float32(rb.area) and boxes_raw[c]["g"] == np.float32(rb.angle_rad)
