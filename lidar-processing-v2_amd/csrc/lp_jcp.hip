@@ -218,18 +218,36 @@ __device__ __forceinline__ void add_slot(uint32_t l, float w, float& wg, float& 
     wo += (l == kSO) ? w : 0.0f;
 }
 
-// Transfer map of queue pixel j (pair state (state(j-2), state(j-1)) ->
-// (state(j-1), state(j))).  Labels of earlier rows come from `fin`.
-__device__ __forceinline__ uint64_t pixel_map(const DevBuf& d, size_t qo, int j, int rs, const uint8_t* fin) {
+// Slot weights, label sources and the sum test of one queue pixel, as
+// written by k_jcp_slots.  They do not depend on solved labels, so the solve
+// loads the next chunk's while it solves the current one.
+struct SlotData {
+    float4 w[6];
+    int4 s[6];
+    uint32_t ok;
+};
+
+__device__ __forceinline__ void load_slots(const DevBuf& d, size_t qo, int j, SlotData& sd) {
     const float4* w4 = reinterpret_cast<const float4*>(d.slot_w + (qo + j) * 24);
     const int4* s4 = reinterpret_cast<const int4*>(d.slot_src + (qo + j) * 24);
-    const bool ok = d.qsum_ok[qo + j] != 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        sd.w[q] = w4[q];
+        sd.s[q] = s4[q];
+    }
+    sd.ok = d.qsum_ok[qo + j];
+}
+
+// Transfer map of queue pixel j (pair state (state(j-2), state(j-1)) ->
+// (state(j-1), state(j))).  Labels of earlier rows come from `fin`.
+__device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs, const uint8_t* fin) {
+    const bool ok = sd.ok != 0;
     float wv[24];
     int32_t src[24];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        const float4 a = w4[q];
-        const int4 b = s4[q];
+        const float4 a = sd.w[q];
+        const int4 b = sd.s[q];
         wv[4 * q] = a.x;
         wv[4 * q + 1] = a.y;
         wv[4 * q + 2] = a.z;
@@ -313,6 +331,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
     const size_t qo = (size_t)f * L.P;
     const int32_t* ql = d.qlist + qo;
     for (int r = tid; r <= s.H; r += kSolveThreads) row_start[r] = lower_bound_i32(ql, nq, r * s.W);
+    // Chunk bases run rs, rs + T, ... below re, then the next non-empty row
+    // starts at re: the first chunk starts at 0 and each next base is known.
+    SlotData cur, nxt;
+    if (tid < nq) load_slots(d, qo, tid, cur);
     __syncthreads();
     for (int r = 0; r < s.H; ++r) {
         const int rs = row_start[r], re = row_start[r + 1];
@@ -320,7 +342,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
         uint32_t carry = kSU * 3 + kSU;
         for (int base = rs; base < re; base += kSolveThreads) {
             const int j = base + tid;
-            uint64_t m = j < re ? pixel_map(d, qo, j, rs, fin) : kIdentityMap;
+            const int jn = ((base + kSolveThreads < re) ? base + kSolveThreads : re) + tid;
+            if (jn < nq) load_slots(d, qo, jn, nxt);
+            uint64_t m = j < re ? pixel_map(cur, j, rs, fin) : kIdentityMap;
             // Inclusive scan inside the wavefront: m := m_j o ... o m_first.
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
@@ -337,6 +361,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
             uint64_t all = kIdentityMap;
             for (int q = 0; q < kSolveWaves; ++q) all = map_compose(wtot[q], all);
             carry = static_cast<uint32_t>(all >> (4 * carry)) & 0xf;
+            cur = nxt;  // the prefetch has had the whole chunk to land
             __syncthreads();
         }
     }
