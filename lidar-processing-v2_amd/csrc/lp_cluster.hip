@@ -155,7 +155,7 @@ __global__ void k_voxel_insert(DevBuf d, Layout L) {
         h = (h + 1) & mask;
     }
     atomicMin(&d.hmin[ho + h], k);
-    atomicAdd(&d.hcount[ho + h], 1u);
+    wave_atomic_inc(&d.hcount[ho + h]);
     d.vslot[ob + k] = static_cast<int32_t>(h);
 }
 
@@ -379,7 +379,7 @@ __global__ void k_cluster_scatter(DevBuf d, Layout L) {
     const size_t ob = (size_t)f * L.N;
     const int32_t lab = d.clabel[ob + k];
     if (lab < 0) return;
-    const uint32_t pos = atomicAdd(&d.cl_cursor[ob + lab], 1u);
+    const uint32_t pos = wave_atomic_inc(&d.cl_cursor[ob + lab]);
     const uint32_t dst = d.cl_off[ob + lab] + pos;
     d.cl_members[ob + dst] = k;
     d.hcx[ob + dst] = static_cast<double>(d.ox[ob + k]);

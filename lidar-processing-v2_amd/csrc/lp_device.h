@@ -216,6 +216,31 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// atomicAdd(ctr, 1) for every active lane, with the lanes of the wavefront
+// that target the same counter combined into one atomic (points of one
+// wavefront mostly share a cell or a cluster, so per-lane atomics on one
+// address would serialise).  Returns what atomicAdd would have: the counter's
+// value before this lane's increment, lanes of one address taking
+// consecutive values in lane order.
+__device__ __forceinline__ uint32_t wave_atomic_inc(uint32_t* ctr) {
+    const int lane = lane_id();
+    const unsigned long long me = reinterpret_cast<unsigned long long>(ctr);
+    uint64_t pending = __ballot(1);
+    uint32_t result = 0;
+    while (pending) {
+        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+        const unsigned long long target = __shfl(me, leader, kWave);
+        const bool mine = ((pending >> lane) & 1ull) && me == target;
+        const uint64_t grp = __ballot(mine);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(ctr, static_cast<uint32_t>(__popcll(grp)));
+        base = __shfl(base, leader, kWave);
+        if (mine) result = base + lanes_below(grp);
+        pending &= ~grp;
+    }
+    return result;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_sum(T v) {
     const int l = lane_id();

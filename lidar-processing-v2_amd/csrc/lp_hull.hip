@@ -124,6 +124,48 @@ __device__ __forceinline__ bool pt_less(const double* X, const double* Y, const 
     return I[a] < I[b];
 }
 
+// One half of Andrew's chain (polygonizer.cpp:66-88) over the n sorted points
+// P: the lower chain visits P[0..n), the upper chain P[n-1..0] starting from
+// P[n-1], which the lower chain always ends with, so the two halves are
+// independent and run on two lanes at once.  The two top stack points live in
+// registers and the next point's coordinates are loaded ahead of the pop
+// loop.  Pops while the exact orientation is <= 0.  Writes the chain to S
+// and returns its length.
+__device__ int half_chain(const double* X, const double* Y, const uint16_t* P, int n, bool upper, uint16_t* S,
+                          uint32_t* inexact) {
+    int kk = 0;
+    double x1 = 0.0, y1 = 0.0, x2 = 0.0, y2 = 0.0;
+    int ns;
+    int p = upper ? P[n - 1] : P[0];
+    double px = X[p], py = Y[p];
+    for (int t = 0; t < n; ++t) {
+        const int pn = (t + 1 < n) ? (upper ? P[n - 2 - t] : P[t + 1]) : p;
+        const double pxn = X[pn], pyn = Y[pn];
+        while (kk > 1) {
+            const int o = orient_exact(x2, y2, x1, y1, px, py, &ns);
+            *inexact += (o != ns) ? 1u : 0u;
+            if (o > 0) break;
+            --kk;
+            x1 = x2;
+            y1 = y2;
+            if (kk > 1) {
+                const int q = S[kk - 2];
+                x2 = X[q];
+                y2 = Y[q];
+            }
+        }
+        S[kk++] = static_cast<uint16_t>(p);
+        x2 = x1;
+        y2 = y1;
+        x1 = px;
+        y1 = py;
+        p = pn;
+        px = pxn;
+        py = pyn;
+    }
+    return kk;
+}
+
 constexpr int kWavesPerBlock = 4;
 constexpr int kHullRounds = 3;
 constexpr int kHullBig = 4096;  // candidates one workgroup finishes after the rounds
@@ -209,7 +251,8 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
     __shared__ int32_t si[kWavesPerBlock][kHullChunk];
     __shared__ uint16_t perm[kWavesPerBlock][kHullChunk];
     __shared__ uint16_t stk[kWavesPerBlock][2 * kHullChunk];
-    __shared__ int nout[kWavesPerBlock];
+    __shared__ int nlo[kWavesPerBlock];
+    __shared__ int nup[kWavesPerBlock];
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const uint32_t total = d.fs[f * kFsCount + kFsHullChunks];
@@ -301,15 +344,29 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
                 wave_sync();
             }
         }
-        if (lane == 0) {
+        // Lane 0 the lower chain, lane 1 the upper one (in S + kHullChunk);
+        // vertex i of the hull is lower[i] for i < kl, then upper[i - kl + 1].
+        if (ns < 3) {
+            if (lane == 0) {
+                uint32_t inexact = 0;
+                nlo[w] = chain_hull(X, Y, I, P, ns, S, &inexact);
+                nup[w] = 2;
+            }
+        } else if (lane < 2) {
             uint32_t inexact = 0;
-            nout[w] = chain_hull(X, Y, I, P, ns, S, &inexact);
+            const int len = half_chain(X, Y, P, ns, lane == 1, lane == 1 ? S + kHullChunk : S, &inexact);
+            if (lane == 0)
+                nlo[w] = len;
+            else
+                nup[w] = len;
             if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
         }
         wave_sync();
-        const int kk = nout[w];
+        const int kl = nlo[w];
+        const int kk = kl + nup[w] - 2;
+        auto vert = [&](int i) -> int { return i < kl ? S[i] : S[kHullChunk + i - kl + 1]; };
         if (final_chunk) {
-            for (int i = lane; i < kk; i += kWave) d.hull_idx[ob + base + i] = I[S[i]];
+            for (int i = lane; i < kk; i += kWave) d.hull_idx[ob + base + i] = I[vert(i)];
             if (lane == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
         } else {
             uint32_t pos = 0;
@@ -317,9 +374,10 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             pos = __shfl(pos, 0, kWave);
             for (int i = lane; i < kk; i += kWave) {
                 const size_t dst = out_b + ob + base + pos + i;
-                d.hcx[dst] = X[S[i]];
-                d.hcy[dst] = Y[S[i]];
-                d.hci[dst] = I[S[i]];
+                const int v = vert(i);
+                d.hcx[dst] = X[v];
+                d.hcy[dst] = Y[v];
+                d.hci[dst] = I[v];
             }
         }
         wave_sync();
@@ -335,7 +393,7 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
     __shared__ int32_t I[kHullBig];
     __shared__ uint16_t P[kHullBig];
     __shared__ uint16_t S[2 * kHullBig];
-    __shared__ int nout;
+    __shared__ int nlo, nup;
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const int tid = threadIdx.x;
@@ -378,14 +436,25 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
                 __syncthreads();
             }
         }
-        if (tid == 0) {
+        if (n < 3) {
+            if (tid == 0) {
+                uint32_t inexact = 0;
+                nlo = chain_hull(X, Y, I, P, n, S, &inexact);
+                nup = 2;
+            }
+        } else if (tid < 2) {
             uint32_t inexact = 0;
-            nout = chain_hull(X, Y, I, P, n, S, &inexact);
+            const int len = half_chain(X, Y, P, n, tid == 1, tid == 1 ? S + kHullBig : S, &inexact);
+            if (tid == 0)
+                nlo = len;
+            else
+                nup = len;
             if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
         }
         __syncthreads();
-        const int kk = nout;
-        for (int i = tid; i < kk; i += 1024) d.hull_idx[ob + base + i] = I[S[i]];
+        const int kl = nlo;
+        const int kk = kl + nup - 2;
+        for (int i = tid; i < kk; i += 1024) d.hull_idx[ob + base + i] = I[i < kl ? S[i] : S[kHullBig + i - kl + 1]];
         if (tid == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
         __syncthreads();
     }

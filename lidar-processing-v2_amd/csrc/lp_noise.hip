@@ -19,6 +19,8 @@
 
 namespace lp {
 
+using namespace lpd;
+
 namespace {
 
 constexpr int kNoiseMaxExtent = 4;        // cells each way; larger radii scan the whole frame
@@ -98,7 +100,7 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
         if (prev == -1 || prev == key) break;
         h = (h + 1) & mask;
     }
-    atomicAdd(&d.hcount[ho + h], 1u);
+    wave_atomic_inc(&d.hcount[ho + h]);
     d.vslot[(size_t)f * L.N + i] = static_cast<int32_t>(h);
 }
 
@@ -111,10 +113,20 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const int gi = fd.pt_off + i;
     const size_t ho = (size_t)f * L.hash_cap;
     const int32_t h = d.vslot[(size_t)f * L.N + i];
-    const int pos = d.hdense[ho + h] + atomicAdd(&d.hmin[ho + h], 1);
+    const int pos = d.hdense[ho + h] + static_cast<int>(wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h])));
     const size_t po = (size_t)f * L.N;
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], 0.0f);
     d.didx[po + pos] = i;
+}
+
+// Squared distance from the point to the box of cell offset (dx, dy, dz)
+// relative to its own cell, per axis from the point's position inside its
+// 1 m cell.  Cells whose box is beyond the radius hold no neighbour.
+__device__ __forceinline__ float cell_gap_sqr(float fx, float fy, float fz, int dx, int dy, int dz) {
+    const float gx = dx > 0 ? (static_cast<float>(dx) - fx) : (dx < 0 ? (fx - static_cast<float>(dx + 1)) : 0.0f);
+    const float gy = dy > 0 ? (static_cast<float>(dy) - fy) : (dy < 0 ? (fy - static_cast<float>(dy + 1)) : 0.0f);
+    const float gz = dz > 0 ? (static_cast<float>(dz) - fz) : (dz < 0 ? (fz - static_cast<float>(dz + 1)) : 0.0f);
+    return gx * gx + gy * gy + gz * gz;
 }
 
 __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
@@ -138,33 +150,37 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
             const int e = static_cast<int>(floorf(r + kNoiseCellSlack)) + 1;
             int cx, cy, cz;
             noise_cell(t.x, t.y, t.z, cx, cy, cz);
+            // Position inside the own cell (exact for in-range coordinates;
+            // clamped cells give values outside [0, 1), which only make the
+            // cull below more permissive).  The cull keeps every cell within
+            // (r + slack): the float test admits a few ulps beyond r.
+            const float fx = t.x - floorf(t.x), fy = t.y - floorf(t.y), fz = t.z - floorf(t.z);
+            const float reach = (r + kNoiseCellSlack) * (r + kNoiseCellSlack);
             bool done = false;
-            // Own cell first: most points find their neighbours there.
-            for (int q = 0; q < (2 * e + 1) * (2 * e + 1) * (2 * e + 1) + 1 && !done; ++q) {
-                int ax = cx, ay = cy, az = cz;
-                if (q > 0) {
-                    const int w = 2 * e + 1;
-                    const int m = q - 1;
-                    const int dx = m % w - e, dy = (m / w) % w - e, dz = m / (w * w) - e;
-                    if (dx == 0 && dy == 0 && dz == 0) continue;
-                    ax += dx;
-                    ay += dy;
-                    az += dz;
-                    if (ax < 0 || ax > 1023 || ay < 0 || ay > 1023 || az < 0 || az > 2047) continue;
-                }
-                const int32_t h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
-                if (h < 0) continue;
-                const int s0 = d.hdense[ho + h];
-                const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
-                for (int s = s0; s < s1; ++s) {
-                    if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) {
-                        done = true;
-                        break;
-                    }
-                }
+            // Shells of growing Chebyshev radius: own cell, its 26
+            // neighbours, then the outer shells, nearest first.
+            for (int sh = 0; sh <= e && !done; ++sh) {
+                for (int dz = -sh; dz <= sh && !done; ++dz)
+                    for (int dy = -sh; dy <= sh && !done; ++dy)
+                        for (int dx = -sh; dx <= sh && !done; ++dx) {
+                            if (max(abs(dx), max(abs(dy), abs(dz))) != sh) continue;
+                            if (sh > 0 && cell_gap_sqr(fx, fy, fz, dx, dy, dz) > reach) continue;
+                            const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                            if (ax < 0 || ax > 1023 || ay < 0 || ay > 1023 || az < 0 || az > 2047) continue;
+                            const int32_t h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
+                            if (h < 0) continue;
+                            const int s0 = d.hdense[ho + h];
+                            const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
+                            for (int s = s0; s < s1; ++s) {
+                                if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) {
+                                    done = true;
+                                    break;
+                                }
+                            }
+                        }
             }
         } else {
-            // Radius beyond the cube (range > ~150 m) or infinite: scan all.
+            // Radius beyond the cell cube (range > ~150 m) or infinite: scan all.
             for (int s = 0; s < n; ++s)
                 if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) break;
         }

@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kBoxWaves = 4;
 constexpr int kBoxPairs = 1024;  // antipodal pairs held per polygon
+constexpr int kBoxVerts = 512;   // polygons up to this size are staged in LDS
 
 __device__ __forceinline__ double tri_area(const double* X, const double* Y, int a, int b, int c) {
     return fabs((X[a] * (Y[b] - Y[c]) + X[b] * (Y[c] - Y[a]) + X[c] * (Y[a] - Y[b])) * 0.5);
@@ -86,6 +87,8 @@ __global__ __launch_bounds__(kBoxWaves * kWave) void k_boxes(DevBuf d, Layout L,
     __shared__ Cand cand_l[kBoxWaves][kWave];
     __shared__ int np_l[kBoxWaves];
     __shared__ double cen_l[kBoxWaves][2];
+    __shared__ double vx_l[kBoxWaves][kBoxVerts];
+    __shared__ double vy_l[kBoxWaves][kBoxVerts];
     const int f = blockIdx.y;
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
@@ -105,6 +108,15 @@ __global__ __launch_bounds__(kBoxWaves * kWave) void k_boxes(DevBuf d, Layout L,
         const int n = static_cast<int>(v1 - v0);
         const double* X = VX + ob + v0;
         const double* Y = VY + ob + v0;
+        if (n <= kBoxVerts) {
+            for (int q = lane; q < n; q += kWave) {
+                vx_l[w][q] = X[q];
+                vy_l[w][q] = Y[q];
+            }
+            wave_sync();
+            X = vx_l[w];
+            Y = vy_l[w];
+        }
         lp_bounding_box box;
         for (int q = 0; q < 8; ++q) box.corners[q] = 0.0;
         box.area = 0.0f;

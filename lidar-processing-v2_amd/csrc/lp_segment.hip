@@ -83,7 +83,7 @@ __global__ void k_project(DevBuf d, Layout L) {
                 const int col = static_cast<uint16_t>((s.w_minus_1 * az) / kTwoPi);
                 cell = slice * s.rings + radial;
                 pix = row * s.W + col;
-                atomicAdd(&d.cell_count[(size_t)f * L.cell_stride + cell], 1u);
+                wave_atomic_inc(&d.cell_count[(size_t)f * L.cell_stride + cell]);
                 const unsigned long long key = (static_cast<unsigned long long>(f2u(dd)) << 32) |
                                                (static_cast<unsigned long long>(slice) << L.idx_bits) |
                                                static_cast<unsigned long long>(i);
@@ -133,7 +133,7 @@ __global__ void k_cell_fill(DevBuf d, Layout L) {
     const int cell = d.pt_cell[fd.pt_off + i];
     if (cell < 0) return;
     const size_t co = (size_t)f * L.cell_stride;
-    const uint32_t slot = atomicAdd(&d.cell_cursor[co + cell], 1u);
+    const uint32_t slot = wave_atomic_inc(&d.cell_cursor[co + cell]);
     const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
     d.cell_items[(size_t)f * L.N + start + slot] = i;
 }
