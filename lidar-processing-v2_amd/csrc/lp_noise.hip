@@ -94,14 +94,31 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
     const int32_t key = noise_key(cx, cy, cz);
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const size_t ho = (size_t)f * L.hash_cap;
-    uint32_t h = noise_slot(key, mask);
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
-        if (prev == -1 || prev == key) break;
-        h = (h + 1) & mask;
+    // Lanes with the same cell (consecutive points mostly share one) insert
+    // once: the lowest such lane probes and adds the group's count.
+    const int lane = lane_id();
+    uint64_t pending = __ballot(1);
+    int32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+        const int32_t lkey = __shfl(key, leader, kWave);
+        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+        const uint64_t grp = __ballot(mine);
+        uint32_t h = 0;
+        if (lane == leader) {
+            h = noise_slot(key, mask);
+            for (uint32_t probe = 0; probe <= mask; ++probe) {
+                const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+                if (prev == -1 || prev == key) break;
+                h = (h + 1) & mask;
+            }
+            atomicAdd(&d.hcount[ho + h], static_cast<uint32_t>(__popcll(grp)));
+        }
+        h = __shfl(h, leader, kWave);
+        if (mine) slot = static_cast<int32_t>(h);
+        pending &= ~grp;
     }
-    wave_atomic_inc(&d.hcount[ho + h]);
-    d.vslot[(size_t)f * L.N + i] = static_cast<int32_t>(h);
+    d.vslot[(size_t)f * L.N + i] = slot;
 }
 
 // hdense holds the exclusive scan of hcount (cell starts) on entry.
@@ -171,11 +188,19 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
                             if (h < 0) continue;
                             const int s0 = d.hdense[ho + h];
                             const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
-                            for (int s = s0; s < s1; ++s) {
-                                if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) {
-                                    done = true;
-                                    break;
-                                }
+                            // Four points per step, loads issued together;
+                            // counting past the k-th hit leaves the label alone.
+                            int s = s0;
+                            for (; s + 4 <= s1 && !done; s += 4) {
+                                const float4 a = d.dsort[po + s], b = d.dsort[po + s + 1];
+                                const float4 c = d.dsort[po + s + 2], q = d.dsort[po + s + 3];
+                                cnt += (noise_dist(t, a) <= r2 ? 1u : 0u) + (noise_dist(t, b) <= r2 ? 1u : 0u) +
+                                       (noise_dist(t, c) <= r2 ? 1u : 0u) + (noise_dist(t, q) <= r2 ? 1u : 0u);
+                                done = cnt >= k;
+                            }
+                            for (; s < s1 && !done; ++s) {
+                                cnt += noise_dist(t, d.dsort[po + s]) <= r2 ? 1u : 0u;
+                                done = cnt >= k;
                             }
                         }
             }

@@ -12,7 +12,8 @@
 //
 // One wavefront per polygon: lane 0 walks the antipodal pairs (sequential by
 // nature, O(n)) into LDS; the lanes then evaluate 64 candidates at a time
-// (O(n) each) and lane 0 applies the sequential selection rule to them.
+// (O(n) each) and lane 0 applies the sequential selection rule to the few
+// that beat the running minimum.
 #include "lp_device.h"
 #include "lp_internal.h"
 #include "lp_kernels.h"
@@ -24,20 +25,22 @@ using namespace lpd;
 namespace {
 
 constexpr int kBoxWaves = 4;
-constexpr int kBoxPairs = 1024;  // antipodal pairs held per polygon
-constexpr int kBoxVerts = 512;   // polygons up to this size are staged in LDS
-
-__device__ __forceinline__ double tri_area(const double* X, const double* Y, int a, int b, int c) {
-    return fabs((X[a] * (Y[b] - Y[c]) + X[b] * (Y[c] - Y[a]) + X[c] * (Y[a] - Y[b])) * 0.5);
-}
+constexpr int kBoxPairs = 512;   // antipodal pairs held per polygon (about 1.5 per vertex)
+constexpr int kBoxVerts = 256;   // polygons up to this size are staged in LDS
 
 struct Cand {
     double area, ux, uy, mnx, mxx, mny, mxy;
     int valid;
 };
 
-// Antipodal pairs of polygon (X, Y, n) in the reference's order; returns the
-// count, or -1 when the walk exceeds 4n + 8 steps or the LDS capacity.
+// areaOfTriangle (polygonizer.hpp:228-231), same operand order.
+__device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, double by, double cx, double cy) {
+    return fabs((ax * (by - cy) + bx * (cy - ay) + cx * (ay - by)) * 0.5);
+}
+
+// Antipodal pairs of polygon (X, Y, n >= 3) in the reference's order; returns
+// the count, or -1 when the walk exceeds 4n + 8 steps or the LDS capacity.
+// Vertices i, i+1, j and j+1 stay in registers: each step loads one vertex.
 __device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pairs) {
     int count = 0;
     auto push = [&](int a, int b) -> bool {
@@ -48,18 +51,37 @@ __device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pai
     auto nx = [n](int k) { return k + 1 == n ? 0 : k + 1; };
     const int i0 = n - 1;
     int i = 0, j = 1;
+    double xi = X[0], yi = Y[0], xi1 = X[1], yi1 = Y[1];
+    double xj = X[1], yj = Y[1], xj1 = X[2], yj1 = Y[2];
+    auto adv_i = [&]() {
+        i = nx(i);
+        xi = xi1;
+        yi = yi1;
+        const int q = nx(i);
+        xi1 = X[q];
+        yi1 = Y[q];
+    };
+    auto adv_j = [&]() {
+        j = nx(j);
+        xj = xj1;
+        yj = yj1;
+        const int q = nx(j);
+        xj1 = X[q];
+        yj1 = Y[q];
+    };
     const int limit = 4 * n + 8;
     int guard = 0;
-    while (tri_area(X, Y, i, nx(i), nx(j)) > tri_area(X, Y, i, nx(i), j)) {
-        j = nx(j);
+    // areaOfTriangle(i, i+1, j+1) > areaOfTriangle(i, i+1, j)
+    while (tri_area_xy(xi, yi, xi1, yi1, xj1, yj1) > tri_area_xy(xi, yi, xi1, yi1, xj, yj)) {
+        adv_j();
         if (++guard > limit) return -1;
     }
     const int j0 = j;
     while (i != j0) {
-        i = nx(i);
+        adv_i();
         if (!push(i, j)) return -1;
-        while (tri_area(X, Y, i, nx(i), nx(j)) > tri_area(X, Y, i, nx(i), j)) {
-            j = nx(j);
+        while (tri_area_xy(xi, yi, xi1, yi1, xj1, yj1) > tri_area_xy(xi, yi, xi1, yi1, xj, yj)) {
+            adv_j();
             if (!(i == j0 && j == i0)) {
                 if (!push(i, j)) return -1;
             } else {
@@ -67,7 +89,8 @@ __device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pai
             }
             if (++guard > limit) return -1;
         }
-        if (tri_area(X, Y, j, nx(i), nx(j)) == tri_area(X, Y, i, nx(i), j)) {
+        // areaOfTriangle(j, i+1, j+1) == areaOfTriangle(i, i+1, j)
+        if (tri_area_xy(xj, yj, xi1, yi1, xj1, yj1) == tri_area_xy(xi, yi, xi1, yi1, xj, yj)) {
             if (!(i == j0 && j == i0)) {
                 if (!push(i, nx(j))) return -1;
             } else {
@@ -186,16 +209,22 @@ __global__ __launch_bounds__(kBoxWaves * kWave) void k_boxes(DevBuf d, Layout L,
                     }
                     cl[lane] = r;
                     wave_sync();
+                    // Only candidates below the round's starting minimum can
+                    // replace the box (the minimum only falls within a round);
+                    // lane 0 applies the rule to them in candidate order.
+                    uint64_t beat = __ballot(r.valid && r.area < static_cast<double>(best));
                     if (lane == 0) {
-                        const int m = min(kWave, nc - base);
-                        for (int q = 0; q < m; ++q) {
+                        while (beat) {
+                            const int q = __ffsll(static_cast<unsigned long long>(beat)) - 1;
+                            beat &= beat - 1;
                             const Cand& e = cl[q];
-                            if (e.valid && e.area < static_cast<double>(best)) {
+                            if (e.area < static_cast<double>(best)) {
                                 best = static_cast<float>(e.area);
                                 sel = e;
                             }
                         }
                     }
+                    best = __shfl(best, 0, kWave);
                     wave_sync();
                 }
                 if (lane == 0 && sel.valid) {
