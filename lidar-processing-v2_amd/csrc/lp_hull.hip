@@ -41,16 +41,12 @@ __device__ __forceinline__ void two_prod(double a, double b, double& p, double& 
     e = fma(a, b, -p);
 }
 
-// Sign of (bx-ax)*(cy-ay) - (by-ay)*(cx-ax), exact.  *naive_sign receives
-// the sign of the expression evaluated in double the way the reference does.
-__device__ int orient_exact(double ax, double ay, double bx, double by, double cx, double cy, int* naive_sign) {
-    const double t1 = (bx - ax) * (cy - ay);
-    const double t2 = (by - ay) * (cx - ax);
-    const double det = t1 - t2;
-    const int ns = (det > 0.0) - (det < 0.0);
-    *naive_sign = ns;
-    const double bound = 3.3306690738754716e-16 * (fabs(t1) + fabs(t2));
-    if (det > bound || -det > bound) return ns;
+// Exact sign of (bx-ax)*(cy-ay) - (by-ay)*(cx-ax) when the double value
+// `det` (sign ns) cannot be certified by the error bound.  Kept out of line:
+// it is rare, and inlined at every call site its expansion arithmetic would
+// crowd the instruction cache of the hot loops.
+__device__ __attribute__((noinline)) int orient_tail(double ax, double ay, double bx, double by, double cx,
+                                                     double cy, int ns) {
     double d[4][2];
     two_diff(bx, ax, d[0][0], d[0][1]);  // bx - ax
     two_diff(cy, ay, d[1][0], d[1][1]);  // cy - ay
@@ -116,6 +112,20 @@ __device__ int orient_exact(double ax, double ay, double bx, double by, double c
     return 0;
 }
 
+// Sign of (bx-ax)*(cy-ay) - (by-ay)*(cx-ax), exact.  *naive_sign receives
+// the sign of the expression evaluated in double the way the reference does.
+__device__ __forceinline__ int orient_exact(double ax, double ay, double bx, double by, double cx, double cy,
+                                            int* naive_sign) {
+    const double t1 = (bx - ax) * (cy - ay);
+    const double t2 = (by - ay) * (cx - ax);
+    const double det = t1 - t2;
+    const int ns = (det > 0.0) - (det < 0.0);
+    *naive_sign = ns;
+    const double bound = 3.3306690738754716e-16 * (fabs(t1) + fabs(t2));
+    if (det > bound || -det > bound) return ns;
+    return orient_tail(ax, ay, bx, by, cx, cy, ns);
+}
+
 // ---- chunk hull ----------------------------------------------------------
 // (x, y, idx) lexicographic order; idx breaks exact-duplicate ties.
 __device__ __forceinline__ bool pt_less(const double* X, const double* Y, const int32_t* I, int a, int b) {
@@ -166,6 +176,84 @@ __device__ int half_chain(const double* X, const double* Y, const uint16_t* P, i
     return kk;
 }
 
+// Splits the n >= 3 sorted points P by the line from P[0] to P[n-1]: LO gets
+// P[0], the points strictly below it (in order), P[n-1]; UP the same with the
+// points strictly above.  Only those can be vertices of the lower / upper
+// chain, so each chain sees about half the points.  One wavefront; returns
+// the two list lengths through *nlo, *nup.
+__device__ void split_by_chord(const double* X, const double* Y, const uint16_t* P, int n, uint16_t* LO,
+                               uint16_t* UP, int* nlo, int* nup) {
+    const int lane = lane_id();
+    const int a = P[0], b = P[n - 1];
+    const double ax = X[a], ay = Y[a], bx = X[b], by = Y[b];
+    int cl = 0, cu = 0;
+    for (int i0 = 1; i0 < n - 1; i0 += kWave) {
+        const int i = i0 + lane;
+        int side = 0;
+        int p = 0;
+        if (i < n - 1) {
+            int dummy;
+            p = P[i];
+            side = orient_exact(ax, ay, bx, by, X[p], Y[p], &dummy);
+        }
+        const uint64_t ml = __ballot(side < 0), mu = __ballot(side > 0);
+        if (side < 0) LO[1 + cl + lanes_below(ml)] = static_cast<uint16_t>(p);
+        if (side > 0) UP[1 + cu + lanes_below(mu)] = static_cast<uint16_t>(p);
+        cl += static_cast<int>(__popcll(ml));
+        cu += static_cast<int>(__popcll(mu));
+    }
+    if (lane == 0) {
+        LO[0] = static_cast<uint16_t>(a);
+        LO[cl + 1] = static_cast<uint16_t>(b);
+        UP[0] = static_cast<uint16_t>(a);
+        UP[cu + 1] = static_cast<uint16_t>(b);
+    }
+    *nlo = cl + 2;
+    *nup = cu + 2;
+}
+
+// Parallel pruning of a half-chain list L[0..n) (sorted by (x, y); first and
+// last kept): a point that does not make a strict turn of sign `turn` (+1
+// lower, -1 upper) with two current neighbours of other coordinates lies on
+// or beyond the segment between them, so it is no vertex of that half hull;
+// all such points are dropped at once, and the pass repeats on the survivors.  The half hull of
+// the survivors is the half hull of the list.  One wavefront; T is scratch of
+// n entries.  Returns the new length.
+__device__ int prune_chain(const double* X, const double* Y, uint16_t* L, uint16_t* T, int n, int turn,
+                           int passes) {
+    const int lane = lane_id();
+    for (int pass = 0; pass < passes && n > 2; ++pass) {
+        int m = 0;
+        for (int i0 = 0; i0 < n; i0 += kWave) {
+            const int i = i0 + lane;
+            bool keep = false;
+            int b = 0;
+            if (i < n) {
+                b = L[i];
+                keep = true;
+                if (i > 0 && i < n - 1) {
+                    const int a = L[i - 1], c = L[i + 1];
+                    // A neighbour with the same coordinates is no witness:
+                    // copies of one vertex would remove each other.
+                    const bool dup = (X[a] == X[b] && Y[a] == Y[b]) || (X[c] == X[b] && Y[c] == Y[b]);
+                    int dummy;
+                    keep = dup || orient_exact(X[a], Y[a], X[b], Y[b], X[c], Y[c], &dummy) * turn > 0;
+                }
+            }
+            const uint64_t mk = __ballot(keep);
+            if (keep) T[m + lanes_below(mk)] = static_cast<uint16_t>(b);
+            m += static_cast<int>(__popcll(mk));
+        }
+        wave_sync();
+        if (m == n) break;
+        for (int i = lane; i < m; i += kWave) L[i] = T[i];
+        wave_sync();
+        n = m;
+    }
+    return n;
+}
+
+constexpr int kPrunePasses = 6;
 constexpr int kWavesPerBlock = 4;
 constexpr int kHullRounds = 3;
 constexpr int kHullBig = 4096;  // candidates one workgroup finishes after the rounds
@@ -251,6 +339,8 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
     __shared__ int32_t si[kWavesPerBlock][kHullChunk];
     __shared__ uint16_t perm[kWavesPerBlock][kHullChunk];
     __shared__ uint16_t stk[kWavesPerBlock][2 * kHullChunk];
+    __shared__ uint16_t lol[kWavesPerBlock][kHullChunk];
+    __shared__ uint16_t upl[kWavesPerBlock][kHullChunk];
     __shared__ int nlo[kWavesPerBlock];
     __shared__ int nup[kWavesPerBlock];
     const int f = blockIdx.y;
@@ -302,6 +392,17 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             if (X[c2] > X[ex[2]] || (X[c2] == X[ex[2]] && c2 < ex[2])) ex[2] = c2;
             if (Y[c3] > Y[ex[3]] || (Y[c3] == Y[ex[3]] && c3 < ex[3])) ex[3] = c3;
         }
+        // The distinct extremes, in counter-clockwise order; repeated ones
+        // (e.g. the leftmost point is also the lowest) are dropped so no edge
+        // has zero length.  With fewer than three there is no interior.
+        int poly[4];
+        int npoly = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int v = ex[e];
+            if (npoly == 0 || X[v] != X[poly[npoly - 1]] || Y[v] != Y[poly[npoly - 1]]) poly[npoly++] = v;
+        }
+        if (npoly > 1 && X[poly[npoly - 1]] == X[poly[0]] && Y[poly[npoly - 1]] == Y[poly[0]]) --npoly;
         int ns = 0;
         for (int i0 = 0; i0 < n; i0 += kWave) {
             const int i = i0 + lane;
@@ -309,12 +410,14 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             if (i < n) {
                 int dummy;
                 keep = true;
-                if (n >= 4) {
+                if (npoly >= 3) {
                     bool inside = true;
 #pragma unroll
-                    for (int e = 0; e < 4 && inside; ++e) {
-                        const int a = ex[e], b = ex[(e + 1) & 3];
-                        inside = orient_exact(X[a], Y[a], X[b], Y[b], X[i], Y[i], &dummy) > 0;
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < npoly && inside) {
+                            const int a = poly[e], b = poly[e + 1 < npoly ? e + 1 : 0];
+                            inside = orient_exact(X[a], Y[a], X[b], Y[b], X[i], Y[i], &dummy) > 0;
+                        }
                     }
                     keep = !inside;
                 }
@@ -352,14 +455,24 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
                 nlo[w] = chain_hull(X, Y, I, P, ns, S, &inexact);
                 nup[w] = 2;
             }
-        } else if (lane < 2) {
-            uint32_t inexact = 0;
-            const int len = half_chain(X, Y, P, ns, lane == 1, lane == 1 ? S + kHullChunk : S, &inexact);
-            if (lane == 0)
-                nlo[w] = len;
-            else
-                nup[w] = len;
-            if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+        } else {
+            int cl, cu;
+            split_by_chord(X, Y, P, ns, lol[w], upl[w], &cl, &cu);
+            wave_sync();
+            cl = prune_chain(X, Y, lol[w], S, cl, 1, kPrunePasses);
+            cu = prune_chain(X, Y, upl[w], S + kHullChunk, cu, -1, kPrunePasses);
+            if (lane < 2) {
+                uint32_t inexact = 0;
+                // One call with per-lane arguments, so the two chains run in
+                // lock-step instead of one after the other.
+                const int len = half_chain(X, Y, lane == 0 ? lol[w] : upl[w], lane == 0 ? cl : cu, lane == 1,
+                                           lane == 1 ? S + kHullChunk : S, &inexact);
+                if (lane == 0)
+                    nlo[w] = len;
+                else
+                    nup[w] = len;
+                if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+            }
         }
         wave_sync();
         const int kl = nlo[w];
@@ -393,7 +506,9 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
     __shared__ int32_t I[kHullBig];
     __shared__ uint16_t P[kHullBig];
     __shared__ uint16_t S[2 * kHullBig];
-    __shared__ int nlo, nup;
+    __shared__ uint16_t LO[kHullBig];
+    __shared__ uint16_t UP[kHullBig];
+    __shared__ int nlo, nup, nlist[2];
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const int tid = threadIdx.x;
@@ -442,14 +557,29 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
                 nlo = chain_hull(X, Y, I, P, n, S, &inexact);
                 nup = 2;
             }
-        } else if (tid < 2) {
-            uint32_t inexact = 0;
-            const int len = half_chain(X, Y, P, n, tid == 1, tid == 1 ? S + kHullBig : S, &inexact);
-            if (tid == 0)
-                nlo = len;
-            else
-                nup = len;
-            if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+        } else {
+            if (tid < kWave) {
+                int cl, cu;
+                split_by_chord(X, Y, P, n, LO, UP, &cl, &cu);
+                wave_sync();
+                cl = prune_chain(X, Y, LO, S, cl, 1, kPrunePasses);
+                cu = prune_chain(X, Y, UP, S + kHullBig, cu, -1, kPrunePasses);
+                if (tid == 0) {
+                    nlist[0] = cl;
+                    nlist[1] = cu;
+                }
+            }
+            __syncthreads();
+            if (tid < 2) {
+                uint32_t inexact = 0;
+                const int len = half_chain(X, Y, tid == 0 ? LO : UP, nlist[tid], tid == 1,
+                                           tid == 1 ? S + kHullBig : S, &inexact);
+                if (tid == 0)
+                    nlo = len;
+                else
+                    nup = len;
+                if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+            }
         }
         __syncthreads();
         const int kl = nlo;
