@@ -22,6 +22,10 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "lidar-processing-v2_amd")]
 import numpy as np  # noqa: E402
 
 METRIC = "frames/s + points/s at 64x2048 (batched vs plain GPU vs CPU); p50/p99 ms"
+# Profiled stages that consist of exactly one kernel launch.
+SINGLE_KERNEL_STAGES = {"noise_count": "k_noise_count", "project": "k_project", "recm": "k_recm",
+                        "pixels": "k_pixel_label", "jcp_slots": "k_jcp_slots", "jcp_solve": "k_jcp_solve",
+                        "labels": "k_seg_labels"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -49,7 +53,9 @@ def algorithmic_bytes(n_raw, n, p, n_obs, voxels, hull_vertices, clusters):
 def stage_bytes(stage, n, p, q, n_obs, voxels, hull_vertices, n_cand, n_raw=0, clusters=0):
     """Algorithmic bytes one launch of a stage moves (per frame)."""
     return {
-        "noise": 13 * n_raw + 14 * n,                # xyz in, label out, kept points out
+        "noise": 12 * n_raw + 4 * n_raw + 16 * n_raw,  # xyz in, cell slot out, cell-sorted points out
+        "noise_count": 25 * n_raw,                   # cell-sorted point + index in, label + keep flag out
+        "noise_compact": 4 * n_raw + 14 * n,         # keep scan in, kept xyz+ring out
         "project": 14 * n + 8 * n,                  # xyz+ring in, cell+pixel out
         "cells": 4 * n * 3 + 4 * 18000 * 2,          # scatter + sort of point indices, zmin out
         "recm": 4 * 18000 * 2,
@@ -211,6 +217,30 @@ def run_cpu(frames, budget_s):
                 p50_ms=float(np.percentile(t * 1e3, 50)))
 
 
+def run_probe():
+    """Counter-collection workload: 8 full-pipeline frames run eagerly three
+    times in one batch, then a 256 MiB device-to-device copy whose known size
+    calibrates FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    from lidar_amd import native as N
+    frames = frames_c2(8)
+    ctx = N.Context(max_frames=8, max_points=max(f.n for f in frames), max_height=64, max_width=2048)
+    ctx.set_noise_filter(N.noise_config())
+    ctx.set_oriented_boxes(True)
+    ctx.stage([(f.x, f.y, f.z, f.ring) for f in frames], N.segmenter_config())
+    for _ in range(3):
+        ctx.run(use_graph=False)
+    ctx.synchronize()
+    log("probe statuses", [o["status"] for o in ctx.fetch()])
+    ctx.close()
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy bytes", a.numel() * 4)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,7 +252,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the c3/c4 batches and the serving run")
+    ap.add_argument("--probe", action="store_true", help="only the counter-collection workload (no JSON line)")
     args = ap.parse_args()
+    if args.probe:
+        run_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard across GPUs "
               "with no exchange, so multi-GPU throughput is N independent copies of this run)", file=sys.stderr)
@@ -259,7 +293,10 @@ def main():
     q = mean("jcp_queue")
     ncand = float(np.mean([o["ransac"]["candidates"] for o in outs]))
     P = 64 * 2048
-    stages = [(s, ms) for s, ms in g["stages"] if s not in ("clear",)]
+    # Dominant kernel: the slowest stage that is a single kernel launch (its
+    # HIP-event time is then that kernel's duration, comparable to rocprof).
+    single = {s: k for s, k in SINGLE_KERNEL_STAGES.items()}
+    stages = [(s, ms) for s, ms in g["stages"] if s in single]
     dom, dom_ms = max(stages, key=lambda t: t[1])
     dom_bytes = stage_bytes(dom, n, P, q, n_obs, vox, hv, ncand, n_raw, ncl) * B
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -267,8 +304,10 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(dom)
-        except (OSError, ValueError):
+            t = json.load(open(tpath)).get(single[dom])
+            if t:
+                traffic = round(t["hbm_bytes_per_frame"] * B)
+        except (OSError, ValueError, KeyError):
             traffic = None
     alg_frame = algorithmic_bytes(n_raw, n, P, n_obs, vox, hv, ncl)
     line = {
@@ -299,7 +338,7 @@ def main():
         "stage_ms_per_batch": {s: round(ms, 4) for s, ms in g["stages"]},
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": single[dom],
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

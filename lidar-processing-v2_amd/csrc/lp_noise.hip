@@ -246,7 +246,9 @@ void launch_noise(lp_context* c, int B, bool compact) {
     hipLaunchKernelGGL(k_noise_insert, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.hcount, reinterpret_cast<uint32_t*>(d.hdense), L.hash_cap, kCountHash, 0, -1);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "noise_count");
     hipLaunchKernelGGL(k_noise_count, dim3(nb, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "noise_compact");
     if (compact) {
         launch_scan(c, B, d.obs_flag, reinterpret_cast<uint32_t*>(d.obs_src), L.N, kCountRawPoints, 0, kFsKept);
         hipLaunchKernelGGL(k_noise_compact, dim3(nb, B), dim3(256), 0, st, d, L);

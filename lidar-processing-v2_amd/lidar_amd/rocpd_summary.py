@@ -1,0 +1,66 @@
+"""Per-kernel summaries of rocprofv3 result databases (rocpd SQLite).
+
+  python -m lidar_amd.rocpd_summary trace <run_results.db> [frames_per_launch]
+      kernel durations (calls, total, average, min, max), optionally only the
+      launches whose grid covers that many frames (grid y / block y)
+  python -m lidar_amd.rocpd_summary pmc <run_results.db>
+      per-kernel counter values averaged per dispatch
+"""
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def _short(name):
+    m = re.search(r"(k_\w+)", name)
+    if m:
+        return m.group(1)
+    return name.replace("void ", "")[:40]
+
+
+def trace(db, frames=None):
+    con = sqlite3.connect(db)
+    agg = {}
+    for name, dur, gy, wy in con.execute("select name, duration, grid_y, workgroup_y from kernels"):
+        nb = gy // max(wy, 1) if gy else 1
+        if frames is not None and nb != frames:
+            continue
+        a = agg.setdefault(_short(name), [0, 0.0, float("inf"), 0.0])
+        a[0] += 1
+        a[1] += dur
+        a[2] = min(a[2], dur)
+        a[3] = max(a[3], dur)
+    tot = sum(v[1] for v in agg.values()) or 1.0
+    lines = [f"{'kernel':32s} {'calls':>6s} {'total_ms':>9s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}"]
+    for k, (c, t, mn, mx) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"{k[:32]:32s} {c:6d} {t/1e6:9.3f} {t/c/1e3:9.2f} {mn/1e3:9.2f} {mx/1e3:9.2f} {100*t/tot:6.2f}")
+    return "\n".join(lines)
+
+
+def pmc(db):
+    con = sqlite3.connect(db)
+    per = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
+    dur = defaultdict(dict)
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, value, duration, grid_size from counters_collection")
+    for did, name, cn, v, du, gs in rows:
+        k = _short(name) + ("" if "k_" in name else f"[grid {gs}]")
+        per[k][cn] += v
+        disp[k].add(did)
+        dur[k][did] = du
+    names = sorted({cn for k in per for cn in per[k]})
+    lines = [f"{'kernel':40s} {'disp':>5s} {'avg_us':>9s} " + " ".join(f"{n:>14s}" for n in names)]
+    for k in sorted(per, key=lambda k: -sum(dur[k].values())):
+        nd = len(disp[k])
+        lines.append(f"{k[:40]:40s} {nd:5d} {sum(dur[k].values())/nd/1e3:9.1f} " +
+                     " ".join(f"{per[k][n]/nd:14.1f}" for n in names))
+    return "\n".join(lines)
+
+
+if __name__ == "__main__":
+    mode, db = sys.argv[1], sys.argv[2]
+    if mode == "trace":
+        print(trace(db, int(sys.argv[3]) if len(sys.argv) > 3 else None))
+    else:
+        print(pmc(db))
