@@ -208,6 +208,11 @@ constexpr uint64_t kIdentityMap = 0x876543210ull;
 constexpr int kSolveThreads = 256;
 constexpr int kSolveWaves = kSolveThreads / kWave;
 constexpr int kFinLds = 65536;
+// LDS label table of the solve: entries 0, 1, 2 are the static labels of
+// sources kSrcObstacle, kSrcGround, kSrcUnknown; queue pixel j is entry j + 3.
+constexpr int kFinBase = 3;
+static_assert(kSrcObstacle + kFinBase == 0 && kSrcGround + kFinBase == 1 && kSrcUnknown + kFinBase == 2,
+              "static sources map to the first three label entries");
 
 // wg += w for a GROUND slot, wo += w for an OBSTACLE slot (segmenter.cpp:
 // 613-626).  Written as two selects: every weight is +0 or positive and the
@@ -260,18 +265,15 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
     // Static and earlier-row labels.  A source in this row can only be the
     // queue pixel before (j-1) or two before (j-2), through slots (0,-2) and
     // (0,-1) (indices 10, 11): stale slots always read earlier rows.
+    // `fin` holds the three static labels below the queue states (entry
+    // sv + kFinBase), so every slot is one unconditional LDS read; sources in
+    // this row (sv >= rs) read as pending here and come from the pair state.
     uint32_t lab[24];
 #pragma unroll
     for (int i = 0; i < 24; ++i) {
         const int32_t sv = src[i];
-        uint32_t l = kSU;
-        if (sv == kSrcGround)
-            l = kSG;
-        else if (sv == kSrcObstacle)
-            l = kSO;
-        else if (sv >= 0 && sv < rs)
-            l = fin[sv];
-        lab[i] = l;
+        const uint32_t fv = fin[sv + kFinBase];
+        lab[i] = sv < rs ? fv : kSU;
     }
     // Only sources inside this row come from the scan's pair state; a stale
     // column slot at column 0 or 1 can name the last queue pixels of the
@@ -314,7 +316,7 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
 // of its queue pixels, a wavefront shuffle scan of the maps, one LDS step
 // across the four wavefronts.
 __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L) {
-    __shared__ uint8_t fin[kFinLds];
+    __shared__ uint8_t fin[kFinBase + kFinLds];
     __shared__ int row_start[1025];
     __shared__ uint64_t wtot[kSolveWaves];
     const int f = blockIdx.x;
@@ -328,13 +330,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
         if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrQueueCapacity);
         return;
     }
+    if (nq == 0) return;
     const size_t qo = (size_t)f * L.P;
     const int32_t* ql = d.qlist + qo;
     for (int r = tid; r <= s.H; r += kSolveThreads) row_start[r] = lower_bound_i32(ql, nq, r * s.W);
+    if (tid < kFinBase) fin[tid] = static_cast<uint8_t>(tid == 0 ? kSO : (tid == 1 ? kSG : kSU));
     // Chunk bases run rs, rs + T, ... below re, then the next non-empty row
     // starts at re: the first chunk starts at 0 and each next base is known.
+    // Slot loads are unconditional (index clamped to the last queue pixel):
+    // with the same number of loads in flight on every path the compiler
+    // waits for the current chunk's data only, not for the prefetch.
     SlotData cur, nxt;
-    if (tid < nq) load_slots(d, qo, tid, cur);
+    load_slots(d, qo, min(tid, nq - 1), cur);
     __syncthreads();
     for (int r = 0; r < s.H; ++r) {
         const int rs = row_start[r], re = row_start[r + 1];
@@ -343,7 +350,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
         for (int base = rs; base < re; base += kSolveThreads) {
             const int j = base + tid;
             const int jn = ((base + kSolveThreads < re) ? base + kSolveThreads : re) + tid;
-            if (jn < nq) load_slots(d, qo, jn, nxt);
+            load_slots(d, qo, min(jn, nq - 1), nxt);
             uint64_t m = j < re ? pixel_map(cur, j, rs, fin) : kIdentityMap;
             // Inclusive scan inside the wavefront: m := m_j o ... o m_first.
 #pragma unroll
@@ -357,7 +364,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
             for (int q = 0; q < w; ++q) pre = map_compose(wtot[q], pre);
             m = map_compose(m, pre);
             const uint32_t ps = static_cast<uint32_t>(m >> (4 * carry)) & 0xf;
-            if (j < re) fin[j] = static_cast<uint8_t>(ps % 3);
+            if (j < re) fin[kFinBase + j] = static_cast<uint8_t>(ps % 3);
             uint64_t all = kIdentityMap;
             for (int q = 0; q < kSolveWaves; ++q) all = map_compose(wtot[q], all);
             carry = static_cast<uint32_t>(all >> (4 * carry)) & 0xf;
@@ -366,7 +373,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
         }
     }
     for (int j = tid; j < nq; j += kSolveThreads) {
-        const uint32_t v = fin[j];
+        const uint32_t v = fin[kFinBase + j];
         d.fin[qo + j] = v == kSG ? kGround : (v == kSO ? kObstacle : kPending);
     }
 }
