@@ -176,7 +176,7 @@ __device__ int half_chain(const double* X, const double* Y, const uint16_t* P, i
     return kk;
 }
 
-// Splits the n >= 3 sorted points P by the line from P[0] to P[n-1]: LO gets
+// Splits the n >= 2 sorted points P by the line from P[0] to P[n-1]: LO gets
 // P[0], the points strictly below it (in order), P[n-1]; UP the same with the
 // points strictly above.  Only those can be vertices of the lower / upper
 // chain, so each chain sees about half the points.  One wavefront; returns
@@ -447,9 +447,44 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
                 wave_sync();
             }
         }
+        // Exact duplicates (same x and y) are adjacent after the sort, the
+        // smallest index first.  A second copy of a point is never a hull
+        // vertex of a set with two or more distinct points (the reference's
+        // chain pops it: its turn is zero), so only the first copy stays;
+        // a list of one repeated point keeps two copies, the reference's
+        // result for it.  Vertical surfaces repeat one (x, y) once per beam,
+        // so this often shrinks a chunk several times over.
+        const int ns_in = ns;
+        if (ns >= 3) {
+            int nd = 0;
+            for (int i0 = 0; i0 < ns; i0 += kWave) {
+                const int i = i0 + lane;
+                bool keep = false;
+                int p = 0;
+                if (i < ns) {
+                    p = P[i];
+                    const int q = P[i > 0 ? i - 1 : 0];
+                    keep = i == 0 || X[p] != X[q] || Y[p] != Y[q];
+                }
+                const uint64_t mk = __ballot(keep);
+                if (keep) S[nd + lanes_below(mk)] = static_cast<uint16_t>(p);
+                nd += static_cast<int>(__popcll(mk));
+            }
+            if (nd == 1) {
+                if (lane == 0) S[1] = P[1];
+                nd = 2;
+            }
+            wave_sync();
+            for (int i = lane; i < nd; i += kWave) P[i] = S[i];
+            wave_sync();
+            ns = nd;
+        }
         // Lane 0 the lower chain, lane 1 the upper one (in S + kHullChunk);
         // vertex i of the hull is lower[i] for i < kl, then upper[i - kl + 1].
-        if (ns < 3) {
+        // Fewer than three input points take the reference's short path; two
+        // distinct points left after the duplicates go through the chain,
+        // which returns them in (x, y) order.
+        if (ns_in < 3) {
             if (lane == 0) {
                 uint32_t inexact = 0;
                 nlo[w] = chain_hull(X, Y, I, P, ns, S, &inexact);
@@ -599,56 +634,137 @@ __global__ void k_hull_check(DevBuf d, Layout L) {
 }
 
 // Hull vertices as cluster-local indices: rank of the obstacle index among
-// the cluster's members (the gather order of processor.cpp:640-658).  One
-// wavefront per output vertex.  Among members with exactly the vertex's
-// coordinates the smallest index is reported: which of several identical
-// points a chain keeps depends on how the rounds split the cluster (and the
-// reference's own choice follows its unstable std::sort), so the coordinates
-// are the defined result and the index is made canonical.  Coordinates come
-// from the obstacle points, or for standalone lists from the copy staged in
-// `list_xy` (list offset + list-local index).
+// the cluster's members (the gather order of processor.cpp:640-658).  Among
+// members with exactly a vertex's coordinates the smallest index is reported:
+// which of several identical points a chain keeps depends on how the rounds
+// split a cluster (and the reference's own choice follows its unstable
+// std::sort), so the coordinates are the defined result and the index is made
+// canonical.  Coordinates come from the obstacle points, or for standalone
+// lists from the copy staged in `list_xy` (list offset + list-local index).
+//
+// One workgroup per cluster: the hull (up to kLocalMax vertices) is staged in
+// LDS; one pass over the members lowers each vertex's index to the smallest
+// member index with its coordinates, a second pass bins every member by the
+// number of (sorted) vertex indices not above it, and the prefix sums of the
+// bins are the ranks.  Larger hulls take one wavefront per vertex.
+constexpr int kLocalMax = 512;
+
+__device__ __forceinline__ void member_xy(const DevBuf& d, const double2* list_xy, size_t ob, uint32_t base, int32_t m,
+                                          double& x, double& y) {
+    if (list_xy) {
+        x = list_xy[base + m].x;
+        y = list_xy[base + m].y;
+    } else {
+        x = d.ox[ob + m];
+        y = d.oy[ob + m];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const double2* list_xy) {
+    __shared__ int32_t canon[kLocalMax];
+    __shared__ int32_t sorted[kLocalMax];
+    __shared__ int16_t spos[kLocalMax];
+    __shared__ double vx[kLocalMax];
+    __shared__ double vy[kLocalMax];
+    __shared__ uint32_t bins[kLocalMax + 1];
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
-    const uint32_t total = d.fs[f * kFsCount + kFsHullTotal];
-    const int w = threadIdx.x / kWave;
+    const int tid = threadIdx.x;
+    const int w = tid / kWave;
     const int lane = lane_id();
     const size_t ob = (size_t)f * L.N;
-    for (uint32_t v = blockIdx.x * kWavesPerBlock + w; v < total; v += gridDim.x * kWavesPerBlock) {
-        const int c = upper_bound_u32(d.hull_off + ob, K, v) - 1;
-        const uint32_t i = v - d.hull_off[ob + c];
+    for (int c = blockIdx.x; c < K; c += gridDim.x) {
+        const int V = static_cast<int>(d.hull_count[ob + c]);
+        const uint32_t hoff = d.hull_off[ob + c];
         const uint32_t base = d.cl_off[ob + c];
         const uint32_t size = d.cl_size[ob + c];
-        const int32_t idx = d.hull_idx[ob + base + i];
-        double vx, vy;
-        if (list_xy) {
-            vx = list_xy[base + idx].x;
-            vy = list_xy[base + idx].y;
-        } else {
-            vx = d.ox[ob + idx];
-            vy = d.oy[ob + idx];
-        }
-        int32_t canon = idx;
-        for (uint32_t q = lane; q < size; q += kWave) {
-            const int32_t m = d.cl_members[ob + base + q];
-            if (m >= canon) continue;
-            double mx, my;
-            if (list_xy) {
-                mx = list_xy[base + m].x;
-                my = list_xy[base + m].y;
-            } else {
-                mx = d.ox[ob + m];
-                my = d.oy[ob + m];
-            }
-            if (mx == vx && my == vy) canon = m;
-        }
+        const int32_t* members = d.cl_members + ob + base;
+        if (V > kLocalMax) {
+            for (int i = w; i < V; i += kWavesPerBlock) {
+                const int32_t idx = d.hull_idx[ob + base + i];
+                double px, py;
+                member_xy(d, list_xy, ob, base, idx, px, py);
+                int32_t cn = idx;
+                for (uint32_t q = lane; q < size; q += kWave) {
+                    const int32_t m = members[q];
+                    if (m >= cn) continue;
+                    double mx, my;
+                    member_xy(d, list_xy, ob, base, m, mx, my);
+                    if (mx == px && my == py) cn = m;
+                }
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) canon = min(canon, __shfl_xor(canon, o, kWave));
-        d.hull_idx[ob + base + i] = canon;
-        uint32_t cnt = 0;
-        for (uint32_t q = lane; q < size; q += kWave) cnt += d.cl_members[ob + base + q] < canon ? 1u : 0u;
-        cnt = wave_sum(cnt);
-        if (lane == 0) d.hull_local[ob + v] = static_cast<int32_t>(cnt);
+                for (int o = kWave / 2; o > 0; o >>= 1) cn = min(cn, __shfl_xor(cn, o, kWave));
+                uint32_t cnt = 0;
+                for (uint32_t q = lane; q < size; q += kWave) cnt += members[q] < cn ? 1u : 0u;
+                cnt = wave_sum(cnt);
+                if (lane == 0) {
+                    d.hull_idx[ob + base + i] = cn;
+                    d.hull_local[ob + hoff + i] = static_cast<int32_t>(cnt);
+                }
+            }
+            continue;
+        }
+        for (int i = tid; i < V; i += blockDim.x) {
+            const int32_t idx = d.hull_idx[ob + base + i];
+            canon[i] = idx;
+            member_xy(d, list_xy, ob, base, idx, vx[i], vy[i]);
+        }
+        for (int i = tid; i <= V; i += blockDim.x) bins[i] = 0;
+        __syncthreads();
+        for (uint32_t q = tid; q < size; q += blockDim.x) {
+            const int32_t m = members[q];
+            double mx, my;
+            member_xy(d, list_xy, ob, base, m, mx, my);
+            for (int i = 0; i < V; ++i)
+                if (mx == vx[i] && my == vy[i] && m < canon[i]) atomicMin(&canon[i], m);
+        }
+        __syncthreads();
+        // Sorted order of the vertex indices (ties, i.e. one point reported
+        // twice, in vertex order).
+        for (int i = tid; i < V; i += blockDim.x) {
+            const int32_t ci = canon[i];
+            int pos = 0;
+            for (int j = 0; j < V; ++j) {
+                const int32_t cj = canon[j];
+                pos += (cj < ci || (cj == ci && j < i)) ? 1 : 0;
+            }
+            sorted[pos] = ci;
+            spos[i] = static_cast<int16_t>(pos);
+            d.hull_idx[ob + base + i] = ci;
+        }
+        __syncthreads();
+        // Member m lands in bin b = number of sorted indices <= m; it is below
+        // sorted[k] exactly when b <= k.
+        for (uint32_t q = tid; q < size; q += blockDim.x) {
+            const int32_t m = members[q];
+            int lo = 0, hi = V;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sorted[mid] <= m)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            atomicAdd(&bins[lo], 1u);
+        }
+        __syncthreads();
+        if (w == 0) {
+            uint32_t carry = 0;
+            for (int i0 = 0; i0 < V; i0 += kWave) {
+                const int i = i0 + lane;
+                uint32_t v = i < V ? bins[i] : 0u;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t t = __shfl_up(v, o, kWave);
+                    if (lane >= o) v += t;
+                }
+                if (i < V) bins[i] = carry + v;  // members below sorted[i]
+                carry += __shfl(v, kWave - 1, kWave);
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < V; i += blockDim.x) d.hull_local[ob + hoff + i] = static_cast<int32_t>(bins[spos[i]]);
+        __syncthreads();
     }
 }
 
@@ -702,7 +818,7 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
     hipLaunchKernelGGL(k_hull_check, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
-    hipLaunchKernelGGL(k_hull_local, dim3(512, B), dim3(256), 0, st, d, L, list_xy);
+    hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(256), 0, st, d, L, list_xy);
 }
 
 }  // namespace lp

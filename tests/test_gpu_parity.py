@@ -227,6 +227,37 @@ def test_standalone_hulls(ctx, oracle):
         assert np.array_equal(np.stack([x[h], y[h]], 1), np.stack([x[r], y[r]], 1))
 
 
+def test_standalone_hulls_duplicates(ctx, oracle):
+    """Points repeated many times over (a vertical surface gives one (x, y)
+    per beam): same vertex coordinates as the reference chain, and each index
+    is the smallest with its coordinates."""
+    rng = np.random.default_rng(12)
+    lists = []
+    for k in range(30):
+        m = int(rng.integers(1, 200))
+        rep = int(rng.integers(2, 33))
+        bx = rng.normal(0, 3, m).astype(np.float32).astype(np.float64)
+        by = rng.normal(0, 1, m).astype(np.float32).astype(np.float64)
+        if k % 3 == 0:
+            by = bx * 0.25  # collinear columns
+        perm = rng.permutation(m * rep)
+        lists.append((np.repeat(bx, rep)[perm], np.repeat(by, rep)[perm]))
+    # Two distinct points, repeated, the larger one listed first.
+    lists.append((np.array([2.0, 1.0, 2.0, 1.0]), np.zeros(4)))
+    lists.append((np.array([5.0, 5.0, 5.0, 1.0]), np.ones(4)))
+    # Chunks that each hold one repeated point.
+    lists.append((np.r_[np.full(300, 3.0), np.full(100, -1.0)], np.r_[np.full(300, 2.0), np.full(100, 4.0)]))
+    lists.append((np.r_[np.full(100, -1.0), np.full(300, 3.0)], np.r_[np.full(100, 4.0), np.full(300, 2.0)]))
+    lists.append((np.full(700, 1.5), np.full(700, -2.5)))
+    st, hulls = ctx.convex_hulls(lists)
+    assert st == 0
+    for (x, y), h in zip(lists, hulls):
+        r = oracle.convex_hull(x, y)
+        assert np.array_equal(np.stack([x[h], y[h]], 1), np.stack([x[r], y[r]], 1))
+        for i in h:
+            assert i == np.flatnonzero((x == x[i]) & (y == y[i]))[0]
+
+
 def test_cpp_dropin_processor(tmp_path, oracle):
     """The reference processor's calls compiled against the drop-in C++
     classes (cpp/processor_example.cpp) give the oracle's results."""
