@@ -153,6 +153,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.noise, B * N);
     go(x.dsort, B * N);
     go(x.didx, B * N);
+    go(x.nocc, B * kNoiseOccWords);
     go(x.pt_cell, B * N);
     go(x.pt_pix, B * N);
     go(x.labels, B * N);

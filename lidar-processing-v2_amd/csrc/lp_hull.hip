@@ -643,11 +643,16 @@ __global__ void k_hull_check(DevBuf d, Layout L) {
 // lists from the copy staged in `list_xy` (list offset + list-local index).
 //
 // One workgroup per cluster: the hull (up to kLocalMax vertices) is staged in
-// LDS; one pass over the members lowers each vertex's index to the smallest
-// member index with its coordinates, a second pass bins every member by the
-// number of (sorted) vertex indices not above it, and the prefix sums of the
-// bins are the ranks.  Larger hulls take one wavefront per vertex.
+// LDS with a hash of its coordinates; one pass over the members lowers each
+// vertex's index to the smallest member index with its coordinates, a second
+// pass bins every member by the number of (sorted) vertex indices not above
+// it, and the prefix sums of the bins are the ranks.  Each thread takes four
+// members at a time so their loads and searches overlap.  Larger hulls take
+// one wavefront per vertex.
 constexpr int kLocalMax = 512;
+constexpr int kLocalHash = 2 * kLocalMax;
+constexpr int kLocalThreads = 256;
+constexpr int kLocalUnroll = 4;
 
 __device__ __forceinline__ void member_xy(const DevBuf& d, const double2* list_xy, size_t ob, uint32_t base, int32_t m,
                                           double& x, double& y) {
@@ -660,10 +665,20 @@ __device__ __forceinline__ void member_xy(const DevBuf& d, const double2* list_x
     }
 }
 
-__global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const double2* list_xy) {
+// Hash of a coordinate pair that agrees with ==: +0.0 and -0.0 hash alike.
+__device__ __forceinline__ uint32_t xy_hash(double x, double y) {
+    const uint64_t a = static_cast<uint64_t>(__double_as_longlong(x + 0.0));
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(y + 0.0));
+    uint64_t h = a * 0x9E3779B97F4A7C15ull ^ (b + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
+    h ^= h >> 29;
+    return static_cast<uint32_t>(h);
+}
+
+__global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L, const double2* list_xy) {
     __shared__ int32_t canon[kLocalMax];
     __shared__ int32_t sorted[kLocalMax];
     __shared__ int16_t spos[kLocalMax];
+    __shared__ int32_t table[kLocalHash];
     __shared__ double vx[kLocalMax];
     __shared__ double vy[kLocalMax];
     __shared__ uint32_t bins[kLocalMax + 1];
@@ -677,15 +692,15 @@ __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const do
         const int V = static_cast<int>(d.hull_count[ob + c]);
         const uint32_t hoff = d.hull_off[ob + c];
         const uint32_t base = d.cl_off[ob + c];
-        const uint32_t size = d.cl_size[ob + c];
+        const int size = static_cast<int>(d.cl_size[ob + c]);
         const int32_t* members = d.cl_members + ob + base;
         if (V > kLocalMax) {
-            for (int i = w; i < V; i += kWavesPerBlock) {
+            for (int i = w; i < V; i += kLocalThreads / kWave) {
                 const int32_t idx = d.hull_idx[ob + base + i];
                 double px, py;
                 member_xy(d, list_xy, ob, base, idx, px, py);
                 int32_t cn = idx;
-                for (uint32_t q = lane; q < size; q += kWave) {
+                for (int q = lane; q < size; q += kWave) {
                     const int32_t m = members[q];
                     if (m >= cn) continue;
                     double mx, my;
@@ -695,7 +710,7 @@ __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const do
 #pragma unroll
                 for (int o = kWave / 2; o > 0; o >>= 1) cn = min(cn, __shfl_xor(cn, o, kWave));
                 uint32_t cnt = 0;
-                for (uint32_t q = lane; q < size; q += kWave) cnt += members[q] < cn ? 1u : 0u;
+                for (int q = lane; q < size; q += kWave) cnt += members[q] < cn ? 1u : 0u;
                 cnt = wave_sum(cnt);
                 if (lane == 0) {
                     d.hull_idx[ob + base + i] = cn;
@@ -704,24 +719,47 @@ __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const do
             }
             continue;
         }
-        for (int i = tid; i < V; i += blockDim.x) {
-            const int32_t idx = d.hull_idx[ob + base + i];
-            canon[i] = idx;
-            member_xy(d, list_xy, ob, base, idx, vx[i], vy[i]);
-        }
-        for (int i = tid; i <= V; i += blockDim.x) bins[i] = 0;
+        const uint32_t hmask = static_cast<uint32_t>(kLocalHash - 1);
+        for (int i = tid; i < kLocalHash; i += kLocalThreads) table[i] = -1;
+        for (int i = tid; i <= V; i += kLocalThreads) bins[i] = 0;
         __syncthreads();
-        for (uint32_t q = tid; q < size; q += blockDim.x) {
-            const int32_t m = members[q];
-            double mx, my;
-            member_xy(d, list_xy, ob, base, m, mx, my);
-            for (int i = 0; i < V; ++i)
-                if (mx == vx[i] && my == vy[i] && m < canon[i]) atomicMin(&canon[i], m);
+        for (int i = tid; i < V; i += kLocalThreads) {
+            const int32_t idx = d.hull_idx[ob + base + i];
+            double x, y;
+            member_xy(d, list_xy, ob, base, idx, x, y);
+            canon[i] = idx;
+            vx[i] = x;
+            vy[i] = y;
+            uint32_t h = xy_hash(x, y) & hmask;  // load <= 1/2: a free slot exists
+            while (atomicCAS(&table[h], -1, i) != -1) h = (h + 1) & hmask;
+        }
+        __syncthreads();
+        // Smallest member index per vertex coordinates.
+        for (int q0 = tid; q0 < size; q0 += kLocalThreads * kLocalUnroll) {
+            int32_t m[kLocalUnroll];
+            double mx[kLocalUnroll], my[kLocalUnroll];
+#pragma unroll
+            for (int u = 0; u < kLocalUnroll; ++u) {
+                const int q = q0 + u * kLocalThreads;
+                m[u] = q < size ? members[q] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < kLocalUnroll; ++u)
+                if (m[u] >= 0) member_xy(d, list_xy, ob, base, m[u], mx[u], my[u]);
+#pragma unroll
+            for (int u = 0; u < kLocalUnroll; ++u) {
+                if (m[u] < 0) continue;
+                for (uint32_t h = xy_hash(mx[u], my[u]) & hmask;; h = (h + 1) & hmask) {
+                    const int e = table[h];
+                    if (e < 0) break;
+                    if (vx[e] == mx[u] && vy[e] == my[u] && m[u] < canon[e]) atomicMin(&canon[e], m[u]);
+                }
+            }
         }
         __syncthreads();
         // Sorted order of the vertex indices (ties, i.e. one point reported
         // twice, in vertex order).
-        for (int i = tid; i < V; i += blockDim.x) {
+        for (int i = tid; i < V; i += kLocalThreads) {
             const int32_t ci = canon[i];
             int pos = 0;
             for (int j = 0; j < V; ++j) {
@@ -734,18 +772,45 @@ __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const do
         }
         __syncthreads();
         // Member m lands in bin b = number of sorted indices <= m; it is below
-        // sorted[k] exactly when b <= k.
-        for (uint32_t q = tid; q < size; q += blockDim.x) {
-            const int32_t m = members[q];
-            int lo = 0, hi = V;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (sorted[mid] <= m)
-                    lo = mid + 1;
-                else
-                    hi = mid;
+        // sorted[k] exactly when b <= k.  Neighbouring lanes mostly share a
+        // bin: each run of equal bins adds its length once.
+        int steps = 0;
+        while ((1 << steps) <= V) ++steps;
+        // Wave-uniform trip count: the run detection below reads all lanes.
+        for (int q0 = w * kWave + lane; q0 - lane < size; q0 += kLocalThreads * kLocalUnroll) {
+            int32_t m[kLocalUnroll];
+            int lo[kLocalUnroll], hi[kLocalUnroll];
+#pragma unroll
+            for (int u = 0; u < kLocalUnroll; ++u) {
+                const int q = q0 + u * kLocalThreads;
+                m[u] = q < size ? members[q] : -1;
+                lo[u] = 0;
+                hi[u] = V;
             }
-            atomicAdd(&bins[lo], 1u);
+            for (int st = 0; st < steps; ++st) {
+#pragma unroll
+                for (int u = 0; u < kLocalUnroll; ++u) {
+                    if (lo[u] < hi[u]) {
+                        const int mid = (lo[u] + hi[u]) >> 1;
+                        if (sorted[mid] <= m[u])
+                            lo[u] = mid + 1;
+                        else
+                            hi[u] = mid;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kLocalUnroll; ++u) {
+                const int b = m[u] >= 0 ? lo[u] : -1;
+                const int prev = __shfl_up(b, 1, kWave);
+                const bool start = lane == 0 || b != prev;
+                const uint64_t starts = __ballot(start);
+                if (start && b >= 0) {
+                    const uint64_t above = starts & ~((2ull << lane) - 1ull);
+                    const int next = above ? __ffsll(static_cast<unsigned long long>(above)) - 1 : kWave;
+                    atomicAdd(&bins[b], static_cast<uint32_t>(next - lane));
+                }
+            }
         }
         __syncthreads();
         if (w == 0) {
@@ -763,7 +828,7 @@ __global__ __launch_bounds__(256) void k_hull_local(DevBuf d, Layout L, const do
             }
         }
         __syncthreads();
-        for (int i = tid; i < V; i += blockDim.x) d.hull_local[ob + hoff + i] = static_cast<int32_t>(bins[spos[i]]);
+        for (int i = tid; i < V; i += kLocalThreads) d.hull_local[ob + hoff + i] = static_cast<int32_t>(bins[spos[i]]);
         __syncthreads();
     }
 }
@@ -818,7 +883,7 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
     hipLaunchKernelGGL(k_hull_check, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
-    hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(256), 0, st, d, L, list_xy);
+    hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(kLocalThreads), 0, st, d, L, list_xy);
 }
 
 }  // namespace lp

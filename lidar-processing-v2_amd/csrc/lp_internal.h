@@ -22,6 +22,7 @@ constexpr int kSmallCell = 512;       // cells up to this size are sorted by one
 constexpr int kBigCell = 8192;        // larger cells: one workgroup; beyond: capacity error
 constexpr int kHullChunk = 256;       // points per chunk-hull wave
 constexpr int kHullFinal = 8192;      // candidates a final hull workgroup can hold
+constexpr int kNoiseOccWords = 1024 * 1024 / 32;  // noise filter: 1024 x 1024 column bits per frame
 
 // Per-frame error bits (device), mapped to status codes on fetch.
 enum : uint32_t {
@@ -93,6 +94,7 @@ struct DevBuf {
     uint8_t* noise;
     float4* dsort;
     int32_t* didx;
+    uint32_t* nocc;          // per frame: kNoiseOccWords words, one bit per occupied 1 m (x, y) column
     // per point
     int32_t* pt_cell;
     int32_t* pt_pix;
@@ -191,6 +193,7 @@ enum FsSlot : int {
     // kept across the per-run clear: written by the noise filter before it
     kFsKept,           // points kept by the noise filter
     kFsNoise,          // points labelled NOISE
+    kFsNoiseHard,      // points left to the neighbour-cell search
     kFsCount
 };
 

@@ -9,10 +9,16 @@
 //
 // Layout: the frame's points are bucketed into 1 m grid cells through an
 // open-addressing hash of cell keys (one slot per occupied cell, load <= 0.5),
-// counted, scanned and scattered into a cell-sorted float4 array.  One thread
-// per point (in cell-sorted order, so a wavefront reads neighbouring cells)
-// visits its own cell and then the cube of cells within ceil(r) + 1 and stops
-// at the min_neighbours-th hit.
+// counted, scanned and scattered into a cell-sorted float4 array (w holds the
+// cell's hash slot).  The count runs in two passes.  The first, one thread per
+// point in cell order, checks the point's own cell outward from its own
+// position (points next to each other in scan order are mostly neighbours in
+// space), so almost every point of a dense surface reaches min_neighbours
+// after a few distance tests; a point whose ball lies inside its cell is
+// decided there too.  The rest (rain, isolated returns, points near a cell
+// face) are listed and the second pass, one thread per listed point, visits
+// the cube of cells within ceil(r) + 1 nearest first: the slow searches run
+// side by side instead of holding up wavefronts of easy points.
 #include "lp_device.h"
 #include "lp_internal.h"
 #include "lp_kernels.h"
@@ -77,9 +83,11 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
         d.hmin[o] = 0;
         d.hcount[o] = 0;
     }
+    for (int q = t0; q < kNoiseOccWords; q += stride) d.nocc[(size_t)f * kNoiseOccWords + q] = 0;
     if (t0 == 0) {
         d.fs[f * kFsCount + kFsKept] = 0;
         d.fs[f * kFsCount + kFsNoise] = 0;
+        d.fs[f * kFsCount + kFsNoiseHard] = 0;
     }
 }
 
@@ -113,6 +121,7 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
                 h = (h + 1) & mask;
             }
             atomicAdd(&d.hcount[ho + h], static_cast<uint32_t>(__popcll(grp)));
+            atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
         }
         h = __shfl(h, leader, kWave);
         if (mine) slot = static_cast<int32_t>(h);
@@ -132,7 +141,7 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const int32_t h = d.vslot[(size_t)f * L.N + i];
     const int pos = d.hdense[ho + h] + static_cast<int>(wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h])));
     const size_t po = (size_t)f * L.N;
-    d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], 0.0f);
+    d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
     d.didx[po + pos] = i;
 }
 
@@ -146,6 +155,21 @@ __device__ __forceinline__ float cell_gap_sqr(float fx, float fy, float fz, int 
     return gx * gx + gy * gy + gz * gz;
 }
 
+// Writes the label of point i and counts NOISE labels per wavefront.
+__device__ __forceinline__ void noise_label(const DevBuf& d, int f, size_t po, int i, bool noise) {
+    d.noise[po + i] = noise ? 1 : 0;
+    d.obs_flag[po + i] = noise ? 0u : 1u;  // keep flag for the compaction scan
+    const uint64_t m = __ballot(noise);
+    if (m && lane_id() == __ffsll(static_cast<unsigned long long>(m)) - 1)
+        atomicAdd(&d.fs[f * kFsCount + kFsNoise], static_cast<uint32_t>(__popcll(m)));
+}
+
+constexpr int kOwnCellBudget = 64;  // distance tests of the own-cell pass
+
+// Own-cell pass.  A point is VALID once min_neighbours points of its cell
+// are within its radius (the count includes itself, and the search order
+// does not change the outcome), NOISE when its whole cell has been checked
+// and no other cell comes within its radius, and listed otherwise.
 __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
@@ -155,65 +179,154 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
     const NoiseConst nc = *d.nc;
     const size_t po = (size_t)f * L.N;
     const size_t ho = (size_t)f * L.hash_cap;
-    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const float4 t = d.dsort[po + pos];
     const int i = d.didx[po + pos];
     const float r2 = noise_radius_sqr(nc, t.x, t.y);
     const uint32_t k = nc.k;
-    uint32_t cnt = 0;
-    if (k > 0 && r2 >= 0.0f) {
+    int state;  // 0 VALID, 1 NOISE, 2 listed
+    uint32_t carry = 0;  // listed: own-cell count and how far the scan got
+    if (k == 0) {
+        state = 0;
+    } else if (!(r2 >= 0.0f)) {
+        state = 1;  // a NaN radius admits no point
+    } else {
+        const int h = __float_as_int(t.w);
+        const int s0 = d.hdense[ho + h];
+        const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
+        uint32_t cnt = 0;
+        int budget = kOwnCellBudget;
+        int up = pos, dn = pos - 1;
+        while (cnt < k && budget > 0 && (up < s1 || dn >= s0)) {
+            // Alternate after and before the point (its scan-order neighbours).
+            const bool fwd = up < s1 && (dn < s0 || ((budget & 1) == 0));
+            const int s = fwd ? up++ : dn--;
+            cnt += noise_dist(t, d.dsort[po + s]) <= r2 ? 1u : 0u;
+            --budget;
+        }
+        if (cnt >= k) {
+            state = 0;
+        } else {
+            state = 2;
+            carry = cnt | static_cast<uint32_t>(up - pos) << 8 | static_cast<uint32_t>(pos - 1 - dn) << 16;
+            const float r = sqrtf(r2);
+            int cx, cy, cz;
+            noise_cell(t.x, t.y, t.z, cx, cy, cz);
+            const bool whole = up >= s1 && dn < s0;
+            const bool interior = cx > 0 && cx < 1023 && cy > 0 && cy < 1023 && cz > 0 && cz < 2047;
+            if (whole && interior && r < static_cast<float>(kNoiseMaxExtent - 1)) {
+                const float fx = t.x - floorf(t.x), fy = t.y - floorf(t.y), fz = t.z - floorf(t.z);
+                const float reach = (r + kNoiseCellSlack) * (r + kNoiseCellSlack);
+                if (cell_gap_sqr(fx, fy, fz, -1, 0, 0) > reach && cell_gap_sqr(fx, fy, fz, 1, 0, 0) > reach &&
+                    cell_gap_sqr(fx, fy, fz, 0, -1, 0) > reach && cell_gap_sqr(fx, fy, fz, 0, 1, 0) > reach &&
+                    cell_gap_sqr(fx, fy, fz, 0, 0, -1) > reach && cell_gap_sqr(fx, fy, fz, 0, 0, 1) > reach)
+                    state = 1;
+            }
+        }
+    }
+    if (state == 2) {
+        const uint32_t q = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseHard]);
+        d.vslot[po + q] = pos;
+        d.obs_src[po + q] = static_cast<int32_t>(carry);
+    } else {
+        noise_label(d, f, po, i, state == 1);
+    }
+}
+
+// Neighbour-cell pass over the listed points, one wavefront per point (a
+// grid-stride loop over the list).  The lanes scan cell ranges 64 points at a
+// time and count hits with a ballot, so a point next to a large cell costs
+// a few steps instead of one lane walking the cell.  The own cell continues
+// from where the first pass left off ([dn, up) was checked); then the cube of
+// floor(r + 1/64) + 1 cells each way, shell by shell (Chebyshev radius), the
+// lanes testing 64 cells at once: box distance within the radius, column
+// occupied (one bit per 1 m (x, y) column), cell present in the hash.  The
+// search stops at the min_neighbours-th hit.  Radii of 3 m and more (ranges
+// beyond ~150 m) scan the whole frame.
+__device__ __forceinline__ uint32_t wave_count_range(const DevBuf& d, size_t po, const float4& t, float r2, int a,
+                                                     int b, uint32_t cnt, uint32_t k) {
+    const int lane = lane_id();
+    for (int base = a; base < b && cnt < k; base += kWave) {
+        const int s = base + lane;
+        const bool hit = s < b && noise_dist(t, d.dsort[po + s]) <= r2;
+        cnt += static_cast<uint32_t>(__popcll(__ballot(hit)));
+    }
+    return cnt;
+}
+
+__global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int n = fd.n_raw;
+    const int nhard = static_cast<int>(d.fs[f * kFsCount + kFsNoiseHard]);
+    const int lane = lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    const NoiseConst nc = *d.nc;
+    const size_t po = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    const uint32_t k = nc.k;
+    const uint32_t* occ = d.nocc + (size_t)f * kNoiseOccWords;
+    uint32_t noise_here = 0;
+    for (int q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < nhard; q += waves) {
+        const int pos = d.vslot[po + q];
+        const uint32_t carry = static_cast<uint32_t>(d.obs_src[po + q]);
+        const float4 t = d.dsort[po + pos];
+        const float r2 = noise_radius_sqr(nc, t.x, t.y);
         const float r = sqrtf(r2);
+        uint32_t cnt;
         if (r < static_cast<float>(kNoiseMaxExtent - 1)) {
+            cnt = carry & 0xffu;
+            const int h0 = __float_as_int(t.w);
+            const int s0 = d.hdense[ho + h0];
+            const int s1 = s0 + static_cast<int>(d.hcount[ho + h0]);
+            const int up = pos + static_cast<int>((carry >> 8) & 0xffu);
+            const int dn = pos - 1 - static_cast<int>((carry >> 16) & 0xffu);
+            cnt = wave_count_range(d, po, t, r2, up, s1, cnt, k);
+            cnt = wave_count_range(d, po, t, r2, s0, dn + 1, cnt, k);
             const int e = static_cast<int>(floorf(r + kNoiseCellSlack)) + 1;
             int cx, cy, cz;
             noise_cell(t.x, t.y, t.z, cx, cy, cz);
             // Position inside the own cell (exact for in-range coordinates;
             // clamped cells give values outside [0, 1), which only make the
-            // cull below more permissive).  The cull keeps every cell within
+            // cull more permissive).  The cull keeps every cell within
             // (r + slack): the float test admits a few ulps beyond r.
             const float fx = t.x - floorf(t.x), fy = t.y - floorf(t.y), fz = t.z - floorf(t.z);
             const float reach = (r + kNoiseCellSlack) * (r + kNoiseCellSlack);
-            bool done = false;
-            // Shells of growing Chebyshev radius: own cell, its 26
-            // neighbours, then the outer shells, nearest first.
-            for (int sh = 0; sh <= e && !done; ++sh) {
-                for (int dz = -sh; dz <= sh && !done; ++dz)
-                    for (int dy = -sh; dy <= sh && !done; ++dy)
-                        for (int dx = -sh; dx <= sh && !done; ++dx) {
-                            if (max(abs(dx), max(abs(dy), abs(dz))) != sh) continue;
-                            if (sh > 0 && cell_gap_sqr(fx, fy, fz, dx, dy, dz) > reach) continue;
-                            const int ax = cx + dx, ay = cy + dy, az = cz + dz;
-                            if (ax < 0 || ax > 1023 || ay < 0 || ay > 1023 || az < 0 || az > 2047) continue;
-                            const int32_t h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
-                            if (h < 0) continue;
-                            const int s0 = d.hdense[ho + h];
-                            const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
-                            // Four points per step, loads issued together;
-                            // counting past the k-th hit leaves the label alone.
-                            int s = s0;
-                            for (; s + 4 <= s1 && !done; s += 4) {
-                                const float4 a = d.dsort[po + s], b = d.dsort[po + s + 1];
-                                const float4 c = d.dsort[po + s + 2], q = d.dsort[po + s + 3];
-                                cnt += (noise_dist(t, a) <= r2 ? 1u : 0u) + (noise_dist(t, b) <= r2 ? 1u : 0u) +
-                                       (noise_dist(t, c) <= r2 ? 1u : 0u) + (noise_dist(t, q) <= r2 ? 1u : 0u);
-                                done = cnt >= k;
-                            }
-                            for (; s < s1 && !done; ++s) {
-                                cnt += noise_dist(t, d.dsort[po + s]) <= r2 ? 1u : 0u;
-                                done = cnt >= k;
-                            }
-                        }
+            for (int sh = 1; sh <= e && cnt < k; ++sh) {
+                const int wd = 2 * sh + 1;
+                const int cube = wd * wd * wd;
+                for (int c0 = 0; c0 < cube && cnt < k; c0 += kWave) {
+                    const int ci = c0 + lane;
+                    const int dz = ci / (wd * wd) - sh, dy = (ci / wd) % wd - sh, dx = ci % wd - sh;
+                    const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                    int32_t h = -1;
+                    if (ci < cube && max(abs(dx), max(abs(dy), abs(dz))) == sh &&
+                        cell_gap_sqr(fx, fy, fz, dx, dy, dz) <= reach && ax >= 0 && ax <= 1023 && ay >= 0 &&
+                        ay <= 1023 && az >= 0 && az <= 2047 && ((occ[ay * 32 + (ax >> 5)] >> (ax & 31)) & 1u))
+                        h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
+                    uint64_t found = __ballot(h >= 0);
+                    while (found && cnt < k) {
+                        const int src = __ffsll(static_cast<unsigned long long>(found)) - 1;
+                        found &= found - 1;
+                        const int hs = __shfl(h, src, kWave);
+                        const int a = d.hdense[ho + hs];
+                        cnt = wave_count_range(d, po, t, r2, a, a + static_cast<int>(d.hcount[ho + hs]), cnt, k);
+                    }
+                }
             }
         } else {
-            // Radius beyond the cell cube (range > ~150 m) or infinite: scan all.
-            for (int s = 0; s < n; ++s)
-                if (noise_dist(t, d.dsort[po + s]) <= r2 && ++cnt >= k) break;
+            // Radius beyond the cell cube (range > ~150 m) or infinite.
+            cnt = wave_count_range(d, po, t, r2, 0, n, 0u, k);
+        }
+        if (lane == 0) {
+            const bool noise = cnt < k;
+            const int i = d.didx[po + pos];
+            d.noise[po + i] = noise ? 1 : 0;
+            d.obs_flag[po + i] = noise ? 0u : 1u;
+            noise_here += noise ? 1u : 0u;
         }
     }
-    const bool noise = cnt < k;
-    d.noise[po + i] = noise ? 1 : 0;
-    d.obs_flag[po + i] = noise ? 0u : 1u;  // keep flag for the compaction scan
-    if (noise) atomicAdd(&d.fs[f * kFsCount + kFsNoise], 1u);
+    if (lane == 0 && noise_here) atomicAdd(&d.fs[f * kFsCount + kFsNoise], noise_here);
 }
 
 // Stable compaction of the kept points into the segmenter's inputs; the
@@ -248,6 +361,10 @@ void launch_noise(lp_context* c, int B, bool compact) {
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_count");
     hipLaunchKernelGGL(k_noise_count, dim3(nb, B), dim3(256), 0, st, d, L);
+    stage_mark(c, "noise_search");
+    // About 2048 wavefronts in all (at least 64 and at most 1024 per frame).
+    const int sb = std::min(256, std::max(16, 512 / B));
+    hipLaunchKernelGGL(k_noise_search, dim3(sb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_compact");
     if (compact) {
         launch_scan(c, B, d.obs_flag, reinterpret_cast<uint32_t*>(d.obs_src), L.N, kCountRawPoints, 0, kFsKept);
