@@ -10,10 +10,16 @@
 // minimum is the box's float `area` member, so near-ties are decided against
 // the float-rounded value exactly as the reference does.
 //
-// One wavefront per polygon: lane 0 walks the antipodal pairs (sequential by
-// nature, O(n)) into LDS; the lanes then evaluate 64 candidates at a time
-// (O(n) each) and lane 0 applies the sequential selection rule to the few
-// that beat the running minimum.
+// One workgroup per polygon.  The first wavefront walks the antipodal pairs
+// (the outer step is sequential; each inner run of j advances is tested on
+// 64 lanes at once) while the second sums the centroid in the reference's
+// order.  The candidates (i, -1) and (i - 1, +1) share one edge with opposite
+// directions, which negates every rotated coordinate exactly and leaves the
+// area bit-identical, so the workgroup evaluates one area per edge, O(n)
+// each; the first wavefront then applies the sequential selection rule over
+// the candidates in the reference's order (64 at a time, with a ballot for
+// those below the running minimum) and recomputes the chosen candidate's
+// extents in its own direction for the corners.
 #include "lp_device.h"
 #include "lp_internal.h"
 #include "lp_kernels.h"
@@ -24,230 +30,280 @@ using namespace lpd;
 
 namespace {
 
-constexpr int kBoxWaves = 4;
-constexpr int kBoxPairs = 512;   // antipodal pairs held per polygon (about 1.5 per vertex)
-constexpr int kBoxVerts = 256;   // polygons up to this size are staged in LDS
-
-struct Cand {
-    double area, ux, uy, mnx, mxx, mny, mxy;
-    int valid;
-};
+constexpr int kBoxThreads = 256;
+constexpr int kBoxVerts = 1024;       // polygon vertices held in LDS (larger: capacity error)
+constexpr int kBoxPairs = 2 * kBoxVerts;  // antipodal pairs (at most 3n/2)
 
 // areaOfTriangle (polygonizer.hpp:228-231), same operand order.
 __device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, double by, double cx, double cy) {
     return fabs((ax * (by - cy) + bx * (cy - ay) + cx * (ay - by)) * 0.5);
 }
 
-// Antipodal pairs of polygon (X, Y, n >= 3) in the reference's order; returns
-// the count, or -1 when the walk exceeds 4n + 8 steps or the LDS capacity.
-// Vertices i, i+1, j and j+1 stay in registers: each step loads one vertex.
+__device__ __forceinline__ int wrap(int k, int n) { return k >= n ? k - n : k; }
+
+// Number of consecutive steps j -> j+1 -> ... the reference's loop
+// `while (area(i, i+1, j+1) > area(i, i+1, j)) ++j` takes from j, counted
+// up to `cap` (one wavefront tests 64 steps at a time).
+__device__ int advance_run(const double* X, const double* Y, int n, int i, int j, int cap) {
+    const int lane = lane_id();
+    const int i1 = wrap(i + 1, n);
+    const double xi = X[i], yi = Y[i], xi1 = X[i1], yi1 = Y[i1];
+    int t = 0;
+    while (t < cap) {
+        const int s = t + lane;
+        const int a = (j + s) % n, b = (j + s + 1) % n;
+        const bool go = tri_area_xy(xi, yi, xi1, yi1, X[b], Y[b]) > tri_area_xy(xi, yi, xi1, yi1, X[a], Y[a]);
+        const uint64_t stop = __ballot(!go);
+        if (stop) return min(cap, t + __ffsll(static_cast<unsigned long long>(stop)) - 1);
+        t += kWave;
+    }
+    return cap;
+}
+
+// Antipodal pairs of polygon (X, Y, n >= 3) in the reference's order
+// (polygonizer.cpp:93-161), by one wavefront; returns the count, or -1 when
+// the walk exceeds 4n + 8 steps or kBoxPairs.
 __device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pairs) {
-    int count = 0;
-    auto push = [&](int a, int b) -> bool {
-        if (count >= kBoxPairs) return false;
-        pairs[count++] = make_int2(a, b);
-        return true;
-    };
-    auto nx = [n](int k) { return k + 1 == n ? 0 : k + 1; };
+    const int lane = lane_id();
     const int i0 = n - 1;
-    int i = 0, j = 1;
-    double xi = X[0], yi = Y[0], xi1 = X[1], yi1 = Y[1];
-    double xj = X[1], yj = Y[1], xj1 = X[2], yj1 = Y[2];
-    auto adv_i = [&]() {
-        i = nx(i);
-        xi = xi1;
-        yi = yi1;
-        const int q = nx(i);
-        xi1 = X[q];
-        yi1 = Y[q];
-    };
-    auto adv_j = [&]() {
-        j = nx(j);
-        xj = xj1;
-        yj = yj1;
-        const int q = nx(j);
-        xj1 = X[q];
-        yj1 = Y[q];
-    };
     const int limit = 4 * n + 8;
-    int guard = 0;
-    // areaOfTriangle(i, i+1, j+1) > areaOfTriangle(i, i+1, j)
-    while (tri_area_xy(xi, yi, xi1, yi1, xj1, yj1) > tri_area_xy(xi, yi, xi1, yi1, xj, yj)) {
-        adv_j();
-        if (++guard > limit) return -1;
+    int guard = 0, count = 0;
+    int i = 0, j = 1;
+    {
+        // Initial advance: every step counts against the guard.
+        const int T = advance_run(X, Y, n, i, j, limit - guard + 1);
+        if (guard + T > limit) return -1;
+        guard += T;
+        j = (j + T) % n;
     }
     const int j0 = j;
     while (i != j0) {
-        adv_i();
-        if (!push(i, j)) return -1;
-        while (tri_area_xy(xi, yi, xi1, yi1, xj1, yj1) > tri_area_xy(xi, yi, xi1, yi1, xj, yj)) {
-            adv_j();
-            if (!(i == j0 && j == i0)) {
-                if (!push(i, j)) return -1;
-            } else {
-                return count;
-            }
-            if (++guard > limit) return -1;
+        i = wrap(i + 1, n);
+        if (count >= kBoxPairs) return -1;
+        if (lane == 0) pairs[count] = make_int2(i, j);
+        ++count;
+        // Inner advance, step t = 1..T: j + t; stop (returning the pairs so
+        // far) when i == j0 and j + t == i0, fail past the guard.
+        const int T = advance_run(X, Y, n, i, j, limit - guard + 1);
+        int t_term = T + 1;
+        if (i == j0) {
+            const int d = ((i0 - j) % n + n) % n;  // first step t >= 1 with j + t == i0 (mod n)
+            const int t_first = d > 0 ? d : n;
+            if (t_first <= T) t_term = t_first;
         }
+        const int t_guard = limit - guard + 1;  // first step that trips the guard
+        if (t_term <= T && t_term <= t_guard) {
+            if (count + t_term - 1 > kBoxPairs) return -1;
+            for (int t = 1 + lane; t < t_term; t += kWave) pairs[count + t - 1] = make_int2(i, (j + t) % n);
+            return count + t_term - 1;
+        }
+        if (t_guard <= T) return -1;
+        if (count + T > kBoxPairs) return -1;
+        for (int t = 1 + lane; t <= T; t += kWave) pairs[count + t - 1] = make_int2(i, (j + t) % n);
+        count += T;
+        guard += T;
+        j = (j + T) % n;
         // areaOfTriangle(j, i+1, j+1) == areaOfTriangle(i, i+1, j)
-        if (tri_area_xy(xj, yj, xi1, yi1, xj1, yj1) == tri_area_xy(xi, yi, xi1, yi1, xj, yj)) {
-            if (!(i == j0 && j == i0)) {
-                if (!push(i, nx(j))) return -1;
-            } else {
-                if (!push(nx(i), j)) return -1;
-            }
+        const int i1 = wrap(i + 1, n), j1 = wrap(j + 1, n);
+        if (tri_area_xy(X[j], Y[j], X[i1], Y[i1], X[j1], Y[j1]) == tri_area_xy(X[i], Y[i], X[i1], Y[i1], X[j], Y[j])) {
+            if (count >= kBoxPairs) return -1;
+            if (lane == 0) pairs[count] = (i == j0 && j == i0) ? make_int2(i1, j) : make_int2(i, j1);
+            ++count;
         }
         if (++guard > limit) return -1;
     }
     return count;
 }
 
-// Polygon k of frame f: vertices VX/VY[vo + off[k] .. off[k+1]) with the
+// Extents of the polygon rotated into the frame of unit direction (ux, uy)
+// about (cx, cy), as the reference computes them (std::min / std::max order).
+struct Extent {
+    double mnx, mxx, mny, mxy;
+};
+
+__device__ __forceinline__ void extent_add(Extent& e, double rx, double ry) {
+    e.mnx = (rx < e.mnx) ? rx : e.mnx;  // std::min(min_x, x)
+    e.mxx = (e.mxx < rx) ? rx : e.mxx;  // std::max(max_x, x)
+    e.mny = (ry < e.mny) ? ry : e.mny;
+    e.mxy = (e.mxy < ry) ? ry : e.mxy;
+}
+
+__device__ __forceinline__ Extent extent_init() {
+    return Extent{1.7976931348623157e308, -1.7976931348623157e308, 1.7976931348623157e308,
+                  -1.7976931348623157e308};
+}
+
+// Candidate edge (idx, idx + off) of the reference: unit direction, or
+// false when shorter than 1e-6.
+__device__ __forceinline__ bool cand_dir(const double* X, const double* Y, int n, int idx, int off, double& ux,
+                                         double& uy) {
+    int nb = idx + off;
+    if (nb < 0)
+        nb += n;
+    else if (nb >= n)
+        nb -= n;
+    const double ex = X[nb] - X[idx];
+    const double ey = Y[nb] - Y[idx];
+    const double len = sqrt(ex * ex + ey * ey);
+    if (len < 1.0e-6) return false;
+    ux = ex / len;
+    uy = ey / len;
+    return true;
+}
+
+// Polygon k of frame f: vertices VX/VY[ob + off[k] .. off[k+1]) with the
 // offsets' end given by `total`.
-__global__ __launch_bounds__(kBoxWaves * kWave) void k_boxes(DevBuf d, Layout L, const double* VX, const double* VY,
-                                                             const uint32_t* off) {
-    __shared__ int2 pairs_l[kBoxWaves][kBoxPairs];
-    __shared__ Cand cand_l[kBoxWaves][kWave];
-    __shared__ int np_l[kBoxWaves];
-    __shared__ double cen_l[kBoxWaves][2];
-    __shared__ double vx_l[kBoxWaves][kBoxVerts];
-    __shared__ double vy_l[kBoxWaves][kBoxVerts];
+__global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const double* VX, const double* VY,
+                                                      const uint32_t* off) {
+    __shared__ double X[kBoxVerts];
+    __shared__ double Y[kBoxVerts];
+    __shared__ double area[kBoxVerts];  // per edge e (e -> e+1); NaN when too short
+    __shared__ int2 pairs[kBoxPairs];
+    __shared__ int np_s;
+    __shared__ double cen_s[2];
     const int f = blockIdx.y;
-    const int w = threadIdx.x / kWave;
+    const int tid = threadIdx.x;
+    const int w = tid / kWave;
     const int lane = lane_id();
     const uint32_t* fs = d.fs + f * kFsCount;
     const int K = static_cast<int>(fs[kFsClusters]);
     const uint32_t total = fs[kFsHullTotal];
     const size_t ob = (size_t)f * L.N;
     if (K > L.box_stride) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
+        if (blockIdx.x == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
         return;
     }
-    int2* pairs = pairs_l[w];
-    Cand* cl = cand_l[w];
-    for (int k = blockIdx.x * kBoxWaves + w; k < K; k += gridDim.x * kBoxWaves) {
+    for (int k = blockIdx.x; k < K; k += gridDim.x) {
         const uint32_t v0 = off[ob + k];
         const uint32_t v1 = k + 1 < K ? off[ob + k + 1] : total;
         const int n = static_cast<int>(v1 - v0);
-        const double* X = VX + ob + v0;
-        const double* Y = VY + ob + v0;
-        if (n <= kBoxVerts) {
-            for (int q = lane; q < n; q += kWave) {
-                vx_l[w][q] = X[q];
-                vy_l[w][q] = Y[q];
-            }
-            wave_sync();
-            X = vx_l[w];
-            Y = vy_l[w];
-        }
         lp_bounding_box box;
         for (int q = 0; q < 8; ++q) box.corners[q] = 0.0;
         box.area = 0.0f;
         box.angle_rad = 0.0f;
         box.is_valid = 0;
         box.reserved = 0;
-        if (n >= 3) {
-            if (lane == 0) {
-                np_l[w] = antipodal_walk(X, Y, n, pairs);
+        if (n > kBoxVerts) {
+            if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
+        } else if (n >= 3) {
+            for (int q = tid; q < n; q += kBoxThreads) {
+                X[q] = VX[ob + v0 + q];
+                Y[q] = VY[ob + v0 + q];
+            }
+            __syncthreads();
+            if (w == 0) {
+                const int np = antipodal_walk(X, Y, n, pairs);
+                if (lane == 0) np_s = np;
+            } else if (w == 1 && lane == 0) {
                 double cx = 0.0, cy = 0.0;
                 for (int q = 0; q < n; ++q) {
                     cx += X[q];
                     cy += Y[q];
                 }
-                cen_l[w][0] = cx / n;
-                cen_l[w][1] = cy / n;
+                cen_s[0] = cx / n;
+                cen_s[1] = cy / n;
             }
-            wave_sync();
-            const int np = np_l[w];
-            const double cx = cen_l[w][0], cy = cen_l[w][1];
+            __syncthreads();
+            const int np = np_s;
+            const double cx = cen_s[0], cy = cen_s[1];
             if (np < 0) {
-                if (lane == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
+                if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
             } else {
-                float best = __builtin_inff();  // float(DBL_MAX)
-                Cand sel{};
-                sel.valid = 0;
-                const int nc = 4 * np;
-                for (int base = 0; base < nc; base += kWave) {
-                    const int c = base + lane;
-                    Cand r{};
-                    r.valid = 0;
-                    if (c < nc) {
-                        const int2 pr = pairs[c >> 2];
-                        const int idx = (c & 2) ? pr.y : pr.x;
-                        int nb = idx + ((c & 1) ? 1 : -1);
-                        if (nb < 0)
-                            nb += n;
-                        else if (nb >= n)
-                            nb -= n;
-                        const double ex = X[nb] - X[idx];
-                        const double ey = Y[nb] - Y[idx];
-                        const double len = sqrt(ex * ex + ey * ey);
-                        if (!(len < 1.0e-6)) {
-                            const double ux = ex / len;
-                            const double uy = ey / len;
-                            double mnx = 1.7976931348623157e308, mxx = -1.7976931348623157e308;
-                            double mny = 1.7976931348623157e308, mxy = -1.7976931348623157e308;
-                            for (int q = 0; q < n; ++q) {
-                                const double tx = X[q] - cx;
-                                const double ty = Y[q] - cy;
-                                const double rx = tx * ux + ty * uy;
-                                const double ry = -tx * uy + ty * ux;
-                                mnx = (rx < mnx) ? rx : mnx;  // std::min(min_x, x)
-                                mxx = (mxx < rx) ? rx : mxx;  // std::max(max_x, x)
-                                mny = (ry < mny) ? ry : mny;
-                                mxy = (mxy < ry) ? ry : mxy;
-                            }
-                            r.area = (mxx - mnx) * (mxy - mny);
-                            r.ux = ux;
-                            r.uy = uy;
-                            r.mnx = mnx;
-                            r.mxx = mxx;
-                            r.mny = mny;
-                            r.mxy = mxy;
-                            r.valid = 1;
+                // Area of every edge in its forward direction.
+                for (int e = tid; e < n; e += kBoxThreads) {
+                    double ux, uy;
+                    double a = __builtin_nan("");
+                    if (cand_dir(X, Y, n, e, 1, ux, uy)) {
+                        Extent ex = extent_init();
+                        for (int q = 0; q < n; ++q) {
+                            const double tx = X[q] - cx;
+                            const double ty = Y[q] - cy;
+                            extent_add(ex, tx * ux + ty * uy, -tx * uy + ty * ux);
                         }
+                        a = (ex.mxx - ex.mnx) * (ex.mxy - ex.mny);
                     }
-                    cl[lane] = r;
-                    wave_sync();
-                    // Only candidates below the round's starting minimum can
-                    // replace the box (the minimum only falls within a round);
-                    // lane 0 applies the rule to them in candidate order.
-                    uint64_t beat = __ballot(r.valid && r.area < static_cast<double>(best));
-                    if (lane == 0) {
+                    area[e] = a;
+                }
+                __syncthreads();
+                if (w == 0) {
+                    // Candidate c: pair c/4, vertex (c & 2 ? j : i), offset (c & 1 ? +1 : -1);
+                    // a candidate replaces the box when its area is below
+                    // the float running minimum (polygonizer.cpp:257).
+                    float best = __builtin_inff();  // float(DBL_MAX)
+                    int sel = -1;
+                    const int nc = 4 * np;
+                    for (int base = 0; base < nc; base += kWave) {
+                        const int c = base + lane;
+                        double a = __builtin_nan("");
+                        if (c < nc) {
+                            const int2 pr = pairs[c >> 2];
+                            const int idx = (c & 2) ? pr.y : pr.x;
+                            a = area[(c & 1) ? idx : (idx == 0 ? n - 1 : idx - 1)];
+                        }
+                        uint64_t beat = __ballot(a < static_cast<double>(best));
                         while (beat) {
                             const int q = __ffsll(static_cast<unsigned long long>(beat)) - 1;
                             beat &= beat - 1;
-                            const Cand& e = cl[q];
-                            if (e.area < static_cast<double>(best)) {
-                                best = static_cast<float>(e.area);
-                                sel = e;
+                            const double aq = __shfl(a, q, kWave);
+                            if (aq < static_cast<double>(best)) {
+                                best = static_cast<float>(aq);
+                                sel = base + q;
                             }
                         }
                     }
-                    best = __shfl(best, 0, kWave);
-                    wave_sync();
-                }
-                if (lane == 0 && sel.valid) {
-                    const double ux = sel.ux, uy = sel.uy;
-                    box.area = best;
-                    box.corners[0] = sel.mnx * ux - sel.mny * uy + cx;
-                    box.corners[1] = sel.mnx * uy + sel.mny * ux + cy;
-                    box.corners[2] = sel.mxx * ux - sel.mny * uy + cx;
-                    box.corners[3] = sel.mxx * uy + sel.mny * ux + cy;
-                    box.corners[4] = sel.mxx * ux - sel.mxy * uy + cx;
-                    box.corners[5] = sel.mxx * uy + sel.mxy * ux + cy;
-                    box.corners[6] = sel.mnx * ux - sel.mxy * uy + cx;
-                    box.corners[7] = sel.mnx * uy + sel.mxy * ux + cy;
-                    box.angle_rad = atan2_approx(static_cast<float>(uy), static_cast<float>(ux));
-                    box.is_valid = 1;
+                    if (sel >= 0) {
+                        // The chosen candidate in its own direction.
+                        const int2 pr = pairs[sel >> 2];
+                        const int idx = (sel & 2) ? pr.y : pr.x;
+                        double ux = 0.0, uy = 0.0;
+                        cand_dir(X, Y, n, idx, (sel & 1) ? 1 : -1, ux, uy);
+                        // Lanes take every 64th vertex; the reduction keeps,
+                        // among equal values, the one of the lowest vertex
+                        // index, which is what the sequential std::min /
+                        // std::max keep (equal values differ at most in the
+                        // sign of a zero).
+                        Extent ex = extent_init();
+                        int qa = n, qb = n, qc = n, qd = n;
+                        for (int q = lane; q < n; q += kWave) {
+                            const double tx = X[q] - cx;
+                            const double ty = Y[q] - cy;
+                            const double rx = tx * ux + ty * uy, ry = -tx * uy + ty * ux;
+                            if (rx < ex.mnx) { ex.mnx = rx; qa = q; }
+                            if (ex.mxx < rx) { ex.mxx = rx; qb = q; }
+                            if (ry < ex.mny) { ex.mny = ry; qc = q; }
+                            if (ex.mxy < ry) { ex.mxy = ry; qd = q; }
+                        }
+#pragma unroll
+                        for (int o = kWave / 2; o > 0; o >>= 1) {
+                            const double a0 = __shfl_xor(ex.mnx, o, kWave), a1 = __shfl_xor(ex.mxx, o, kWave);
+                            const double a2 = __shfl_xor(ex.mny, o, kWave), a3 = __shfl_xor(ex.mxy, o, kWave);
+                            const int b0 = __shfl_xor(qa, o, kWave), b1 = __shfl_xor(qb, o, kWave);
+                            const int b2 = __shfl_xor(qc, o, kWave), b3 = __shfl_xor(qd, o, kWave);
+                            if (a0 < ex.mnx || (a0 == ex.mnx && b0 < qa)) { ex.mnx = a0; qa = b0; }
+                            if (ex.mxx < a1 || (a1 == ex.mxx && b1 < qb)) { ex.mxx = a1; qb = b1; }
+                            if (a2 < ex.mny || (a2 == ex.mny && b2 < qc)) { ex.mny = a2; qc = b2; }
+                            if (ex.mxy < a3 || (a3 == ex.mxy && b3 < qd)) { ex.mxy = a3; qd = b3; }
+                        }
+                        box.area = best;
+                        box.corners[0] = ex.mnx * ux - ex.mny * uy + cx;
+                        box.corners[1] = ex.mnx * uy + ex.mny * ux + cy;
+                        box.corners[2] = ex.mxx * ux - ex.mny * uy + cx;
+                        box.corners[3] = ex.mxx * uy + ex.mny * ux + cy;
+                        box.corners[4] = ex.mxx * ux - ex.mxy * uy + cx;
+                        box.corners[5] = ex.mxx * uy + ex.mxy * ux + cy;
+                        box.corners[6] = ex.mnx * ux - ex.mxy * uy + cx;
+                        box.corners[7] = ex.mnx * uy + ex.mxy * ux + cy;
+                        box.angle_rad = atan2_approx(static_cast<float>(uy), static_cast<float>(ux));
+                        box.is_valid = 1;
+                    }
                 }
             }
         }
-        if (lane == 0) {
+        if (tid == 0) {
             d.boxes[(size_t)f * L.box_stride + k] = box;
             if (box.is_valid) atomicAdd(&d.fs[f * kFsCount + kFsBoxes], 1u);
         }
-        wave_sync();
+        __syncthreads();
     }
 }
 
@@ -286,8 +342,7 @@ void launch_boxes(lp_context* c, int B, bool from_hulls) {
     const size_t BN = (size_t)L.B * L.N;
     stage_mark(c, "boxes");
     if (from_hulls) hipLaunchKernelGGL(k_box_gather, dim3(256, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_boxes, dim3(64, B), dim3(kBoxWaves * kWave), 0, st, d, L, d.hcx + BN, d.hcy + BN,
-                       d.hull_off);
+    hipLaunchKernelGGL(k_boxes, dim3(128, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN, d.hcy + BN, d.hull_off);
 }
 
 }  // namespace lp

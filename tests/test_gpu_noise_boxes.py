@@ -87,6 +87,26 @@ def test_boxes_bit_exact_on_random_polygons(ctx, oracle):
         assert box_equal(boxes[k], ref), k
 
 
+def test_boxes_bit_exact_on_large_and_symmetric_polygons(ctx, oracle):
+    """Hundreds of vertices (long antipodal runs), regular polygons with exact
+    symmetric coordinates (equal triangle areas take the walk's equality
+    branch, equal edge areas test the running-minimum rule) and lattice
+    polygons."""
+    rng = np.random.default_rng(9)
+    polys = [convex_polygon(rng, int(rng.integers(200, 1000))) for _ in range(12)]
+    for m in (4, 6, 8, 12, 16, 64, 360):
+        a = np.arange(m) * (2 * math.pi / m)
+        polys.append(strictly_convex_ccw(np.round(np.cos(a) * 5.0, 6) + 2.0, np.round(np.sin(a) * 5.0, 6) - 1.0))
+    polys.append(strictly_convex_ccw(np.array([0.0, 4.0, 4.0, 0.0]), np.array([0.0, 0.0, 2.0, 2.0])))
+    polys.append(strictly_convex_ccw(np.array([0.0, 2.0, 3.0, 2.0, 0.0, -1.0]), np.array([0.0, 0.0, 1.0, 2.0, 2.0, 1.0])))
+    polys.append(strictly_convex_ccw(np.array([0.0, 1.0, 1.0, 0.0]), np.array([0.0, 0.0, 1.0, 1.0])))
+    st, boxes = ctx.oriented_boxes(polys)
+    assert st == 0
+    for k, (px, py) in enumerate(polys):
+        ref = oracle.obb_rotating_calipers(px, py)
+        assert box_equal(boxes[k], ref), (k, len(px))
+
+
 def full_frame_reference(oracle, f, sco):
     noise, _ = oracle.noise_definition(f.x, f.y, f.z)
     keep = noise == 0
