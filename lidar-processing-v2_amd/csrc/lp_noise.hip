@@ -304,13 +304,28 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
                         cell_gap_sqr(fx, fy, fz, dx, dy, dz) <= reach && ax >= 0 && ax <= 1023 && ay >= 0 &&
                         ay <= 1023 && az >= 0 && az <= 2047 && ((occ[ay * 32 + (ax >> 5)] >> (ax & 31)) & 1u))
                         h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
-                    uint64_t found = __ballot(h >= 0);
-                    while (found && cnt < k) {
-                        const int src = __ffsll(static_cast<unsigned long long>(found)) - 1;
-                        found &= found - 1;
-                        const int hs = __shfl(h, src, kWave);
-                        const int a = d.hdense[ho + hs];
-                        cnt = wave_count_range(d, po, t, r2, a, a + static_cast<int>(d.hcount[ho + hs]), cnt, k);
+                    // The points of all cells found in this batch, 64 at a
+                    // time: point g belongs to the lane whose prefix range
+                    // holds it (binary search over the lanes' inclusive sums).
+                    const int c_n = h >= 0 ? static_cast<int>(d.hcount[ho + h]) : 0;
+                    const int c_a = h >= 0 ? d.hdense[ho + h] : 0;
+                    const int incl = wave_inclusive_sum(c_n);
+                    const int found = __shfl(incl, kWave - 1, kWave);
+                    const int start = c_a - (incl - c_n);  // point g of this lane's cell: start + g
+                    for (int base = 0; base < found && cnt < k; base += kWave) {
+                        const int g = base + lane;
+                        int lo = 0, hi = kWave - 1;
+#pragma unroll
+                        for (int it = 0; it < 6; ++it) {
+                            const int mid = (lo + hi) >> 1;
+                            if (__shfl(incl, mid, kWave) > g)
+                                hi = mid;
+                            else
+                                lo = mid + 1;
+                        }
+                        const int sp = __shfl(start, lo, kWave) + g;
+                        const bool hit = g < found && noise_dist(t, d.dsort[po + sp]) <= r2;
+                        cnt += static_cast<uint32_t>(__popcll(__ballot(hit)));
                     }
                 }
             }
@@ -362,8 +377,9 @@ void launch_noise(lp_context* c, int B, bool compact) {
     stage_mark(c, "noise_count");
     hipLaunchKernelGGL(k_noise_count, dim3(nb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_search");
-    // About 2048 wavefronts in all (at least 64 and at most 1024 per frame).
-    const int sb = std::min(256, std::max(16, 512 / B));
+    // About 8192 wavefronts in all (8 per SIMD: the search is latency-bound),
+    // at least 128 and at most 1024 per frame.
+    const int sb = std::min(256, std::max(32, 2048 / B));
     hipLaunchKernelGGL(k_noise_search, dim3(sb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_compact");
     if (compact) {
