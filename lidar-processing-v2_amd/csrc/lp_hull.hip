@@ -302,6 +302,124 @@ __device__ int chain_hull(const double* X, const double* Y, const int32_t* I, co
     return kk - 1;
 }
 
+// Order-preserving unsigned key of a float (-0 folded onto +0, so equal
+// values give equal keys); NaN is excluded by the caller.
+__device__ __forceinline__ uint32_t float_key(float v) {
+    const uint32_t u = __float_as_uint(v + 0.0f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// Bitonic sort of 64*S (key, value) pairs held in registers, element
+// e = s * 64 + lane in slot s; pairs are distinct (values are unique), so the
+// network needs no tie rule.  Partners within a lane swap registers, others
+// are exchanged with lane shuffles: no LDS round trips.
+template <int S>
+__device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[S]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= kWave * S; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= kWave) {
+                const int js = j / kWave;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s & js) continue;
+                    const int t = s | js;
+                    const bool asc = ((s * kWave) & k) == 0;
+                    const bool gt = key[s] > key[t] || (key[s] == key[t] && val[s] > val[t]);
+                    if (gt == asc) {
+                        const uint64_t tk = key[s];
+                        key[s] = key[t];
+                        key[t] = tk;
+                        const uint32_t tv = val[s];
+                        val[s] = val[t];
+                        val[t] = tv;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const uint64_t ok = __shfl_xor(key[s], j, kWave);
+                    const uint32_t ov = __shfl_xor(val[s], j, kWave);
+                    const bool asc = ((s * kWave + lane) & k) == 0;
+                    const bool lower = (lane & j) == 0;
+                    const bool other_less = ok < key[s] || (ok == key[s] && ov < val[s]);
+                    if (other_less == (lower == asc)) {
+                        key[s] = ok;
+                        val[s] = ov;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Sorts the ns points P[0..ns) of a chunk whose coordinates are all floats
+// (as the obstacle points are) by (x, y) with register keys, drops exact
+// duplicates as described at the call site unless keep_all, and rewrites
+// X, Y, I in sorted order with P the identity.  Returns the new count.
+template <int S>
+__device__ int sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t* P, int ns, bool keep_all) {
+    const int lane = lane_id();
+    uint64_t key[S];
+    uint32_t val[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = s * kWave + lane;
+        if (e < ns) {
+            const int p = P[e];
+            key[s] = (static_cast<uint64_t>(float_key(static_cast<float>(X[p]))) << 32) |
+                     float_key(static_cast<float>(Y[p]));
+            val[s] = static_cast<uint32_t>(p);
+        } else {
+            key[s] = ~0ull;  // padding sorts last
+            val[s] = static_cast<uint32_t>(kHullChunk + e);
+        }
+    }
+    reg_bitonic<S>(key, val);
+    int32_t idx[S];
+    uint64_t prev[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = s * kWave + lane;
+        idx[s] = e < ns ? I[val[s]] : 0;
+        const uint64_t up = __shfl_up(key[s], 1, kWave);
+        const uint64_t wrap = __shfl(key[s > 0 ? s - 1 : 0], kWave - 1, kWave);
+        prev[s] = lane > 0 ? up : wrap;
+    }
+    int nd = 0;
+    uint64_t kmask[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = s * kWave + lane;
+        kmask[s] = __ballot(e < ns && (keep_all || e == 0 || key[s] != prev[s]));
+        nd += static_cast<int>(__popcll(kmask[s]));
+    }
+    if (!keep_all && nd == 1 && ns >= 2) {
+        kmask[0] |= 2ull;  // one repeated point: keep two copies
+        nd = 2;
+    }
+    wave_sync();  // every lane has read I[] before it is rewritten
+    int before = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if ((kmask[s] >> lane) & 1ull) {
+            const int q = before + static_cast<int>(lanes_below(kmask[s]));
+            X[q] = static_cast<double>(key_float(static_cast<uint32_t>(key[s] >> 32)));
+            Y[q] = static_cast<double>(key_float(static_cast<uint32_t>(key[s])));
+            I[q] = idx[s];
+            P[q] = static_cast<uint16_t>(q);
+        }
+        before += static_cast<int>(__popcll(kmask[s]));
+    }
+    wave_sync();
+    return nd;
+}
+
 __device__ __forceinline__ int upper_bound_u32(const uint32_t* a, int n, uint32_t v) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -426,6 +544,32 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             if (keep) P[ns + lanes_below(mask)] = static_cast<uint16_t>(i);
             ns += static_cast<int>(__popcll(mask));
         }
+        // Coordinates that are all floats (every obstacle point) sort as
+        // 64-bit keys in registers; other lists take the LDS sort below.
+        bool as_float = true;
+        for (int i = lane; i < n; i += kWave) {
+            const double x = X[i], y = Y[i];
+            as_float = as_float && static_cast<double>(static_cast<float>(x)) == x &&
+                       static_cast<double>(static_cast<float>(y)) == y;
+        }
+        as_float = __ballot(!as_float) == 0;
+        const int ns_in = ns;
+        // Exact duplicates (same x and y) are adjacent after the sort.  A
+        // second copy of a point is never a hull vertex of a set with two or
+        // more distinct points (the reference's chain pops it: its turn is
+        // zero), so only the first copy stays; a list of one repeated point
+        // keeps two copies, the reference's result for it.  Vertical surfaces
+        // repeat one (x, y) once per beam, so this often shrinks a chunk
+        // several times over.  Fewer than three points are kept as they are.
+        if (as_float) {
+            wave_sync();
+            if (ns <= kWave)
+                ns = sort_float_chunk<1>(X, Y, I, P, ns, ns_in < 3);
+            else if (ns <= 2 * kWave)
+                ns = sort_float_chunk<2>(X, Y, I, P, ns, ns_in < 3);
+            else
+                ns = sort_float_chunk<4>(X, Y, I, P, ns, ns_in < 3);
+        } else {
         int m = 1;
         while (m < ns) m <<= 1;
         for (int i = ns + lane; i < m; i += kWave) P[i] = static_cast<uint16_t>(n + i);  // padding sorts last
@@ -447,14 +591,6 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
                 wave_sync();
             }
         }
-        // Exact duplicates (same x and y) are adjacent after the sort, the
-        // smallest index first.  A second copy of a point is never a hull
-        // vertex of a set with two or more distinct points (the reference's
-        // chain pops it: its turn is zero), so only the first copy stays;
-        // a list of one repeated point keeps two copies, the reference's
-        // result for it.  Vertical surfaces repeat one (x, y) once per beam,
-        // so this often shrinks a chunk several times over.
-        const int ns_in = ns;
         if (ns >= 3) {
             int nd = 0;
             for (int i0 = 0; i0 < ns; i0 += kWave) {
@@ -478,6 +614,7 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             for (int i = lane; i < nd; i += kWave) P[i] = S[i];
             wave_sync();
             ns = nd;
+        }
         }
         // Lane 0 the lower chain, lane 1 the upper one (in S + kHullChunk);
         // vertex i of the hull is lower[i] for i < kl, then upper[i - kl + 1].

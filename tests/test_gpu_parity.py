@@ -219,6 +219,8 @@ def test_standalone_hulls(ctx, oracle):
         if k % 7 == 0:
             x[:] = x[0]
             y[:] = y[0]  # identical
+        if k % 4 == 1:
+            x = x + rng.normal(0, 1e-3, n)  # full double precision (not floats)
         lists.append((x, y))
     st, hulls = ctx.convex_hulls(lists)
     assert st == 0
