@@ -103,6 +103,10 @@ typedef struct lp_capacity {
     int32_t max_image_width;
     int32_t max_voxels;            /* 200000 in the reference (clusterer.cpp:36) */
     int32_t device;                /* HIP device ordinal */
+    int32_t streams;               /* 1..8: a staged batch runs as this many contiguous
+                                      sub-batches, each on its own stream and graph, so the
+                                      latency-bound kernels of one overlap the others'
+                                      work; max_frames is the total over all of them */
 } lp_capacity;
 void lp_capacity_default(lp_capacity* cap);
 
@@ -225,7 +229,11 @@ int lp_process_frames(lp_context* ctx, const lp_segmenter_config* seg_cfgs, int3
 
 /* Split form of lp_process_frames for device-resident pipelines: stage
  * inputs, run (asynchronously on the context stream, captured once per
- * batch shape as a hipGraph and replayed), then fetch results. */
+ * batch shape as a hipGraph and replayed), then fetch results.  A context
+ * created with streams = S > 1 stages the batch as S contiguous shares of
+ * ceil(num_frames / S) frames and runs them on S streams; the context
+ * stream waits for all of them, so lp_synchronize / lp_query and work the
+ * caller queues behind lp_context_stream see the whole batch. */
 int lp_batch_stage(lp_context* ctx, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
                    const lp_clusterer_config* cluster_cfg, int32_t num_frames, const lp_cloud* clouds);
 int lp_batch_run(lp_context* ctx, int32_t use_graph);
@@ -281,15 +289,16 @@ int lp_server_stats_get(lp_server* s, lp_server_stats* st);
 
 /* Timing of the dominant kernel for roofline reporting: average duration (ms)
  * of the last `runs` launches of the named stage, from HIP events recorded on
- * the context stream. */
+ * the context stream.  With streams > 1 it runs the first share of the
+ * staged batch alone. */
 int lp_stage_names(lp_context* ctx, char* buf, int32_t buf_len);
 int lp_profile_run(lp_context* ctx, int32_t runs, float* stage_ms, int32_t max_stages, int32_t* num_stages);
 
 /* Per-context device stream (hipStream_t) for callers that interleave work. */
 void* lp_context_stream(lp_context* ctx);
 
-/* Intermediate state of frame `frame` of the last run, for stage-by-stage
- * parity checks.  No reference counterpart: the reference keeps these in
+/* Intermediate state of frame `frame` of the last run (of the first share
+ * when streams > 1), for stage-by-stage parity checks.  No reference counterpart: the reference keeps these in
  * private members (segmenter.hpp:190-215).  Returns the element count
  * written (or needed, when larger than `capacity`). */
 enum {

@@ -342,12 +342,54 @@ void lp_capacity_default(lp_capacity* cap) {
     cap->max_image_width = 2048;
     cap->max_voxels = 200000;            // clusterer.cpp:36
     cap->device = 0;
+    cap->streams = 1;
 }
 
 const char* lp_last_error(void) { return g_last_error.c_str(); }
 
+namespace {
+
+int create_lane(const lp_capacity* cap, lp_context** out);
+
+}  // namespace
+
 int lp_context_create(const lp_capacity* cap, lp_context** out) {
     if (!cap || !out) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (cap->streams < 1 || cap->streams > 8) return fail(LP_ERR_INVALID_ARGUMENT, "streams must be 1..8");
+    const int S = std::min(cap->streams, std::max(cap->max_frames, 1));
+    lp_capacity lc = *cap;
+    lc.streams = 1;
+    lc.max_frames = (cap->max_frames + S - 1) / S;
+    lp_context* c = nullptr;
+    int st = create_lane(&lc, &c);
+    if (st != LP_OK) return st;
+    for (int k = 1; k < S; ++k) {
+        lp_context* lane = nullptr;
+        st = create_lane(&lc, &lane);
+        hipEvent_t ev = nullptr;
+        if (st == LP_OK) {
+            const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e != hipSuccess) {
+                lp_context_destroy(lane);
+                st = fail(LP_ERR_HIP, std::string("lane event: ") + hipGetErrorString(e));
+            }
+        }
+        if (st != LP_OK) {
+            lp_context_destroy(c);
+            return st;
+        }
+        c->lanes.push_back(lane);
+        c->lane_done.push_back(ev);
+    }
+    *out = c;
+    return LP_OK;
+}
+
+namespace {
+
+// One lane: buffers, stream and graph cache for cap->max_frames frames.
+int create_lane(const lp_capacity* cap, lp_context** out) {
     *out = nullptr;
     if (cap->max_frames < 1 || cap->max_points_per_frame < 1 || cap->max_image_height < 1 ||
         cap->max_image_width < 1 || cap->max_image_height > 1024 || cap->max_voxels < 1)
@@ -396,8 +438,12 @@ int lp_context_create(const lp_capacity* cap, lp_context** out) {
     return LP_OK;
 }
 
+}  // namespace
+
 int lp_context_destroy(lp_context* c) {
     if (!c) return LP_OK;
+    for (auto* lane : c->lanes) lp_context_destroy(lane);
+    for (auto e : c->lane_done) (void)hipEventDestroy(e);
     (void)hipSetDevice(c->device);
     for (auto& g : c->graphs) {
         (void)hipGraphExecDestroy(g.exec);
@@ -466,11 +512,34 @@ int stage_batch(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_
 int lp_batch_stage(lp_context* c, const lp_segmenter_config* seg_cfgs, int32_t num_seg_cfgs,
                    const lp_clusterer_config* ccfg, int32_t num_frames, const lp_cloud* clouds) {
     if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
-    return stage_batch(c, seg_cfgs, num_seg_cfgs, ccfg, num_frames, clouds, c->noise_on);
+    if (c->lanes.empty()) return stage_batch(c, seg_cfgs, num_seg_cfgs, ccfg, num_frames, clouds, c->noise_on);
+    const int S = 1 + static_cast<int>(c->lanes.size());
+    if (!seg_cfgs || !clouds) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_frames < 1 || num_frames > c->L.B * S) return fail(LP_ERR_CAPACITY, "batch larger than max_frames");
+    if (num_seg_cfgs != 1 && num_seg_cfgs != num_frames)
+        return fail(LP_ERR_INVALID_ARGUMENT, "num_seg_cfgs must be 1 or num_frames");
+    // Contiguous shares, as even as possible, lane 0 first.
+    const int g = (num_frames + S - 1) / S;
+    c->lane_frames.assign(S, 0);
+    for (int k = 0; k < S; ++k) {
+        lp_context* lane = k == 0 ? c : c->lanes[k - 1];
+        const int lo = k * g;
+        const int cnt = std::max(0, std::min(g, num_frames - lo));
+        if (cnt == 0) {
+            lane->num_frames = 0;
+            continue;
+        }
+        const int st = stage_batch(lane, num_seg_cfgs == 1 ? seg_cfgs : seg_cfgs + lo, num_seg_cfgs == 1 ? 1 : cnt,
+                                   ccfg, cnt, clouds + lo, lane->noise_on);
+        if (st != LP_OK) return st;
+        c->lane_frames[k] = cnt;
+    }
+    return LP_OK;
 }
 
-int lp_batch_run(lp_context* c, int32_t use_graph) {
-    if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+namespace {
+
+int run_lane(lp_context* c, int32_t use_graph) {
     LP_HIP(hipSetDevice(c->device));
     const int B = c->num_frames;
     if (!use_graph) return run_pipeline(c, B, false);
@@ -502,6 +571,22 @@ int lp_batch_run(lp_context* c, int32_t use_graph) {
     return LP_OK;
 }
 
+}  // namespace
+
+int lp_batch_run(lp_context* c, int32_t use_graph) {
+    if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+    int st = run_lane(c, use_graph);
+    for (size_t k = 0; st == LP_OK && k < c->lanes.size(); ++k) {
+        lp_context* lane = c->lanes[k];
+        if (lane->num_frames < 1) continue;
+        st = run_lane(lane, use_graph);
+        if (st != LP_OK) break;
+        LP_HIP(hipEventRecord(c->lane_done[k], lane->stream));
+        LP_HIP(hipStreamWaitEvent(c->stream, c->lane_done[k], 0));
+    }
+    return st;
+}
+
 int lp_synchronize(lp_context* c) {
     if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
     LP_HIP(hipStreamSynchronize(c->stream));
@@ -517,8 +602,9 @@ int lp_query(lp_context* c) {
     return LP_OK;
 }
 
-int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* results) {
-    if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+namespace {
+
+int fetch_lane(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* results) {
     LP_HIP(hipSetDevice(c->device));
     const int B = c->num_frames;
     const Layout& L = c->L;
@@ -582,6 +668,23 @@ int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_resu
                                   K * sizeof(lp_bounding_box), hipMemcpyDeviceToHost, c->stream));
     }
     LP_HIP(hipStreamSynchronize(c->stream));
+    return overall;
+}
+
+}  // namespace
+
+int lp_batch_fetch(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* results) {
+    if (!c || c->num_frames < 1) return fail(LP_ERR_INVALID_ARGUMENT, "nothing staged");
+    if (c->lanes.empty()) return fetch_lane(c, outputs, results);
+    int overall = LP_OK, off = 0;
+    for (size_t k = 0; k < c->lane_frames.size(); ++k) {
+        const int cnt = c->lane_frames[k];
+        if (cnt == 0) continue;
+        lp_context* lane = k == 0 ? c : c->lanes[k - 1];
+        const int st = fetch_lane(lane, outputs ? outputs + off : nullptr, results ? results + off : nullptr);
+        if (st != LP_OK && overall == LP_OK) overall = st;
+        off += cnt;
+    }
     return overall;
 }
 
@@ -669,16 +772,19 @@ int lp_set_noise_filter(lp_context* c, const lp_noise_config* cfg) {
     if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
     if (!cfg) {
         c->noise_on = false;
+        for (auto* lane : c->lanes) lane->noise_on = false;
         return LP_OK;
     }
     c->nc = noise_const(*cfg);
     c->noise_on = true;
+    for (auto* lane : c->lanes) lp_set_noise_filter(lane, cfg);
     return LP_OK;
 }
 
 int lp_set_oriented_boxes(lp_context* c, int32_t enable) {
     if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
     c->boxes_on = enable != 0;
+    for (auto* lane : c->lanes) lane->boxes_on = c->boxes_on;
     return LP_OK;
 }
 

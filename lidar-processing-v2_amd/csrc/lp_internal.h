@@ -248,4 +248,11 @@ struct lp_context {
     bool profiling = false;
     std::vector<float> stage_ms_acc;
     int32_t profiled_runs = 0;
+    // Further lanes of a multi-stream context (lp_capacity.streams > 1): each
+    // is a context of its own (buffers, stream, graphs) that takes a
+    // contiguous share of a staged batch; this context is lane 0 and its
+    // stream waits for the others after every run.
+    std::vector<lp_context*> lanes;
+    std::vector<int32_t> lane_frames;   // frames staged per lane, lane 0 first
+    std::vector<hipEvent_t> lane_done;  // one per further lane
 };

@@ -128,6 +128,7 @@ int lp_server_create(const lp_server_config* cfg, lp_server** out) {
     s->cfg = *cfg;
     lp_capacity cap = cfg->capacity;
     cap.max_frames = cfg->max_batch_frames;
+    cap.streams = 1;  // slots already run side by side
     for (int k = 0; k < cfg->slots; ++k) {
         lp_context* c = nullptr;
         const int st = lp_context_create(&cap, &c);

@@ -141,6 +141,41 @@ def test_batch_with_uneven_frames(ctx, oracle):
         assert out["num_clusters"] == ref["num_clusters"]
 
 
+@pytest.mark.parametrize("streams,nframes", [(2, 5), (3, 2), (4, 7)])
+def test_multi_stream_batch_matches_single(ctx, oracle, streams, nframes):
+    """A batch split over several streams (contiguous shares, uneven and
+    with idle lanes) gives the single-stream results for every frame, with
+    the noise filter and the boxes on, eager and replayed."""
+    frames = [synth.workload_c3(i) for i in range(nframes)]
+    scgs = [cfgs(oracle, f)[1] for f in frames]
+    ins = [(f.x, f.y, f.z, f.ring) for f in frames]
+    keys = ("seg_labels", "cluster_labels", "hull_offsets", "hull_indices", "noise_labels")
+    ref = N.Context(max_frames=8, max_points=max(f.n for f in frames), max_height=64, max_width=2048)
+    ms = N.Context(max_frames=8, max_points=max(f.n for f in frames), max_height=64, max_width=2048,
+                   streams=streams)
+    try:
+        for c in (ref, ms):
+            c.set_noise_filter(N.noise_config())
+            c.set_oriented_boxes(True)
+            c.stage(ins, scgs)
+        ref.run(use_graph=False)
+        want = ref.fetch()
+        for use_graph in (False, True, True):
+            ms.run(use_graph=use_graph)
+            got = ms.fetch()
+            assert len(got) == nframes
+            for a, b in zip(want, got):
+                assert a["status"] == 0 and b["status"] == 0
+                for k in keys:
+                    assert np.array_equal(a[k], b[k]), k
+                assert all(box_a == box_b for box_a, box_b in zip(
+                    [tuple(np.asarray(x["corners"]).tolist()) for x in a["boxes"]],
+                    [tuple(np.asarray(x["corners"]).tolist()) for x in b["boxes"]]))
+    finally:
+        ref.close()
+        ms.close()
+
+
 def test_graph_replay_matches_eager(ctx, oracle):
     frames = [frame("c1"), frame("c1b")]
     scg = cfgs(oracle, frames[0])[1]
