@@ -287,27 +287,39 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
         return m;
     }
     // Ordered sums of segmenter.cpp:613-626: slots 0..9 are fixed, 10 and 11
-    // may depend on the row's previous states, 12..23 are fixed again.
+    // may depend on the row's previous states, 12..23 are fixed again.  The
+    // ground sum only sees which of slots 10, 11 are GROUND and the obstacle
+    // sum which are OBSTACLE, so four chains of each (q bit 0: slot 10
+    // counts, bit 1: slot 11 counts), each in the reference's order, cover
+    // all nine pair states; the chains are independent and interleave.
     float wg0 = 0.0f, wo0 = 0.0f;
 #pragma unroll
     for (int i = 0; i < 10; ++i) add_slot(lab[i], wv[i], wg0, wo0);
+    float tg[4], to[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float g = wg0, o = wo0;
+        g += (q & 1) ? wv[10] : 0.0f;
+        o += (q & 1) ? wv[10] : 0.0f;
+        g += (q & 2) ? wv[11] : 0.0f;
+        o += (q & 2) ? wv[11] : 0.0f;
+#pragma unroll
+        for (int i = 12; i < 24; ++i) add_slot(lab[i], wv[i], g, o);
+        tg[q] = g;
+        to[q] = o;
+    }
     uint64_t m = 0;
-    const int combos = (sel10 || sel11) ? 9 : 1;
-#pragma unroll 1
-    for (int ps = 0; ps < combos; ++ps) {
+#pragma unroll
+    for (int ps = 0; ps < 9; ++ps) {
         const uint32_t a = ps / 3, b = ps % 3;
-        float wg = wg0, wo = wo0;
-        add_slot(sel10 == 1 ? a : (sel10 == 2 ? b : lab[10]), wv[10], wg, wo);
-        add_slot(sel11 ? b : lab[11], wv[11], wg, wo);
-#pragma unroll
-        for (int i = 12; i < 24; ++i) add_slot(lab[i], wv[i], wg, wo);
-        const uint32_t t = (wo > wg) ? kSO : kSG;
-        if (combos == 1) {
-#pragma unroll
-            for (int q = 0; q < 9; ++q) m |= static_cast<uint64_t>((q % 3) * 3 + t) << (4 * q);
-        } else {
-            m |= static_cast<uint64_t>(b * 3 + t) << (4 * ps);
-        }
+        const uint32_t l10 = sel10 == 1 ? a : (sel10 == 2 ? b : lab[10]);
+        const uint32_t l11 = sel11 ? b : lab[11];
+        const int ig = (l10 == kSG ? 1 : 0) | (l11 == kSG ? 2 : 0);
+        const int io = (l10 == kSO ? 1 : 0) | (l11 == kSO ? 2 : 0);
+        const float g = ig == 0 ? tg[0] : (ig == 1 ? tg[1] : (ig == 2 ? tg[2] : tg[3]));
+        const float o = io == 0 ? to[0] : (io == 1 ? to[1] : (io == 2 ? to[2] : to[3]));
+        const uint32_t t = (o > g) ? kSO : kSG;
+        m |= static_cast<uint64_t>(b * 3 + t) << (4 * ps);
     }
     return m;
 }
