@@ -292,9 +292,19 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
     // sum which are OBSTACLE, so four chains of each (q bit 0: slot 10
     // counts, bit 1: slot 11 counts), each in the reference's order, cover
     // all nine pair states; the chains are independent and interleave.
+    // Per-slot addends (the weight or +0) of the two sums, formed once.
+    float ag[24], ao[24];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        ag[i] = lab[i] == kSG ? wv[i] : 0.0f;
+        ao[i] = lab[i] == kSO ? wv[i] : 0.0f;
+    }
     float wg0 = 0.0f, wo0 = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) add_slot(lab[i], wv[i], wg0, wo0);
+    for (int i = 0; i < 10; ++i) {
+        wg0 += ag[i];
+        wo0 += ao[i];
+    }
     float tg[4], to[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -304,7 +314,10 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
         g += (q & 2) ? wv[11] : 0.0f;
         o += (q & 2) ? wv[11] : 0.0f;
 #pragma unroll
-        for (int i = 12; i < 24; ++i) add_slot(lab[i], wv[i], g, o);
+        for (int i = 12; i < 24; ++i) {
+            g += ag[i];
+            o += ao[i];
+        }
         tg[q] = g;
         to[q] = o;
     }
@@ -314,10 +327,10 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
         const uint32_t a = ps / 3, b = ps % 3;
         const uint32_t l10 = sel10 == 1 ? a : (sel10 == 2 ? b : lab[10]);
         const uint32_t l11 = sel11 ? b : lab[11];
-        const int ig = (l10 == kSG ? 1 : 0) | (l11 == kSG ? 2 : 0);
-        const int io = (l10 == kSO ? 1 : 0) | (l11 == kSO ? 2 : 0);
-        const float g = ig == 0 ? tg[0] : (ig == 1 ? tg[1] : (ig == 2 ? tg[2] : tg[3]));
-        const float o = io == 0 ? to[0] : (io == 1 ? to[1] : (io == 2 ? to[2] : to[3]));
+        const float g01 = l10 == kSG ? tg[1] : tg[0], g23 = l10 == kSG ? tg[3] : tg[2];
+        const float o01 = l10 == kSO ? to[1] : to[0], o23 = l10 == kSO ? to[3] : to[2];
+        const float g = l11 == kSG ? g23 : g01;
+        const float o = l11 == kSO ? o23 : o01;
         const uint32_t t = (o > g) ? kSO : kSG;
         m |= static_cast<uint64_t>(b * 3 + t) << (4 * ps);
     }

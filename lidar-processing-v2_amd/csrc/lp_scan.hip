@@ -38,31 +38,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L
     if (threadIdx.x == 0) d.blk[(size_t)f * L.blk_stride + blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_blocks(DevBuf d, Layout L, ScanCount kind, int slot, int total_slot) {
-    __shared__ uint32_t lds[1024 / kWave + 1];
-    const int f = blockIdx.x;
-    const int n = scan_count(d, L, f, kind, slot);
-    const int nb = (n + kScanTile - 1) / kScanTile;
-    uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
-    uint32_t carry = 0;
-    for (int base = 0; base < nb; base += 1024) {
-        const int i = base + threadIdx.x;
-        const uint32_t v = i < nb ? blk[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_sum<1024>(v, lds, &total);
-        if (i < nb) blk[i] = ex + carry;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && total_slot >= 0) d.fs[f * kFsCount + total_slot] = carry;
-}
-
+// Down pass: each tile adds the totals of the tiles before it (read straight
+// from the reduce pass, at most blk_stride of them) to its own exclusive
+// scan; the last tile writes the frame total.  Two launches per scan.
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, const uint32_t* in, uint32_t* out,
-                                                            size_t stride, ScanCount kind, int slot) {
+                                                            size_t stride, ScanCount kind, int slot, int total_slot) {
     __shared__ uint32_t lds[kScanThreads / kWave + 1];
     const int f = blockIdx.y;
     const int n = scan_count(d, L, f, kind, slot);
-    const int base = blockIdx.x * kScanTile;
-    if (base >= n) return;
+    const int tile = blockIdx.x;
+    const int base = tile * kScanTile;
+    if (base >= n) {
+        if (tile == 0 && threadIdx.x == 0 && total_slot >= 0) d.fs[f * kFsCount + total_slot] = 0;
+        return;
+    }
+    const uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
+    uint32_t pre = 0;
+    for (int i = threadIdx.x; i < tile; i += kScanThreads) pre += blk[i];
+    uint32_t pre_total;
+    block_exclusive_sum<kScanThreads>(pre, lds, &pre_total);
     const uint32_t* a = in + (size_t)f * stride;
     uint32_t* o = out + (size_t)f * stride;
     uint32_t v[kScanItems];
@@ -74,12 +68,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, 
         s += v[k];
     }
     uint32_t total;
-    uint32_t ex = block_exclusive_sum<kScanThreads>(s, lds, &total) + d.blk[(size_t)f * L.blk_stride + blockIdx.x];
+    uint32_t ex = block_exclusive_sum<kScanThreads>(s, lds, &total) + pre_total;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (b + k < n) o[b + k] = ex;
         ex += v[k];
     }
+    if (threadIdx.x == 0 && total_slot >= 0 && base + kScanTile >= n)
+        d.fs[f * kFsCount + total_slot] = pre_total + total;
 }
 
 void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
@@ -89,9 +85,8 @@ void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t
     const int nb = (maxn + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, in, stride, kind,
                        count_slot);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(B), dim3(1024), 0, c->stream, c->d, L, kind, count_slot, total_slot);
     hipLaunchKernelGGL(k_scan_down, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, in, out, stride, kind,
-                       count_slot);
+                       count_slot, total_slot);
 }
 
 }  // namespace lp
