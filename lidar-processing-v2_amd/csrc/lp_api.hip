@@ -201,6 +201,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.hcount, B * HC);
     go(x.vlist, B * N);
     go(x.vpar, B * N);
+    go(x.vnb, B * N * 13);
     go(x.vmin, B * N);
     go(x.vcnt, B * N);
     go(x.cl_root, B * N);

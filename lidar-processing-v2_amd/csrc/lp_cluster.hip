@@ -248,6 +248,24 @@ __device__ __forceinline__ void lds_union(int32_t* par, int32_t a, int32_t b) {
     }
 }
 
+// Forward neighbours of every voxel, looked up in the hash by the whole grid
+// (the lookups are independent; the union-find below then only reads them).
+__global__ void k_voxel_neighbours(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= U) return;
+    const CluConst cc = *d.cc;
+    const Dims dm = dims_of(d, f, cc);
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    const int32_t key = d.hkey[ho + d.vlist[ob + v]];
+    int32_t* nb = d.vnb + ob * 13;
+#pragma unroll
+    for (int q = 0; q < 13; ++q) nb[(size_t)q * L.N + v] = voxel_neighbour(d, ho, mask, dm, key, q);
+}
+
 __global__ __launch_bounds__(1024) void k_voxel_cc(DevBuf d, Layout L) {
     __shared__ int32_t par[kCcLds];
     const int f = blockIdx.x;
@@ -257,19 +275,17 @@ __global__ __launch_bounds__(1024) void k_voxel_cc(DevBuf d, Layout L) {
         if (tid == 0) d.fs[f * kFsCount + kFsCcGlobal] = 1;
         return;
     }
-    const CluConst cc = *d.cc;
-    const Dims dm = dims_of(d, f, cc);
     const size_t ob = (size_t)f * L.N;
-    const size_t ho = (size_t)f * L.hash_cap;
-    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
+    const int32_t* nb = d.vnb + ob * 13;
     for (uint32_t v = tid; v < U; v += 1024) par[v] = static_cast<int32_t>(v);
     __syncthreads();
     for (uint32_t v = tid; v < U; v += 1024) {
-        const int32_t key = d.hkey[ho + d.vlist[ob + v]];
-        for (int q = 0; q < 13; ++q) {
-            const int32_t u = voxel_neighbour(d, ho, mask, dm, key, q);
-            if (u >= 0) lds_union(par, static_cast<int32_t>(v), u);
-        }
+        int32_t u[13];
+#pragma unroll
+        for (int q = 0; q < 13; ++q) u[q] = nb[(size_t)q * L.N + v];
+#pragma unroll
+        for (int q = 0; q < 13; ++q)
+            if (u[q] >= 0) lds_union(par, static_cast<int32_t>(v), u[q]);
     }
     __syncthreads();
     for (uint32_t v = tid; v < U; v += 1024) d.vpar[ob + v] = lds_find(par, static_cast<int32_t>(v));
@@ -297,15 +313,11 @@ __global__ void k_voxel_cc_global(DevBuf d, Layout L) {
     const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= U) return;
-    const CluConst cc = *d.cc;
-    const Dims dm = dims_of(d, f, cc);
     const size_t ob = (size_t)f * L.N;
-    const size_t ho = (size_t)f * L.hash_cap;
-    const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     int32_t* par = d.vpar + ob;
-    const int32_t key = d.hkey[ho + d.vlist[ob + v]];
+    const int32_t* nb = d.vnb + ob * 13;
     for (int q = 0; q < 13; ++q) {
-        const int32_t u = voxel_neighbour(d, ho, mask, dm, key, q);
+        const int32_t u = nb[(size_t)q * L.N + v];
         if (u < 0) continue;
         int32_t a = static_cast<int32_t>(v), b = u;
         for (;;) {
@@ -422,6 +434,7 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     hipLaunchKernelGGL(k_voxel_insert, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_check, dim3((B + 63) / 64), dim3(64), 0, st, d, L);
     stage_mark(c, "components");
+    hipLaunchKernelGGL(k_voxel_neighbours, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_cc, dim3(B), dim3(1024), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_cc_global, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_compress_global, dim3(nb, B), dim3(256), 0, st, d, L);

@@ -143,6 +143,7 @@ struct DevBuf {
     // dense voxels (frame stride N)
     int32_t* vlist;          // dense id -> hash slot (insertion order)
     int32_t* vpar;           // union-find parent / component root (dense ids)
+    int32_t* vnb;            // [13][N] per frame: dense id of each forward neighbour voxel, -1 none
     int32_t* vmin;           // per root: minimum obstacle index of the component
     uint32_t* vcnt;          // per root: points in the component
     // clusters (frame stride N)
