@@ -137,43 +137,33 @@ __global__ void k_voxel_insert(DevBuf d, Layout L) {
     const int32_t key = dm.nr * (dm.na * ei + ai) + ri;
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const size_t ho = (size_t)f * L.hash_cap;
-    // Lanes with the same voxel insert once: the lowest such lane (which holds
-    // the group's smallest point index, lanes follow point order) probes and
-    // records the group's count and minimum index.
-    const int lane = lane_id();
-    uint64_t pending = __ballot(1);
-    int32_t slot = 0;
-    while (pending) {
-        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-        const int32_t lkey = __shfl(key, leader, kWave);
-        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
-        const uint64_t grp = __ballot(mine);
-        uint32_t h = 0;
-        if (lane == leader) {
-            h = hash_slot(key, mask);
-            for (uint32_t probe = 0; probe <= mask; ++probe) {
-                const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
-                if (prev == -1) {
-                    const uint32_t pos = atomicAdd(&d.fs[f * kFsCount + kFsVoxels], 1u);
-                    if (pos < static_cast<uint32_t>(L.N)) {
-                        d.vlist[ob + pos] = static_cast<int32_t>(h);
-                        d.vpar[ob + pos] = static_cast<int32_t>(pos);
-                        d.vmin[ob + pos] = 0x7fffffff;
-                        d.vcnt[ob + pos] = 0;
-                    }
-                    d.hdense[ho + h] = static_cast<int32_t>(pos);
-                    break;
+    // A run of lanes with the same voxel inserts once: its lowest lane (which
+    // holds the run's smallest point index, lanes follow point order) probes
+    // and records the run's count and minimum index; all runs at once.
+    const WaveRun run = wave_runs(key);
+    uint32_t h = 0;
+    if (lane_id() == run.leader) {
+        h = hash_slot(key, mask);
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+            if (prev == -1) {
+                const uint32_t pos = atomicAdd(&d.fs[f * kFsCount + kFsVoxels], 1u);
+                if (pos < static_cast<uint32_t>(L.N)) {
+                    d.vlist[ob + pos] = static_cast<int32_t>(h);
+                    d.vpar[ob + pos] = static_cast<int32_t>(pos);
+                    d.vmin[ob + pos] = 0x7fffffff;
+                    d.vcnt[ob + pos] = 0;
                 }
-                if (prev == key) break;
-                h = (h + 1) & mask;
+                d.hdense[ho + h] = static_cast<int32_t>(pos);
+                break;
             }
-            atomicMin(&d.hmin[ho + h], k);
-            atomicAdd(&d.hcount[ho + h], static_cast<uint32_t>(__popcll(grp)));
+            if (prev == key) break;
+            h = (h + 1) & mask;
         }
-        h = __shfl(h, leader, kWave);
-        if (mine) slot = static_cast<int32_t>(h);
-        pending &= ~grp;
+        atomicMin(&d.hmin[ho + h], k);
+        atomicAdd(&d.hcount[ho + h], run.count);
     }
+    const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
     d.vslot[ob + k] = slot;
 }
 

@@ -216,29 +216,47 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// atomicAdd(ctr, 1) for every active lane, with the lanes of the wavefront
-// that target the same counter combined into one atomic (points of one
-// wavefront mostly share a cell or a cluster, so per-lane atomics on one
-// address would serialise).  Returns what atomicAdd would have: the counter's
-// value before this lane's increment, lanes of one address taking
-// consecutive values in lane order.
-__device__ __forceinline__ uint32_t wave_atomic_inc(uint32_t* ctr) {
+// Runs of equal `key` among the active lanes of the wavefront (consecutive
+// active lanes with one key; points handled by neighbouring lanes mostly
+// share a cell, voxel or cluster).  leader: the run's lowest lane; count: the
+// run's lanes; rank: active lanes of the run below this one.  A key may form
+// several runs when its lanes are not adjacent.
+struct WaveRun {
+    int leader;
+    uint32_t count;
+    uint32_t rank;
+};
+
+template <typename K>
+__device__ __forceinline__ WaveRun wave_runs(K key) {
     const int lane = lane_id();
-    const unsigned long long me = reinterpret_cast<unsigned long long>(ctr);
-    uint64_t pending = __ballot(1);
-    uint32_t result = 0;
-    while (pending) {
-        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-        const unsigned long long target = __shfl(me, leader, kWave);
-        const bool mine = ((pending >> lane) & 1ull) && me == target;
-        const uint64_t grp = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(ctr, static_cast<uint32_t>(__popcll(grp)));
-        base = __shfl(base, leader, kWave);
-        if (mine) result = base + lanes_below(grp);
-        pending &= ~grp;
-    }
-    return result;
+    const uint64_t active = __ballot(1);
+    const uint64_t below = active & ((1ull << lane) - 1ull);
+    const int prev = below ? 63 - __clzll(static_cast<long long>(below)) : lane;
+    const K pk = __shfl(key, prev, kWave);
+    const bool start = !below || pk != key;
+    const uint64_t starts = __ballot(start);
+    const uint64_t upto = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    WaveRun r;
+    r.leader = 63 - __clzll(static_cast<long long>(upto));
+    const uint64_t after = starts & ~(r.leader == 63 ? ~0ull : ((2ull << r.leader) - 1ull));
+    const int next = after ? __ffsll(static_cast<unsigned long long>(after)) - 1 : kWave;
+    const uint64_t from_leader = ~((1ull << r.leader) - 1ull);
+    const uint64_t to_next = next == kWave ? ~0ull : ((1ull << next) - 1ull);
+    r.count = static_cast<uint32_t>(__popcll(active & from_leader & to_next));
+    r.rank = static_cast<uint32_t>(__popcll(active & from_leader & ((1ull << lane) - 1ull)));
+    return r;
+}
+
+// atomicAdd(ctr, 1) for every active lane, with each run of lanes that target
+// the same counter combined into one atomic (all runs at once).  Returns what
+// atomicAdd would have: the counter's value before this lane's increment,
+// the lanes of a run taking consecutive values in lane order.
+__device__ __forceinline__ uint32_t wave_atomic_inc(uint32_t* ctr) {
+    const WaveRun r = wave_runs(reinterpret_cast<unsigned long long>(ctr));
+    uint32_t base = 0;
+    if (lane_id() == r.leader) base = atomicAdd(ctr, r.count);
+    return __shfl(base, r.leader, kWave) + r.rank;
 }
 
 template <typename T>

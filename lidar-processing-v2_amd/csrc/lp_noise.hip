@@ -102,31 +102,22 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
     const int32_t key = noise_key(cx, cy, cz);
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const size_t ho = (size_t)f * L.hash_cap;
-    // Lanes with the same cell (consecutive points mostly share one) insert
-    // once: the lowest such lane probes and adds the group's count.
-    const int lane = lane_id();
-    uint64_t pending = __ballot(1);
-    int32_t slot = 0;
-    while (pending) {
-        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-        const int32_t lkey = __shfl(key, leader, kWave);
-        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
-        const uint64_t grp = __ballot(mine);
-        uint32_t h = 0;
-        if (lane == leader) {
-            h = noise_slot(key, mask);
-            for (uint32_t probe = 0; probe <= mask; ++probe) {
-                const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
-                if (prev == -1 || prev == key) break;
-                h = (h + 1) & mask;
-            }
-            atomicAdd(&d.hcount[ho + h], static_cast<uint32_t>(__popcll(grp)));
-            atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
+    // A run of lanes with the same cell (consecutive points mostly share one)
+    // inserts once: its lowest lane probes and adds the run's count; all
+    // runs at once.
+    const WaveRun run = wave_runs(key);
+    uint32_t h = 0;
+    if (lane_id() == run.leader) {
+        h = noise_slot(key, mask);
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+            if (prev == -1 || prev == key) break;
+            h = (h + 1) & mask;
         }
-        h = __shfl(h, leader, kWave);
-        if (mine) slot = static_cast<int32_t>(h);
-        pending &= ~grp;
+        atomicAdd(&d.hcount[ho + h], run.count);
+        atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
     }
+    const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
     d.vslot[(size_t)f * L.N + i] = slot;
 }
 
