@@ -175,6 +175,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.zmin, B * C);
     go(x.elev, B * C);
     go(x.big_cells, B * C);
+    go(x.cell_list, B * C);
     go(x.cand_count, B * L.cand_stride);
     go(x.cand_start, B * L.cand_stride);
     go(x.cx, B * N);

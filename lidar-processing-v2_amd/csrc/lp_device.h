@@ -259,6 +259,40 @@ __device__ __forceinline__ uint32_t wave_atomic_inc(uint32_t* ctr) {
     return __shfl(base, r.leader, kWave) + r.rank;
 }
 
+// Bitonic sort (ascending) of 64*S u32 keys held in registers, element
+// e = s * 64 + lane in slot s: partners within a lane take min / max of two
+// registers, others exchange through lane shuffles.
+template <int S>
+__device__ __forceinline__ void reg_sort_u32(uint32_t (&key)[S]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= kWave * S; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= kWave) {
+                const int js = j / kWave;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s & js) continue;
+                    const int t = s | js;
+                    const bool asc = ((s * kWave) & k) == 0;
+                    const uint32_t lo = min(key[s], key[t]), hi = max(key[s], key[t]);
+                    key[s] = asc ? lo : hi;
+                    key[t] = asc ? hi : lo;
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const uint32_t other = __shfl_xor(key[s], j, kWave);
+                    const bool asc = ((s * kWave + lane) & k) == 0;
+                    const bool lower = (lane & j) == 0;
+                    key[s] = (lower == asc) ? min(key[s], other) : max(key[s], other);
+                }
+            }
+        }
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_sum(T v) {
     const int l = lane_id();

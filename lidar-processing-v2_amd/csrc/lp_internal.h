@@ -114,6 +114,7 @@ struct DevBuf {
     float* zmin;
     float* elev;
     uint32_t* big_cells;     // per frame list of cells needing a workgroup sort
+    uint32_t* cell_list;     // per frame: the non-empty cells, in cell order
     // RANSAC (per frame)
     uint32_t* cand_count;    // frame stride cand_stride
     uint32_t* cand_start;
@@ -191,6 +192,7 @@ enum FsSlot : int {
     kFsCcGlobal,       // 1: too many voxels for the LDS union-find
     kFsErr,
     kFsBoxes,          // clusters with a box
+    kFsCellsUsed,      // non-empty polar cells
     // kept across the per-run clear: written by the noise filter before it
     kFsKept,           // points kept by the noise filter
     kFsNoise,          // points labelled NOISE

@@ -96,32 +96,49 @@ __global__ void k_project(DevBuf d, Layout L) {
 }
 
 // Exclusive scan of the cell histogram, one 1024-thread workgroup per frame.
+// Cell starts (exclusive scan of the counts), the list of non-empty cells
+// for k_cell_sort, and the empty marker of every empty cell's minimum.
 __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
     __shared__ uint32_t lds[1024 / kWave + 1];
     const int f = blockIdx.x;
     const int cells = d.frames[f].sc.cells;
-    const uint32_t* cnt = d.cell_count + (size_t)f * L.cell_stride;
+    const size_t co = (size_t)f * L.cell_stride;
+    const uint32_t* cnt = d.cell_count + co;
     uint32_t* st = d.cell_start + (size_t)f * (L.cell_stride + 1);
-    uint32_t carry = 0;
+    uint32_t* list = d.cell_list + co;
+    float* zmin = d.zmin + co;
+    uint32_t carry = 0, used = 0;
     for (int base = 0; base < cells; base += 1024 * 8) {
         uint32_t v[8];
-        uint32_t local = 0;
+        uint32_t local = 0, nz = 0;
         const int b = base + threadIdx.x * 8;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             v[k] = (b + k < cells) ? cnt[b + k] : 0u;
             local += v[k];
+            nz += v[k] ? 1u : 0u;
         }
-        uint32_t total;
+        uint32_t total, nz_total;
         uint32_t ex = block_exclusive_sum<1024>(local, lds, &total) + carry;
+        uint32_t lx = block_exclusive_sum<1024>(nz, lds, &nz_total) + used;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            if (b + k < cells) st[b + k] = ex;
+            if (b + k < cells) {
+                st[b + k] = ex;
+                if (v[k])
+                    list[lx++] = static_cast<uint32_t>(b + k);
+                else
+                    zmin[b + k] = u2f(kEmptyZ);
+            }
             ex += v[k];
         }
         carry += total;
+        used += nz_total;
     }
-    if (threadIdx.x == 0) st[cells] = carry;
+    if (threadIdx.x == 0) {
+        st[cells] = carry;
+        d.fs[f * kFsCount + kFsCellsUsed] = used;
+    }
 }
 
 // Scatter point indices into their cells (unordered; sorted per cell next).
@@ -184,10 +201,61 @@ __device__ __forceinline__ float robust_min(const uint32_t* keys, int n, int tid
     return unord_f32(keys[best]);
 }
 
-// One wavefront per cell: sorts the cell's point indices ascending (so the
-// cell lists follow the reference's input order) for the RANSAC cells, and
-// sorts z for the robust minimum.  Cells larger than kSmallCell are queued
-// for k_cell_sort_big.
+// Sorted cell of up to 64*S points held in registers: the point indices
+// ascending for the RANSAC cells, and z for the robust minimum.
+template <int S>
+__device__ __forceinline__ void sort_cell_regs(const DevBuf& d, const FrameDesc& fd, int32_t* items, int n, int radial,
+                                               float* zmin_cell) {
+    const int lane = lane_id();
+    uint32_t key[S];
+    if (radial < kRansacRadialBins) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int e = s * kWave + lane;
+            key[s] = e < n ? static_cast<uint32_t>(items[e]) : 0xffffffffu;
+        }
+        reg_sort_u32<S>(key);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int e = s * kWave + lane;
+            if (e < n) items[e] = static_cast<int32_t>(key[s]);
+        }
+    }
+    if (radial >= 1) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int e = s * kWave + lane;
+            key[s] = e < n ? ord_f32(d.z[fd.pt_off + items[e]]) : 0xffffffffu;
+        }
+        reg_sort_u32<S>(key);
+        // Robust minimum (segmenter.cpp:249-260): the largest i in [1, n/2]
+        // with s[i] - s[i-1] > 0.5f, else 0.
+        int best = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int e = s * kWave + lane;
+            const uint32_t up = __shfl_up(key[s], 1, kWave);
+            const uint32_t wrap = __shfl(key[s > 0 ? s - 1 : 0], kWave - 1, kWave);
+            const uint32_t prev = lane > 0 ? up : wrap;
+            if (e >= 1 && e <= n / 2 && unord_f32(key[s]) - unord_f32(prev) > 0.5F) best = e;
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o, kWave));
+        uint32_t kb = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const uint32_t v = __shfl(key[s], best % kWave, kWave);
+            if (best / kWave == s) kb = v;
+        }
+        if (lane == 0) *zmin_cell = unord_f32(kb);
+    }
+}
+
+// One wavefront per non-empty cell (a grid-stride loop over the cell list):
+// sorts the cell's point indices ascending (so the cell lists follow the
+// reference's input order) for the RANSAC cells, and sorts z for the robust
+// minimum.  Cells up to 128 points sort in registers, up to kSmallCell in
+// LDS; larger ones are queued for k_cell_sort_big.
 __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     __shared__ uint32_t lds[4][kSmallCell];
     const int f = blockIdx.y;
@@ -195,47 +263,55 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     const SegConst& s = fd.sc;
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
-    const int cell = blockIdx.x * 4 + w;
-    if (cell >= s.cells) return;
     const size_t co = (size_t)f * L.cell_stride;
-    const uint32_t n = d.cell_count[co + cell];
+    const int used = static_cast<int>(d.fs[f * kFsCount + kFsCellsUsed]);
+    const int waves = gridDim.x * 4;
     float* zmin = d.zmin + co;
-    if (n == 0) {
-        if (lane == 0) zmin[cell] = u2f(kEmptyZ);
-        return;
-    }
-    if (n > static_cast<uint32_t>(kSmallCell)) {
-        if (lane == 0) {
-            const uint32_t k = atomicAdd(&d.fs[f * kFsCount + kFsNumBig], 1u);
-            if (k < static_cast<uint32_t>(L.cell_stride)) d.big_cells[co + k] = cell;
-        }
-        return;
-    }
-    const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
-    int32_t* items = d.cell_items + (size_t)f * L.N + start;
     uint32_t* a = lds[w];
-    int m = 1;
-    while (m < static_cast<int>(n)) m <<= 1;
-    const int radial = cell % s.rings;
     auto sync = [] { wave_lds_sync(); };
-    if (radial < kRansacRadialBins) {
-        for (int t = lane; t < m; t += kWave) a[t] = t < static_cast<int>(n) ? static_cast<uint32_t>(items[t]) : 0xffffffffu;
-        wave_lds_sync();
-        bitonic_u32<kWave>(a, m, lane, sync);
-        for (int t = lane; t < static_cast<int>(n); t += kWave) items[t] = static_cast<int32_t>(a[t]);
-        wave_lds_sync();
-    }
-    if (radial >= 1) {
-        for (int t = lane; t < m; t += kWave)
-            a[t] = t < static_cast<int>(n) ? ord_f32(d.z[fd.pt_off + items[t]]) : 0xffffffffu;
-        wave_lds_sync();
-        bitonic_u32<kWave>(a, m, lane, sync);
-        const float r = robust_min<kWave>(a, static_cast<int>(n), lane, sync, [](int v) {
+    for (int q = blockIdx.x * 4 + w; q < used; q += waves) {
+        const int cell = static_cast<int>(d.cell_list[co + q]);
+        const uint32_t n = d.cell_count[co + cell];
+        if (n > static_cast<uint32_t>(kSmallCell)) {
+            if (lane == 0) {
+                const uint32_t k = atomicAdd(&d.fs[f * kFsCount + kFsNumBig], 1u);
+                if (k < static_cast<uint32_t>(L.cell_stride)) d.big_cells[co + k] = cell;
+            }
+            continue;
+        }
+        const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+        int32_t* items = d.cell_items + (size_t)f * L.N + start;
+        const int radial = cell % s.rings;
+        if (n <= static_cast<uint32_t>(kWave)) {
+            sort_cell_regs<1>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+            continue;
+        }
+        if (n <= static_cast<uint32_t>(2 * kWave)) {
+            sort_cell_regs<2>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+            continue;
+        }
+        int m = 1;
+        while (m < static_cast<int>(n)) m <<= 1;
+        if (radial < kRansacRadialBins) {
+            for (int t = lane; t < m; t += kWave) a[t] = t < static_cast<int>(n) ? static_cast<uint32_t>(items[t]) : 0xffffffffu;
+            wave_lds_sync();
+            bitonic_u32<kWave>(a, m, lane, sync);
+            for (int t = lane; t < static_cast<int>(n); t += kWave) items[t] = static_cast<int32_t>(a[t]);
+            wave_lds_sync();
+        }
+        if (radial >= 1) {
+            for (int t = lane; t < m; t += kWave)
+                a[t] = t < static_cast<int>(n) ? ord_f32(d.z[fd.pt_off + items[t]]) : 0xffffffffu;
+            wave_lds_sync();
+            bitonic_u32<kWave>(a, m, lane, sync);
+            const float r = robust_min<kWave>(a, static_cast<int>(n), lane, sync, [](int v) {
 #pragma unroll
-            for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-            return v;
-        });
-        if (lane == 0) zmin[cell] = r;
+                for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+                return v;
+            });
+            if (lane == 0) zmin[cell] = r;
+        }
+        wave_lds_sync();  // the next cell reuses this wave's LDS
     }
 }
 
@@ -645,7 +721,8 @@ void launch_segment_front(lp_context* c, int B) {
     stage_mark(c, "cells");
     hipLaunchKernelGGL(k_cell_scan, dim3(B), dim3(1024), 0, st, d, L);
     hipLaunchKernelGGL(k_cell_fill, dim3(nb, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_cell_sort, dim3((L.cell_stride + 3) / 4, B), dim3(256), 0, st, d, L);
+    // About 8192 wavefronts in all, at least 64 and at most 4096 per frame.
+    hipLaunchKernelGGL(k_cell_sort, dim3(std::min(1024, std::max(16, 2048 / B)), B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_cell_sort_big, dim3(64, B), dim3(1024), 0, st, d, L);
     stage_mark(c, "recm");
     hipLaunchKernelGGL(k_recm, dim3((L.cand_stride / kRansacRadialBins + 255) / 256, B), dim3(256), 0, st, d, L);
