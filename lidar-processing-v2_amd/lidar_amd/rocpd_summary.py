@@ -5,7 +5,12 @@
       launches whose grid covers that many frames (grid y / block y)
   python -m lidar_amd.rocpd_summary pmc <run_results.db>
       per-kernel counter values averaged per dispatch
+  python -m lidar_amd.rocpd_summary traffic <fetch.db> <write.db> <frames> <copy_bytes>
+      HBM bytes per frame of every kernel as JSON: FETCH_SIZE and WRITE_SIZE
+      (KiB per dispatch, one run each) scaled so that the device-to-device
+      copy of copy_bytes in the same runs reads and writes exactly its size
 """
+import json
 import re
 import sqlite3
 import sys
@@ -38,6 +43,44 @@ def trace(db, frames=None):
     return "\n".join(lines)
 
 
+def _per_dispatch(db, counter):
+    con = sqlite3.connect(db)
+    per = defaultdict(float)
+    disp = defaultdict(set)
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, value, grid_size from counters_collection")
+    for did, name, cn, v, gs in rows:
+        if cn != counter:
+            continue
+        k = _short(name) if "k_" in name else f"copy {gs}" if "copyBuffer" in name else None
+        if k is None:
+            continue
+        per[k] += v
+        disp[k].add(did)
+    return {k: (per[k] / len(disp[k]), len(disp[k])) for k in per}
+
+
+def traffic(fetch_db, write_db, frames, copy_bytes):
+    fe = _per_dispatch(fetch_db, "FETCH_SIZE")
+    wr = _per_dispatch(write_db, "WRITE_SIZE")
+    # The calibration copy is the copy launch with the largest grid.
+    big = max((k for k in fe if k.startswith("copy ")), key=lambda k: int(k.split()[1]))
+    fe["copy"], wr["copy"] = fe[big], wr[big]
+    fs = copy_bytes / 1024.0 / fe["copy"][0]
+    ws = copy_bytes / 1024.0 / wr["copy"][0]
+    out = {"calibration": {"copy_bytes": copy_bytes, "fetch_kib": round(fe["copy"][0], 3),
+                           "write_kib": round(wr["copy"][0], 3), "fetch_scale": round(fs, 4),
+                           "write_scale": round(ws, 4)}}
+    for k in sorted(set(fe) & set(wr)):
+        if k.startswith("copy"):
+            continue
+        f_kib, nd = fe[k]
+        w_kib = wr[k][0]
+        out[k] = {"fetch_kib_per_dispatch": round(f_kib, 1), "write_kib_per_dispatch": round(w_kib, 1),
+                  "dispatches": nd, "frames_per_dispatch": frames,
+                  "hbm_bytes_per_frame": round((f_kib * 1024 * fs + w_kib * 1024 * ws) / frames)}
+    return out
+
+
 def pmc(db):
     con = sqlite3.connect(db)
     per = defaultdict(lambda: defaultdict(float))
@@ -62,5 +105,7 @@ if __name__ == "__main__":
     mode, db = sys.argv[1], sys.argv[2]
     if mode == "trace":
         print(trace(db, int(sys.argv[3]) if len(sys.argv) > 3 else None))
+    elif mode == "traffic":
+        print(json.dumps(traffic(db, sys.argv[3], int(sys.argv[4]), int(sys.argv[5])), indent=1))
     else:
         print(pmc(db))
