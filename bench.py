@@ -23,7 +23,8 @@ import numpy as np  # noqa: E402
 
 METRIC = "frames/s + points/s at 64x2048 (batched vs plain GPU vs CPU); p50/p99 ms"
 # Profiled stages that consist of exactly one kernel launch.
-SINGLE_KERNEL_STAGES = {"noise_count": "k_noise_count", "project": "k_project", "recm": "k_recm",
+SINGLE_KERNEL_STAGES = {"noise_count": "k_noise_count", "noise_search": "k_noise_search", "project": "k_project",
+                        "recm": "k_recm",
                         "pixels": "k_pixel_label", "jcp_slots": "k_jcp_slots", "jcp_solve": "k_jcp_solve",
                         "labels": "k_seg_labels"}
 HBM_PEAK_GBS = 8000.0
@@ -60,11 +61,12 @@ def algorithmic_bytes(n_raw, n, p, n_obs, voxels, hull_vertices, clusters):
             + 16 * hull_vertices + 16 * hull_vertices + 80 * clusters)
 
 
-def stage_bytes(stage, n, p, q, n_obs, voxels, hull_vertices, n_cand, n_raw=0, clusters=0):
+def stage_bytes(stage, n, p, q, n_obs, voxels, hull_vertices, n_cand, n_raw=0, clusters=0, searched=0):
     """Algorithmic bytes one launch of a stage moves (per frame)."""
     return {
         "noise": 12 * n_raw + 4 * n_raw + 16 * n_raw,  # xyz in, cell slot out, cell-sorted points out
         "noise_count": 25 * n_raw,                   # cell-sorted point + index in, label + keep flag out
+        "noise_search": 29 * searched,               # list entry + point in, label + keep flag out
         "noise_compact": 4 * n_raw + 14 * n,         # keep scan in, kept xyz+ring out
         "project": 14 * n + 8 * n,                  # xyz+ring in, cell+pixel out
         "cells": 4 * n * 3 + 4 * 18000 * 2,          # scatter + sort of point indices, zmin out
@@ -357,13 +359,14 @@ def main():
     hv = float(np.mean([len(o["hull_indices"]) for o in outs]))
     q = mean("jcp_queue")
     ncand = float(np.mean([o["ransac"]["candidates"] for o in outs]))
+    searched = mean("num_noise_searched")
     P = 64 * 2048
     # Dominant kernel: the slowest stage that is a single kernel launch (its
     # HIP-event time is then that kernel's duration, comparable to rocprof).
     single = {s: k for s, k in SINGLE_KERNEL_STAGES.items()}
     stages = [(s, ms) for s, ms in g["stages"] if s in single]
     dom, dom_ms = max(stages, key=lambda t: t[1])
-    dom_bytes = stage_bytes(dom, n, P, q, n_obs, vox, hv, ncand, n_raw, ncl) * B
+    dom_bytes = stage_bytes(dom, n, P, q, n_obs, vox, hv, ncand, n_raw, ncl, searched) * B
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")

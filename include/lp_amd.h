@@ -211,6 +211,7 @@ typedef struct lp_frame_result {
     lp_ransac_result ransac;
     int32_t num_noise;          /* points removed by the noise filter */
     int32_t num_boxes;          /* clusters with an oriented box computed */
+    int32_t num_noise_searched; /* points the noise filter took past their own grid cell */
 } lp_frame_result;
 
 /* Optional stages of the fused path, kept by the context until changed:

@@ -623,6 +623,7 @@ int fetch_lane(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* 
         const int n = c->staged_filtered ? static_cast<int>(s[kFsKept]) : fd.n;
         r.num_points = n;
         r.num_noise = c->staged_filtered ? static_cast<int32_t>(s[kFsNoise]) : 0;
+        r.num_noise_searched = c->staged_filtered ? static_cast<int32_t>(s[kFsNoiseHard]) : 0;
         r.num_boxes = c->boxes_on ? static_cast<int32_t>(s[kFsBoxes]) : 0;
         r.num_obstacles = static_cast<int32_t>(s[kFsObstacles]);
         r.num_voxels = static_cast<int32_t>(s[kFsVoxels]);

@@ -167,6 +167,7 @@ class FrameResult(ctypes.Structure):
         ("ransac", RansacResult),
         ("num_noise", ctypes.c_int32),
         ("num_boxes", ctypes.c_int32),
+        ("num_noise_searched", ctypes.c_int32),
     ]
 
 
@@ -506,6 +507,7 @@ class Context:
             no, K, V = r.num_obstacles, r.num_clusters, r.num_hull_vertices
             out.append(dict(status=r.status, seg_labels=b["seg"][:r.num_points], obstacle_index=b["obs"][:no],
                             num_points=r.num_points, num_noise=r.num_noise, noise_labels=b["noise"][:len(frames[i][0])],
+                            num_noise_searched=r.num_noise_searched,
                             boxes=[box_dict(b["boxes"][k]) for k in range(K)] if self._boxes_on else [],
                             cluster_labels=b["clab"][:no], num_clusters=K, hull_offsets=b["hoff"][:K + 1],
                             hull_indices=b["hidx"][:V], num_voxels=r.num_voxels, jcp_queue=r.jcp_queue,
