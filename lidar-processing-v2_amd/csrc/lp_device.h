@@ -216,6 +216,24 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear workgroup id.  For a
+// grid of gridDim.x tiles x gridDim.y frames with gridDim.y a multiple of 8,
+// this maps the workgroup to a (tile, frame) such that every tile of frame f
+// runs on XCD f % 8, so a frame's arrays stay in one XCD's L2 (other grids
+// keep the plain mapping).  The mapping is a bijection either way.
+__device__ __forceinline__ void xcd_tile_frame(int& tile, int& frame) {
+    const int T = static_cast<int>(gridDim.x), B = static_cast<int>(gridDim.y);
+    if (B % 8 != 0) {
+        tile = static_cast<int>(blockIdx.x);
+        frame = static_cast<int>(blockIdx.y);
+        return;
+    }
+    const int p = static_cast<int>(blockIdx.x) + static_cast<int>(blockIdx.y) * T;
+    const int q = p >> 3;
+    frame = (p & 7) + 8 * (q / T);
+    tile = q % T;
+}
+
 // Runs of equal `key` among the active lanes of the wavefront (consecutive
 // active lanes with one key; points handled by neighbouring lanes mostly
 // share a cell, voxel or cluster).  leader: the run's lowest lane; count: the

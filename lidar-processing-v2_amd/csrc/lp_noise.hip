@@ -92,9 +92,10 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
 }
 
 __global__ void k_noise_insert(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tile * blockDim.x + threadIdx.x;
     if (i >= fd.n_raw) return;
     const int gi = fd.pt_off + i;
     int cx, cy, cz;
@@ -123,9 +124,10 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
 
 // hdense holds the exclusive scan of hcount (cell starts) on entry.
 __global__ void k_noise_scatter(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tile * blockDim.x + threadIdx.x;
     if (i >= fd.n_raw) return;
     const int gi = fd.pt_off + i;
     const size_t ho = (size_t)f * L.hash_cap;
@@ -162,10 +164,11 @@ constexpr int kOwnCellBudget = 64;  // distance tests of the own-cell pass
 // does not change the outcome), NOISE when its whole cell has been checked
 // and no other cell comes within its radius, and listed otherwise.
 __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
     const int n = fd.n_raw;
-    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pos = tile * blockDim.x + threadIdx.x;
     if (pos >= n) return;
     const NoiseConst nc = *d.nc;
     const size_t po = (size_t)f * L.N;
@@ -245,7 +248,8 @@ __device__ __forceinline__ uint32_t wave_count_range(const DevBuf& d, size_t po,
 }
 
 __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
     const int n = fd.n_raw;
     const int nhard = static_cast<int>(d.fs[f * kFsCount + kFsNoiseHard]);
@@ -258,7 +262,7 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
     const uint32_t k = nc.k;
     const uint32_t* occ = d.nocc + (size_t)f * kNoiseOccWords;
     uint32_t noise_here = 0;
-    for (int q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < nhard; q += waves) {
+    for (int q = tile * (blockDim.x / kWave) + threadIdx.x / kWave; q < nhard; q += waves) {
         const int pos = d.vslot[po + q];
         const uint32_t carry = static_cast<uint32_t>(d.obs_src[po + q]);
         const float4 t = d.dsort[po + pos];
@@ -338,12 +342,13 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
 // Stable compaction of the kept points into the segmenter's inputs; the
 // frame's point count becomes the kept count.
 __global__ void k_noise_compact(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     FrameDesc& fd = d.frames[f];
     const int n = fd.n_raw;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tile * blockDim.x + threadIdx.x;
     const size_t po = (size_t)f * L.N;
-    if (blockIdx.x == 0 && threadIdx.x == 0) fd.n = static_cast<int32_t>(d.fs[f * kFsCount + kFsKept]);
+    if (tile == 0 && threadIdx.x == 0) fd.n = static_cast<int32_t>(d.fs[f * kFsCount + kFsKept]);
     if (i >= n || !d.obs_flag[po + i]) return;
     const int gi = fd.pt_off + i;
     const int go = fd.pt_off + static_cast<int>(reinterpret_cast<const uint32_t*>(d.obs_src)[po + i]);
