@@ -124,9 +124,10 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t key, uint32_t mask) {
 // Voxel keys (clusterer.cpp:102-120, flatVoxelIndex clusterer.hpp:135-141)
 // inserted into the per-frame hash set.
 __global__ void k_voxel_insert(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = tile * blockDim.x + threadIdx.x;
     if (k >= nobs) return;
     const CluConst cc = *d.cc;
     const Dims dm = dims_of(d, f, cc);
@@ -241,9 +242,10 @@ __device__ __forceinline__ void lds_union(int32_t* par, int32_t a, int32_t b) {
 // Forward neighbours of every voxel, looked up in the hash by the whole grid
 // (the lookups are independent; the union-find below then only reads them).
 __global__ void k_voxel_neighbours(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = tile * blockDim.x + threadIdx.x;
     if (v >= U) return;
     const CluConst cc = *d.cc;
     const Dims dm = dims_of(d, f, cc);

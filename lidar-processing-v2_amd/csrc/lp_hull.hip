@@ -461,7 +461,8 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
     __shared__ uint16_t upl[kWavesPerBlock][kHullChunk];
     __shared__ int nlo[kWavesPerBlock];
     __shared__ int nup[kWavesPerBlock];
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const uint32_t total = d.fs[f * kFsCount + kFsHullChunks];
     const int w = threadIdx.x / kWave;
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
     uint16_t* S = stk[w];
     const size_t in_b = (size_t)(r & 1) * BN;
     const size_t out_b = (size_t)((r + 1) & 1) * BN;
-    for (uint32_t t = blockIdx.x * kWavesPerBlock + w; t < total; t += gridDim.x * kWavesPerBlock) {
+    for (uint32_t t = tile * kWavesPerBlock + w; t < total; t += gridDim.x * kWavesPerBlock) {
         const int c = upper_bound_u32(choff, K, t) - 1;
         const uint32_t ci = t - choff[c];
         const uint32_t cnt = r == 0 ? d.cl_size[ob + c] : d.hc_count[in_b + ob + c];
@@ -819,13 +820,14 @@ __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L
     __shared__ double vx[kLocalMax];
     __shared__ double vy[kLocalMax];
     __shared__ uint32_t bins[kLocalMax + 1];
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const int tid = threadIdx.x;
     const int w = tid / kWave;
     const int lane = lane_id();
     const size_t ob = (size_t)f * L.N;
-    for (int c = blockIdx.x; c < K; c += gridDim.x) {
+    for (int c = tile; c < K; c += gridDim.x) {
         const int V = static_cast<int>(d.hull_count[ob + c]);
         const uint32_t hoff = d.hull_off[ob + c];
         const uint32_t base = d.cl_off[ob + c];

@@ -131,11 +131,12 @@ __device__ __forceinline__ SlotVal eval_slot(const DevBuf& d, size_t po, size_t 
 // Per queue pixel: the 24 slot weights w_i = u_i / sum and label sources,
 // including stale slots resolved through their writer.
 __global__ void k_jcp_slots(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
     const SegConst& s = fd.sc;
     const int nq = static_cast<int>(d.fs[f * kFsCount + kFsQueue]);
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = tile * blockDim.x + threadIdx.x;
     if (j >= nq) return;
     const int H = s.H, W = s.W;
     const size_t po = fd.pix_off;

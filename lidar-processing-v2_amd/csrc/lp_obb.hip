@@ -161,7 +161,8 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
     __shared__ int2 pairs[kBoxPairs];
     __shared__ int np_s;
     __shared__ double cen_s[2];
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const int tid = threadIdx.x;
     const int w = tid / kWave;
     const int lane = lane_id();
@@ -170,10 +171,10 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
     const uint32_t total = fs[kFsHullTotal];
     const size_t ob = (size_t)f * L.N;
     if (K > L.box_stride) {
-        if (blockIdx.x == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
+        if (tile == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
         return;
     }
-    for (int k = blockIdx.x; k < K; k += gridDim.x) {
+    for (int k = tile; k < K; k += gridDim.x) {
         const uint32_t v0 = off[ob + k];
         const uint32_t v1 = k + 1 < K ? off[ob + k + 1] : total;
         const int n = static_cast<int>(v1 - v0);

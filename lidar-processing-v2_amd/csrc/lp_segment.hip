@@ -258,7 +258,8 @@ __device__ __forceinline__ void sort_cell_regs(const DevBuf& d, const FrameDesc&
 // LDS; larger ones are queued for k_cell_sort_big.
 __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     __shared__ uint32_t lds[4][kSmallCell];
-    const int f = blockIdx.y;
+    int tile, f;
+    xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
     const SegConst& s = fd.sc;
     const int w = threadIdx.x / kWave;
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     float* zmin = d.zmin + co;
     uint32_t* a = lds[w];
     auto sync = [] { wave_lds_sync(); };
-    for (int q = blockIdx.x * 4 + w; q < used; q += waves) {
+    for (int q = tile * 4 + w; q < used; q += waves) {
         const int cell = static_cast<int>(d.cell_list[co + q]);
         const uint32_t n = d.cell_count[co + cell];
         if (n > static_cast<uint32_t>(kSmallCell)) {
