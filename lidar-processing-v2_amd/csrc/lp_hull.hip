@@ -218,12 +218,18 @@ __device__ void split_by_chord(const double* X, const double* Y, const uint16_t*
 // or beyond the segment between them, so it is no vertex of that half hull;
 // all such points are dropped at once, and the pass repeats on the survivors.  The half hull of
 // the survivors is the half hull of the list.  One wavefront; T is scratch of
-// n entries.  Returns the new length.
+// n entries.  Returns the new length.  *done becomes true when a pass drops
+// nothing and keeps no point for a duplicate neighbour: every interior point
+// then makes a strict turn, and an x-monotone chain of strict turns of one
+// sign is its own half hull, so the chain walk can be skipped.  *inexact
+// (per lane) counts tests whose double evaluation has another sign.
 __device__ int prune_chain(const double* X, const double* Y, uint16_t* L, uint16_t* T, int n, int turn,
-                           int passes) {
+                           int passes, bool* done, uint32_t* inexact) {
     const int lane = lane_id();
+    *done = n <= 2;
     for (int pass = 0; pass < passes && n > 2; ++pass) {
         int m = 0;
+        bool dup_kept = false;
         for (int i0 = 0; i0 < n; i0 += kWave) {
             const int i = i0 + lane;
             bool keep = false;
@@ -236,8 +242,11 @@ __device__ int prune_chain(const double* X, const double* Y, uint16_t* L, uint16
                     // A neighbour with the same coordinates is no witness:
                     // copies of one vertex would remove each other.
                     const bool dup = (X[a] == X[b] && Y[a] == Y[b]) || (X[c] == X[b] && Y[c] == Y[b]);
-                    int dummy;
-                    keep = dup || orient_exact(X[a], Y[a], X[b], Y[b], X[c], Y[c], &dummy) * turn > 0;
+                    int ns;
+                    const int o = orient_exact(X[a], Y[a], X[b], Y[b], X[c], Y[c], &ns);
+                    *inexact += (o != ns) ? 1u : 0u;
+                    keep = dup || o * turn > 0;
+                    dup_kept = dup_kept || dup;
                 }
             }
             const uint64_t mk = __ballot(keep);
@@ -245,7 +254,10 @@ __device__ int prune_chain(const double* X, const double* Y, uint16_t* L, uint16
             m += static_cast<int>(__popcll(mk));
         }
         wave_sync();
-        if (m == n) break;
+        if (m == n) {
+            *done = __ballot(dup_kept) == 0;
+            break;
+        }
         for (int i = lane; i < m; i += kWave) L[i] = T[i];
         wave_sync();
         n = m;
@@ -254,6 +266,7 @@ __device__ int prune_chain(const double* X, const double* Y, uint16_t* L, uint16
 }
 
 constexpr int kPrunePasses = 6;
+constexpr int kBigPrunePasses = 32;
 constexpr int kWavesPerBlock = 4;
 constexpr int kHullRounds = 3;
 constexpr int kHullBig = 4096;  // candidates one workgroup finishes after the rounds
@@ -632,10 +645,20 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             int cl, cu;
             split_by_chord(X, Y, P, ns, lol[w], upl[w], &cl, &cu);
             wave_sync();
-            cl = prune_chain(X, Y, lol[w], S, cl, 1, kPrunePasses);
-            cu = prune_chain(X, Y, upl[w], S + kHullChunk, cu, -1, kPrunePasses);
-            if (lane < 2) {
-                uint32_t inexact = 0;
+            bool dl, du;
+            uint32_t inexact = 0;
+            cl = prune_chain(X, Y, lol[w], S, cl, 1, kPrunePasses, &dl, &inexact);
+            cu = prune_chain(X, Y, upl[w], S + kHullChunk, cu, -1, kPrunePasses, &du, &inexact);
+            if (dl && du) {
+                // Both halves are already chains: lower as listed, upper
+                // from the right end back.
+                for (int i = lane; i < cl; i += kWave) S[i] = lol[w][i];
+                for (int i = lane; i < cu; i += kWave) S[kHullChunk + i] = upl[w][cu - 1 - i];
+                if (lane == 0) {
+                    nlo[w] = cl;
+                    nup[w] = cu;
+                }
+            } else if (lane < 2) {
                 // One call with per-lane arguments, so the two chains run in
                 // lock-step instead of one after the other.
                 const int len = half_chain(X, Y, lane == 0 ? lol[w] : upl[w], lane == 0 ? cl : cu, lane == 1,
@@ -644,8 +667,9 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
                     nlo[w] = len;
                 else
                     nup[w] = len;
-                if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
             }
+            inexact = wave_sum(inexact);
+            if (lane == 0 && inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
         }
         wave_sync();
         const int kl = nlo[w];
@@ -682,6 +706,7 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
     __shared__ uint16_t LO[kHullBig];
     __shared__ uint16_t UP[kHullBig];
     __shared__ int nlo, nup, nlist[2];
+    __shared__ bool nlist_done;
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const int tid = threadIdx.x;
@@ -732,18 +757,56 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
             }
         } else {
             if (tid < kWave) {
-                int cl, cu;
-                split_by_chord(X, Y, P, n, LO, UP, &cl, &cu);
+                // Exact duplicates (copies from different chunks) are
+                // adjacent now: keep the first of each, or two copies of a
+                // single repeated point (as in the chunk rounds).
+                const int lane = tid;
+                int nd = 0;
+                for (int i0 = 0; i0 < n; i0 += kWave) {
+                    const int i = i0 + lane;
+                    bool keep = false;
+                    int p = 0;
+                    if (i < n) {
+                        p = P[i];
+                        const int q = P[i > 0 ? i - 1 : 0];
+                        keep = i == 0 || X[p] != X[q] || Y[p] != Y[q];
+                    }
+                    const uint64_t mk = __ballot(keep);
+                    if (keep) LO[nd + lanes_below(mk)] = static_cast<uint16_t>(p);
+                    nd += static_cast<int>(__popcll(mk));
+                }
+                if (nd == 1) {
+                    if (lane == 0) LO[1] = P[1];
+                    nd = 2;
+                }
                 wave_sync();
-                cl = prune_chain(X, Y, LO, S, cl, 1, kPrunePasses);
-                cu = prune_chain(X, Y, UP, S + kHullBig, cu, -1, kPrunePasses);
+                for (int i = lane; i < nd; i += kWave) P[i] = LO[i];
+                wave_sync();
+                int cl, cu;
+                split_by_chord(X, Y, P, nd, LO, UP, &cl, &cu);
+                wave_sync();
+                bool dl, du;
+                uint32_t inexact = 0;
+                cl = prune_chain(X, Y, LO, S, cl, 1, kBigPrunePasses, &dl, &inexact);
+                cu = prune_chain(X, Y, UP, S + kHullBig, cu, -1, kBigPrunePasses, &du, &inexact);
+                if (dl && du) {
+                    for (int i = tid; i < cl; i += kWave) S[i] = LO[i];
+                    for (int i = tid; i < cu; i += kWave) S[kHullBig + i] = UP[cu - 1 - i];
+                }
+                inexact = wave_sum(inexact);
                 if (tid == 0) {
                     nlist[0] = cl;
                     nlist[1] = cu;
+                    nlist_done = dl && du;
+                    if (dl && du) {
+                        nlo = cl;
+                        nup = cu;
+                    }
+                    if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
                 }
             }
             __syncthreads();
-            if (tid < 2) {
+            if (!nlist_done && tid < 2) {
                 uint32_t inexact = 0;
                 const int len = half_chain(X, Y, tid == 0 ? LO : UP, nlist[tid], tid == 1,
                                            tid == 1 ? S + kHullBig : S, &inexact);
