@@ -131,6 +131,27 @@ typedef struct lp_cloud {
     int32_t on_device;      /* 1: the pointers are device pointers */
 } lp_cloud;
 
+/* A sensor_msgs::msg::PointCloud2 field as Processor::run reads it: its name
+ * and byte offset inside a point. */
+typedef struct lp_point_field {
+    const char* name;
+    uint32_t offset;
+} lp_point_field;
+
+/* Processor::run's field dispatch (processor.cpp:494-549) and convert<>
+ * (processor.cpp:42-179) without the AoS copy.  The field names must be
+ * exactly {x,y,z}, {x,y,z,intensity}, {x,y,z,ring} or {x,y,z,intensity,ring}
+ * (else LP_ERR_INVALID_ARGUMENT, the reference's "Unsupported point format"
+ * error); x, y, z are the floats at fields[0..2].offset and ring the uint16 at
+ * the ring field's offset of every point.  Rows hold `width` points of
+ * `point_step` bytes, `row_step` bytes apart.  Fills *cloud with views into
+ * `data` (no copy) when the rows are packed (height <= 1 or row_step ==
+ * width * point_step); other layouts return LP_ERR_INVALID_ARGUMENT (pack
+ * the rows first).  No GPU work; intensity is not read by the path. */
+int lp_cloud_from_pointcloud2(const uint8_t* data, uint32_t width, uint32_t height, uint32_t point_step,
+                              uint32_t row_step, const lp_point_field* fields, int32_t num_fields,
+                              lp_cloud* cloud);
+
 /* The RANSAC plane the reference keeps in locals (segmenter.cpp:363-374). */
 typedef struct lp_ransac_result {
     uint32_t candidates;

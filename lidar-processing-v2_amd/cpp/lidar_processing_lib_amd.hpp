@@ -15,6 +15,7 @@
 // std::runtime_error.
 #pragma once
 
+#include <algorithm>
 #include <cstddef>
 #include <array>
 #include <cstdint>
@@ -144,6 +145,46 @@ lp_cloud make_cloud(const CloudT& cloud, bool use_ring) {
 
 }  // namespace detail
 
+// Zero-copy view of a sensor_msgs::msg::PointCloud2 (or any type with the
+// same members: fields[i].name / .offset, data, width, height, point_step,
+// row_step) as segmenter input, with Processor::run's field dispatch
+// (processor.cpp:494-549): x, y, z [, intensity] [, ring] in that order or
+// std::runtime_error("Unsupported point format ...").  Rows padded beyond
+// width * point_step are packed into the view's own buffer first.
+class CloudView {
+  public:
+    template <typename MsgT>
+    explicit CloudView(const MsgT& msg) {
+        std::vector<lp_point_field> f;
+        names_.reserve(msg.fields.size());
+        for (const auto& fld : msg.fields) names_.push_back(std::string(fld.name));
+        for (std::size_t k = 0; k < msg.fields.size(); ++k)
+            f.push_back({names_[k].c_str(), static_cast<std::uint32_t>(msg.fields[k].offset)});
+        const std::uint8_t* data = msg.data.data();
+        std::uint32_t row_step = msg.row_step;
+        const std::size_t packed_row = static_cast<std::size_t>(msg.width) * msg.point_step;
+        if (msg.height > 1 && msg.row_step != packed_row) {
+            packed_.resize(packed_row * msg.height);
+            for (std::uint32_t r = 0; r < msg.height; ++r)
+                std::copy(data + static_cast<std::size_t>(r) * msg.row_step,
+                          data + static_cast<std::size_t>(r) * msg.row_step + packed_row,
+                          packed_.begin() + static_cast<std::ptrdiff_t>(r * packed_row));
+            data = packed_.data();
+            row_step = static_cast<std::uint32_t>(packed_row);
+        }
+        detail::check(lp_cloud_from_pointcloud2(data, msg.width, msg.height, msg.point_step, row_step, f.data(),
+                                                static_cast<std::int32_t>(f.size()), &cloud_));
+    }
+
+    const lp_cloud& cloud() const noexcept { return cloud_; }
+    std::size_t size() const noexcept { return static_cast<std::size_t>(cloud_.num_points); }
+
+  private:
+    lp_cloud cloud_{};
+    std::vector<std::string> names_;
+    std::vector<std::uint8_t> packed_;
+};
+
 class Segmenter {
   public:
     Segmenter() : ctx_(new detail::Context()) {}
@@ -157,6 +198,17 @@ class Segmenter {
         lp_segmenter_config c = to_c();
         lp_cloud in = detail::make_cloud(cloud, !config_.assume_unorganized_cloud);
         labels.assign(cloud.points.size(), Label::UNKNOWN);
+        detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
+    }
+
+    // A PointCloud2 message as it arrives (Processor::run without the
+    // conversion to a pcl::PointCloud).  A ring field is used unless the
+    // configuration assumes an unorganized cloud.
+    void segment(const CloudView& view, std::vector<Label>& labels) {
+        lp_segmenter_config c = to_c();
+        lp_cloud in = view.cloud();
+        if (config_.assume_unorganized_cloud) in.ring = nullptr;
+        labels.assign(view.size(), Label::UNKNOWN);
         detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
     }
 

@@ -2,7 +2,7 @@
 // (src/processor/src/processor.cpp:556-670) written against the drop-in
 // lidar_processing_lib classes of lidar_processing_lib_amd.hpp.
 //
-// Usage: processor_example <in.bin> <out.bin> [full]
+// Usage: processor_example <in.bin> <out.bin> [full | msg]
 //   in.bin : int32 n, then n records {float x, y, z, pad, intensity; uint16 ring; 10 bytes padding}
 //            (the 32-byte pcl::PointXYZIR layout)
 //   out.bin: uint32 seg_labels[n], int32 n_obs, int32 cluster_labels[n_obs],
@@ -12,6 +12,9 @@
 //            every hull polygon gets a rotating-calipers box; out.bin then
 //            starts with int32 n_kept, uint8 noise[n] and ends with k boxes of
 //            {double corners[8]; float area; float angle; int32 valid}.
+//   msg    : the records are handed to the segmenter as a PointCloud2 message
+//            (fields x, y, z, intensity, ring; CloudView, no conversion to a
+//            pcl::PointCloud); the outputs are those of the default mode.
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -42,11 +45,12 @@ struct Cloud {  // the part of pcl::PointCloud<P> the library uses
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3 && !(argc == 4 && std::string(argv[3]) == "full")) {
-        std::fprintf(stderr, "usage: %s in.bin out.bin [full]\n", argv[0]);
+    if (argc != 3 && !(argc == 4 && (std::string(argv[3]) == "full" || std::string(argv[3]) == "msg"))) {
+        std::fprintf(stderr, "usage: %s in.bin out.bin [full | msg]\n", argv[0]);
         return 2;
     }
-    const bool full = argc == 4;
+    const bool full = argc == 4 && std::string(argv[3]) == "full";
+    const bool as_msg = argc == 4 && std::string(argv[3]) == "msg";
     std::FILE* fi = std::fopen(argv[1], "rb");
     if (!fi) return 2;
     std::int32_t n_in = 0;
@@ -79,7 +83,28 @@ int main(int argc, char** argv) {
     clusterer.config(ccfg);
 
     std::vector<lidar_processing_lib::Label> labels;
-    segmenter.segment(cloud, labels);  // processor.cpp:556
+    if (as_msg) {
+        // The message a PointCloud2 subscriber receives (processor.cpp:494).
+        struct Field {
+            std::string name;
+            std::uint32_t offset;
+        };
+        struct Message {
+            std::vector<Field> fields;
+            std::vector<std::uint8_t> data;
+            std::uint32_t width, height, point_step, row_step;
+        } msg;
+        msg.fields = {{"x", 0}, {"y", 4}, {"z", 8}, {"intensity", 16}, {"ring", 20}};
+        msg.width = static_cast<std::uint32_t>(n);
+        msg.height = 1;
+        msg.point_step = sizeof(PointXYZIR);
+        msg.row_step = msg.width * msg.point_step;
+        const auto* raw = reinterpret_cast<const std::uint8_t*>(cloud.points.data());
+        msg.data.assign(raw, raw + static_cast<std::size_t>(n) * sizeof(PointXYZIR));
+        segmenter.segment(lidar_processing_lib::CloudView(msg), labels);
+    } else {
+        segmenter.segment(cloud, labels);  // processor.cpp:556
+    }
 
     Cloud<PointXYZ> obstacles;  // processor.cpp:562-579
     for (std::int32_t i = 0; i < n; ++i)

@@ -999,4 +999,59 @@ int lp_profile_run(lp_context* c, int32_t runs, float* stage_ms, int32_t max_sta
     return LP_OK;
 }
 
+// PointCloud2 field dispatch of Processor::run (src/processor/src/
+// processor.cpp:494-549) and the per-layout conversions convert<PointT>
+// (processor.cpp:42-179): the reference copies every point into a
+// pcl::PointCloud; here the message buffer itself becomes the lp_cloud
+// (byte-strided views), so the points go to the GPU straight from it.
+static bool field_names_are(const lp_point_field* f, int32_t n, std::initializer_list<const char*> want) {
+    if (n != static_cast<int32_t>(want.size())) return false;
+    int32_t k = 0;
+    for (const char* w : want) {
+        if (!f[k].name || std::strcmp(f[k].name, w) != 0) return false;
+        ++k;
+    }
+    return true;
+}
+
+int lp_cloud_from_pointcloud2(const uint8_t* data, uint32_t width, uint32_t height, uint32_t point_step,
+                              uint32_t row_step, const lp_point_field* fields, int32_t num_fields, lp_cloud* cloud) {
+    if (!fields || !cloud || num_fields < 0) return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
+    int ring_field = -1;
+    if (field_names_are(fields, num_fields, {"x", "y", "z"}) ||
+        field_names_are(fields, num_fields, {"x", "y", "z", "intensity"})) {
+        ring_field = -1;
+    } else if (field_names_are(fields, num_fields, {"x", "y", "z", "ring"})) {
+        ring_field = 3;
+    } else if (field_names_are(fields, num_fields, {"x", "y", "z", "intensity", "ring"})) {
+        ring_field = 4;
+    } else {
+        return fail(LP_ERR_INVALID_ARGUMENT,
+                    "Unsupported point format. Acceptable formats: pcl::PointXYZ, pcl::PointXYZI, "
+                    "pcl::PointXYZR, pcl::PointXYZIR");
+    }
+    const uint64_t n = static_cast<uint64_t>(width) * height;
+    if (n > 0x7fffffffull) return fail(LP_ERR_CAPACITY, "more than 2^31 - 1 points");
+    if (height > 1 && static_cast<uint64_t>(row_step) != static_cast<uint64_t>(width) * point_step)
+        return fail(LP_ERR_INVALID_ARGUMENT, "rows are not packed (row_step != width * point_step)");
+    if (point_step > 0x7fffffffu) return fail(LP_ERR_INVALID_ARGUMENT, "point_step too large");
+    for (int k = 0; k < 3; ++k)
+        if (static_cast<uint64_t>(fields[k].offset) + 4 > point_step)
+            return fail(LP_ERR_INVALID_ARGUMENT, "a coordinate field lies outside the point");
+    if (ring_field >= 0 && static_cast<uint64_t>(fields[ring_field].offset) + 2 > point_step)
+        return fail(LP_ERR_INVALID_ARGUMENT, "the ring field lies outside the point");
+    if (n > 0 && !data) return fail(LP_ERR_INVALID_ARGUMENT, "null data");
+    lp_cloud c{};
+    c.num_points = static_cast<int32_t>(n);
+    c.xyz_stride = static_cast<int32_t>(point_step);
+    c.ring_stride = static_cast<int32_t>(point_step);
+    c.on_device = 0;
+    c.x = data ? data + fields[0].offset : nullptr;
+    c.y = data ? data + fields[1].offset : nullptr;
+    c.z = data ? data + fields[2].offset : nullptr;
+    c.ring = (ring_field >= 0 && data) ? data + fields[ring_field].offset : nullptr;
+    *cloud = c;
+    return LP_OK;
+}
+
 }  // extern "C"

@@ -29,7 +29,7 @@ EXPORTED = [
     "lp_math_eval", "lp_math_check_range", "lp_add_ring_info", "lp_query", "lp_noise_config_default",
     "lp_remove_noise", "lp_oriented_boxes", "lp_set_noise_filter", "lp_set_oriented_boxes",
     "lp_server_config_default", "lp_server_create", "lp_server_destroy", "lp_server_submit", "lp_server_poll",
-    "lp_server_stats_get",
+    "lp_server_stats_get", "lp_cloud_from_pointcloud2",
 ]
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
@@ -303,6 +303,29 @@ def noise_config(**kw):
 def box_dict(b):
     return dict(corners=np.array(b.corners[:], np.float64).reshape(4, 2), area=np.float32(b.area),
                 angle_rad=np.float32(b.angle_rad), is_valid=bool(b.is_valid))
+
+
+class PointField(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("offset", ctypes.c_uint32)]
+
+
+def pointcloud2_cloud(data, fields, width, height, point_step, row_step):
+    """Processor::run's PointCloud2 field dispatch as a zero-copy view:
+    data is the message's byte buffer (numpy uint8), fields [(name, offset)].
+    Returns (status, Cloud); the Cloud points into `data`."""
+    arr = (PointField * max(len(fields), 1))()
+    names = [n.encode() for n, _ in fields]
+    for k, (n, off) in enumerate(fields):
+        arr[k].name = names[k]
+        arr[k].offset = off
+    c = Cloud()
+    L = lib()
+    L.lp_cloud_from_pointcloud2.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+    L.lp_cloud_from_pointcloud2.restype = ctypes.c_int32
+    st = L.lp_cloud_from_pointcloud2(data.ctypes.data if data is not None and len(data) else None, width, height,
+                                     point_step, row_step, ctypes.addressof(arr), len(fields), ctypes.byref(c))
+    return st, c
 
 
 def soa_cloud(x, y, z, ring=None):

@@ -327,3 +327,8 @@ def test_cpp_dropin_processor(tmp_path, oracle):
     hg = hull_coords(ox, oy, clab, k, hoff, hidx)
     hr = hull_coords(ox, oy, ref["cluster_labels"], k, ref["hull_offsets"], ref["hull_indices"])
     assert all(np.array_equal(a, b) for a, b in zip(hg, hr))
+    # The same records handed over as a PointCloud2 message (CloudView, no
+    # conversion to a point cloud): identical output file.
+    dst2 = tmp_path / "out_msg.bin"
+    subprocess.run([exe, str(src), str(dst2), "msg"], check=True, timeout=120)
+    assert open(dst, "rb").read() == open(dst2, "rb").read()
