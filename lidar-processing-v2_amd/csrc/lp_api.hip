@@ -154,6 +154,9 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.dsort, B * N);
     go(x.didx, B * N);
     go(x.nocc, B * kNoiseOccWords);
+    go(x.ncell_slot, B * N);
+    go(x.ncell_start, B * N);
+    go(x.hstart, B * HC);
     go(x.pt_cell, B * N);
     go(x.pt_pix, B * N);
     go(x.labels, B * N);

@@ -95,6 +95,9 @@ struct DevBuf {
     float4* dsort;
     int32_t* didx;
     uint32_t* nocc;          // per frame: kNoiseOccWords words, one bit per occupied 1 m (x, y) column
+    int32_t* ncell_slot;     // per frame (stride N): hash slot of each occupied noise cell, by dense id
+    uint32_t* ncell_start;   // per frame (stride N): first cell-sorted position of each cell, by dense id
+    int32_t* hstart;         // per frame (stride hash_cap): first cell-sorted position of the slot's cell
     // per point
     int32_t* pt_cell;
     int32_t* pt_pix;
@@ -197,6 +200,7 @@ enum FsSlot : int {
     kFsKept,           // points kept by the noise filter
     kFsNoise,          // points labelled NOISE
     kFsNoiseHard,      // points left to the neighbour-cell search
+    kFsNoiseCells,     // occupied noise-filter cells
     kFsCount
 };
 

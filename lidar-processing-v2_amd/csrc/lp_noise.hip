@@ -88,6 +88,7 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
         d.fs[f * kFsCount + kFsKept] = 0;
         d.fs[f * kFsCount + kFsNoise] = 0;
         d.fs[f * kFsCount + kFsNoiseHard] = 0;
+        d.fs[f * kFsCount + kFsNoiseCells] = 0;
     }
 }
 
@@ -108,21 +109,31 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
     // runs at once.
     const WaveRun run = wave_runs(key);
     uint32_t h = 0;
+    bool fresh = false;
     if (lane_id() == run.leader) {
         h = noise_slot(key, mask);
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+            fresh = prev == -1;
             if (prev == -1 || prev == key) break;
             h = (h + 1) & mask;
         }
         atomicAdd(&d.hcount[ho + h], run.count);
         atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
     }
+    // Every occupied cell gets a dense id (in any order): the cell counts are
+    // scanned over the ids, not over the whole table.
+    if (fresh) {
+        const uint32_t id = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseCells]);
+        d.ncell_slot[(size_t)f * L.N + id] = static_cast<int32_t>(h);
+        d.hdense[ho + h] = static_cast<int32_t>(id);
+    }
     const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
     d.vslot[(size_t)f * L.N + i] = slot;
 }
 
-// hdense holds the exclusive scan of hcount (cell starts) on entry.
+// ncell_start holds the exclusive scan of the cell counts by dense id on
+// entry; the first position of every slot's cell goes to hstart.
 __global__ void k_noise_scatter(DevBuf d, Layout L) {
     int tile, f;
     xcd_tile_frame(tile, f);
@@ -132,7 +143,9 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const int gi = fd.pt_off + i;
     const size_t ho = (size_t)f * L.hash_cap;
     const int32_t h = d.vslot[(size_t)f * L.N + i];
-    const int pos = d.hdense[ho + h] + static_cast<int>(wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h])));
+    const int start = static_cast<int>(d.ncell_start[(size_t)f * L.N + d.hdense[ho + h]]);
+    const int pos = start + static_cast<int>(wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h])));
+    d.hstart[ho + h] = start;  // every point of the cell writes the same value
     const size_t po = (size_t)f * L.N;
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
     d.didx[po + pos] = i;
@@ -185,7 +198,7 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
         state = 1;  // a NaN radius admits no point
     } else {
         const int h = __float_as_int(t.w);
-        const int s0 = d.hdense[ho + h];
+        const int s0 = d.hstart[ho + h];
         const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
         uint32_t cnt = 0;
         int budget = kOwnCellBudget;
@@ -272,7 +285,7 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
         if (r < static_cast<float>(kNoiseMaxExtent - 1)) {
             cnt = carry & 0xffu;
             const int h0 = __float_as_int(t.w);
-            const int s0 = d.hdense[ho + h0];
+            const int s0 = d.hstart[ho + h0];
             const int s1 = s0 + static_cast<int>(d.hcount[ho + h0]);
             const int up = pos + static_cast<int>((carry >> 8) & 0xffu);
             const int dn = pos - 1 - static_cast<int>((carry >> 16) & 0xffu);
@@ -303,7 +316,7 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
                     // time: point g belongs to the lane whose prefix range
                     // holds it (binary search over the lanes' inclusive sums).
                     const int c_n = h >= 0 ? static_cast<int>(d.hcount[ho + h]) : 0;
-                    const int c_a = h >= 0 ? d.hdense[ho + h] : 0;
+                    const int c_a = h >= 0 ? d.hstart[ho + h] : 0;
                     const int incl = wave_inclusive_sum(c_n);
                     const int found = __shfl(incl, kWave - 1, kWave);
                     const int start = c_a - (incl - c_n);  // point g of this lane's cell: start + g
@@ -368,7 +381,7 @@ void launch_noise(lp_context* c, int B, bool compact) {
     stage_mark(c, "noise");
     hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_noise_insert, dim3(nb, B), dim3(256), 0, st, d, L);
-    launch_scan(c, B, d.hcount, reinterpret_cast<uint32_t*>(d.hdense), L.hash_cap, kCountHash, 0, -1);
+    launch_scan_gather(c, B, d.hcount, L.hash_cap, d.ncell_slot, d.ncell_start, L.N, kCountSlot, kFsNoiseCells, -1);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_count");
     hipLaunchKernelGGL(k_noise_count, dim3(nb, B), dim3(256), 0, st, d, L);

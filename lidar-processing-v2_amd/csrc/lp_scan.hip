@@ -20,19 +20,31 @@ __device__ __forceinline__ int scan_count(const DevBuf& d, const Layout& L, int 
     return static_cast<int>(d.fs[f * kFsCount + slot]);
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L, const uint32_t* in, size_t stride,
+// Element i of frame f: in[f * in_stride + i], or with a gather index
+// in[f * in_stride + gather[f * stride + i]].
+struct ScanIn {
+    const uint32_t* in;
+    size_t in_stride;
+    const int32_t* gather;
+};
+
+__device__ __forceinline__ uint32_t scan_value(const ScanIn& si, size_t stride, int f, int i) {
+    const uint32_t* a = si.in + (size_t)f * si.in_stride;
+    return si.gather ? a[si.gather[(size_t)f * stride + i]] : a[i];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L, ScanIn si, size_t stride,
                                                               ScanCount kind, int slot) {
     __shared__ uint32_t lds[kScanThreads / kWave + 1];
     const int f = blockIdx.y;
     const int n = scan_count(d, L, f, kind, slot);
     const int base = blockIdx.x * kScanTile;
     if (base >= n) return;
-    const uint32_t* a = in + (size_t)f * stride;
     uint32_t s = 0;
     const int b = base + threadIdx.x * kScanItems;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
-        if (b + k < n) s += a[b + k];
+        if (b + k < n) s += scan_value(si, stride, f, b + k);
     uint32_t total;
     block_exclusive_sum<kScanThreads>(s, lds, &total);
     if (threadIdx.x == 0) d.blk[(size_t)f * L.blk_stride + blockIdx.x] = total;
@@ -41,7 +53,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L
 // Down pass: each tile adds the totals of the tiles before it (read straight
 // from the reduce pass, at most blk_stride of them) to its own exclusive
 // scan; the last tile writes the frame total.  Two launches per scan.
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, const uint32_t* in, uint32_t* out,
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, ScanIn si, uint32_t* out,
                                                             size_t stride, ScanCount kind, int slot, int total_slot) {
     __shared__ uint32_t lds[kScanThreads / kWave + 1];
     const int f = blockIdx.y;
@@ -57,14 +69,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, 
     for (int i = threadIdx.x; i < tile; i += kScanThreads) pre += blk[i];
     uint32_t pre_total;
     block_exclusive_sum<kScanThreads>(pre, lds, &pre_total);
-    const uint32_t* a = in + (size_t)f * stride;
     uint32_t* o = out + (size_t)f * stride;
     uint32_t v[kScanItems];
     uint32_t s = 0;
     const int b = base + threadIdx.x * kScanItems;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (b + k < n) ? a[b + k] : 0u;
+        v[k] = (b + k < n) ? scan_value(si, stride, f, b + k) : 0u;
         s += v[k];
     }
     uint32_t total;
@@ -78,15 +89,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, 
         d.fs[f * kFsCount + total_slot] = pre_total + total;
 }
 
-void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
-                 int count_slot, int total_slot) {
+static void launch_scan_in(lp_context* c, int B, ScanIn si, uint32_t* out, size_t stride, ScanCount kind,
+                           int count_slot, int total_slot) {
     const Layout& L = c->L;
     const int maxn = static_cast<int>(stride);
     const int nb = (maxn + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, in, stride, kind,
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, si, stride, kind,
                        count_slot);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, in, out, stride, kind,
+    hipLaunchKernelGGL(k_scan_down, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, si, out, stride, kind,
                        count_slot, total_slot);
+}
+
+void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
+                 int count_slot, int total_slot) {
+    launch_scan_in(c, B, ScanIn{in, stride, nullptr}, out, stride, kind, count_slot, total_slot);
+}
+
+void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, const int32_t* gather,
+                        uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot) {
+    launch_scan_in(c, B, ScanIn{in, in_stride, gather}, out, stride, kind, count_slot, total_slot);
 }
 
 }  // namespace lp

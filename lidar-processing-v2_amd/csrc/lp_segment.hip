@@ -254,10 +254,10 @@ __device__ __forceinline__ void sort_cell_regs(const DevBuf& d, const FrameDesc&
 // One wavefront per non-empty cell (a grid-stride loop over the cell list):
 // sorts the cell's point indices ascending (so the cell lists follow the
 // reference's input order) for the RANSAC cells, and sorts z for the robust
-// minimum.  Cells up to 128 points sort in registers, up to kSmallCell in
-// LDS; larger ones are queued for k_cell_sort_big.
+// minimum.  Cells up to kSmallCell (512) points sort in registers; larger
+// ones are queued for k_cell_sort_big.
 __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
-    __shared__ uint32_t lds[4][kSmallCell];
+    static_assert(kSmallCell == 8 * kWave, "the register sorts cover cells up to kSmallCell points");
     int tile, f;
     xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
@@ -268,8 +268,6 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     const int used = static_cast<int>(d.fs[f * kFsCount + kFsCellsUsed]);
     const int waves = gridDim.x * 4;
     float* zmin = d.zmin + co;
-    uint32_t* a = lds[w];
-    auto sync = [] { wave_lds_sync(); };
     for (int q = tile * 4 + w; q < used; q += waves) {
         const int cell = static_cast<int>(d.cell_list[co + q]);
         const uint32_t n = d.cell_count[co + cell];
@@ -291,28 +289,11 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
             sort_cell_regs<2>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
             continue;
         }
-        int m = 1;
-        while (m < static_cast<int>(n)) m <<= 1;
-        if (radial < kRansacRadialBins) {
-            for (int t = lane; t < m; t += kWave) a[t] = t < static_cast<int>(n) ? static_cast<uint32_t>(items[t]) : 0xffffffffu;
-            wave_lds_sync();
-            bitonic_u32<kWave>(a, m, lane, sync);
-            for (int t = lane; t < static_cast<int>(n); t += kWave) items[t] = static_cast<int32_t>(a[t]);
-            wave_lds_sync();
+        if (n <= static_cast<uint32_t>(4 * kWave)) {
+            sort_cell_regs<4>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+            continue;
         }
-        if (radial >= 1) {
-            for (int t = lane; t < m; t += kWave)
-                a[t] = t < static_cast<int>(n) ? ord_f32(d.z[fd.pt_off + items[t]]) : 0xffffffffu;
-            wave_lds_sync();
-            bitonic_u32<kWave>(a, m, lane, sync);
-            const float r = robust_min<kWave>(a, static_cast<int>(n), lane, sync, [](int v) {
-#pragma unroll
-                for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-                return v;
-            });
-            if (lane == 0) zmin[cell] = r;
-        }
-        wave_lds_sync();  // the next cell reuses this wave's LDS
+        sort_cell_regs<8>(d, fd, items, static_cast<int>(n), radial, zmin + cell);  // n <= kSmallCell
     }
 }
 
