@@ -371,28 +371,83 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[
     }
 }
 
-// Sorts the ns points P[0..ns) of a chunk whose coordinates are all floats
-// (as the obstacle points are) by (x, y) with register keys, drops exact
-// duplicates as described at the call site unless keep_all, and rewrites
-// X, Y, I in sorted order with P the identity.  Returns the new count.
+// The whole chunk step for float coordinates, in registers: 64-bit keys of
+// the n loaded points, the Akl-Toussaint cull against the distinct extremes
+// (key minima / maxima: x-major gives the leftmost and rightmost points,
+// y-major the lowest and highest; any extreme points serve), then the sort,
+// the sort by (x, y), the duplicate drop (unless fewer than three points
+// survive the cull) and the write of X, Y, I in sorted order with P the
+// identity.  Returns the new count; *ns_in receives the number of points the
+// cull kept.
 template <int S>
-__device__ int sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t* P, int ns, bool keep_all) {
+__device__ int cull_sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t* P, int n, int* ns_in) {
     const int lane = lane_id();
     uint64_t key[S];
     uint32_t val[S];
+    uint64_t mnx = ~0ull, mxx = 0ull, mny = ~0ull, mxy = 0ull;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int e = s * kWave + lane;
-        if (e < ns) {
-            const int p = P[e];
-            key[s] = (static_cast<uint64_t>(float_key(static_cast<float>(X[p]))) << 32) |
-                     float_key(static_cast<float>(Y[p]));
-            val[s] = static_cast<uint32_t>(p);
+        if (e < n) {
+            key[s] = (static_cast<uint64_t>(float_key(static_cast<float>(X[e]))) << 32) |
+                     float_key(static_cast<float>(Y[e]));
+            val[s] = static_cast<uint32_t>(e);
+            const uint64_t ky = (key[s] << 32) | (key[s] >> 32);
+            mnx = min(mnx, key[s]);
+            mxx = max(mxx, key[s]);
+            mny = min(mny, ky);
+            mxy = max(mxy, ky);
         } else {
-            key[s] = ~0ull;  // padding sorts last
+            key[s] = ~0ull;
             val[s] = static_cast<uint32_t>(kHullChunk + e);
         }
     }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        mnx = min(mnx, static_cast<uint64_t>(__shfl_xor(mnx, o, kWave)));
+        mxx = max(mxx, static_cast<uint64_t>(__shfl_xor(mxx, o, kWave)));
+        mny = min(mny, static_cast<uint64_t>(__shfl_xor(mny, o, kWave)));
+        mxy = max(mxy, static_cast<uint64_t>(__shfl_xor(mxy, o, kWave)));
+    }
+    // Counter-clockwise: leftmost, lowest, rightmost, highest; repeated
+    // points dropped so no edge has zero length.
+    const uint64_t ext[4] = {mnx, (mny << 32) | (mny >> 32), mxx, (mxy << 32) | (mxy >> 32)};
+    double px[4], py[4];
+    int npoly = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool fresh = npoly == 0 || ext[e] != ext[e > 0 ? e - 1 : 0];
+        if (fresh) {
+            px[npoly] = static_cast<double>(key_float(static_cast<uint32_t>(ext[e] >> 32)));
+            py[npoly] = static_cast<double>(key_float(static_cast<uint32_t>(ext[e])));
+            ++npoly;
+        }
+    }
+    if (npoly > 1 && ext[3] == ext[0]) --npoly;
+    int ns = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = s * kWave + lane;
+        bool keep = e < n;
+        if (keep && npoly >= 3) {
+            const double qx = static_cast<double>(key_float(static_cast<uint32_t>(key[s] >> 32)));
+            const double qy = static_cast<double>(key_float(static_cast<uint32_t>(key[s])));
+            bool inside = true;
+            int dummy;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < npoly && inside) {
+                    const int b = k + 1 < npoly ? k + 1 : 0;
+                    inside = orient_exact(px[k], py[k], px[b], py[b], qx, qy, &dummy) > 0;
+                }
+            }
+            keep = !inside;
+        }
+        if (e < n && !keep) key[s] = ~0ull;  // culled: sorts last
+        ns += static_cast<int>(__popcll(__ballot(keep)));
+    }
+    *ns_in = ns;
+    const bool keep_all = ns < 3;
     reg_bitonic<S>(key, val);
     int32_t idx[S];
     uint64_t prev[S];
@@ -505,6 +560,33 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             I[i] = d.hci[src];
         }
         wave_sync();
+        // Coordinates that are all floats (every obstacle point) take the
+        // register path: keys, cull, sort and duplicate drop in one pass.
+        // Other lists use the LDS forms below.
+        bool as_float = true;
+        for (int i = lane; i < n; i += kWave) {
+            const double x = X[i], y = Y[i];
+            as_float = as_float && static_cast<double>(static_cast<float>(x)) == x &&
+                       static_cast<double>(static_cast<float>(y)) == y;
+        }
+        as_float = __ballot(!as_float) == 0;
+        int ns = 0;
+        int ns_in = 0;
+        // Exact duplicates (same x and y) are adjacent after the sort.  A
+        // second copy of a point is never a hull vertex of a set with two or
+        // more distinct points (the reference's chain pops it: its turn is
+        // zero), so only the first copy stays; a list of one repeated point
+        // keeps two copies, the reference's result for it.  Vertical surfaces
+        // repeat one (x, y) once per beam, so this often shrinks a chunk
+        // several times over.  Fewer than three points are kept as they are.
+        if (as_float) {
+            if (n <= kWave)
+                ns = cull_sort_float_chunk<1>(X, Y, I, P, n, &ns_in);
+            else if (n <= 2 * kWave)
+                ns = cull_sort_float_chunk<2>(X, Y, I, P, n, &ns_in);
+            else
+                ns = cull_sort_float_chunk<4>(X, Y, I, P, n, &ns_in);
+        } else {
         // Akl-Toussaint cull: points strictly inside the quadrilateral of the
         // leftmost, lowest, rightmost and highest points (counter-clockwise
         // along the hull boundary) are no hull vertices; exact tests only.
@@ -535,7 +617,6 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             if (npoly == 0 || X[v] != X[poly[npoly - 1]] || Y[v] != Y[poly[npoly - 1]]) poly[npoly++] = v;
         }
         if (npoly > 1 && X[poly[npoly - 1]] == X[poly[0]] && Y[poly[npoly - 1]] == Y[poly[0]]) --npoly;
-        int ns = 0;
         for (int i0 = 0; i0 < n; i0 += kWave) {
             const int i = i0 + lane;
             bool keep = false;
@@ -558,32 +639,7 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
             if (keep) P[ns + lanes_below(mask)] = static_cast<uint16_t>(i);
             ns += static_cast<int>(__popcll(mask));
         }
-        // Coordinates that are all floats (every obstacle point) sort as
-        // 64-bit keys in registers; other lists take the LDS sort below.
-        bool as_float = true;
-        for (int i = lane; i < n; i += kWave) {
-            const double x = X[i], y = Y[i];
-            as_float = as_float && static_cast<double>(static_cast<float>(x)) == x &&
-                       static_cast<double>(static_cast<float>(y)) == y;
-        }
-        as_float = __ballot(!as_float) == 0;
-        const int ns_in = ns;
-        // Exact duplicates (same x and y) are adjacent after the sort.  A
-        // second copy of a point is never a hull vertex of a set with two or
-        // more distinct points (the reference's chain pops it: its turn is
-        // zero), so only the first copy stays; a list of one repeated point
-        // keeps two copies, the reference's result for it.  Vertical surfaces
-        // repeat one (x, y) once per beam, so this often shrinks a chunk
-        // several times over.  Fewer than three points are kept as they are.
-        if (as_float) {
-            wave_sync();
-            if (ns <= kWave)
-                ns = sort_float_chunk<1>(X, Y, I, P, ns, ns_in < 3);
-            else if (ns <= 2 * kWave)
-                ns = sort_float_chunk<2>(X, Y, I, P, ns, ns_in < 3);
-            else
-                ns = sort_float_chunk<4>(X, Y, I, P, ns, ns_in < 3);
-        } else {
+        ns_in = ns;
         int m = 1;
         while (m < ns) m <<= 1;
         for (int i = ns + lane; i < m; i += kWave) P[i] = static_cast<uint16_t>(n + i);  // padding sorts last
