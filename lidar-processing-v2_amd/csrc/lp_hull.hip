@@ -908,7 +908,7 @@ __global__ void k_hull_check(DevBuf d, Layout L) {
 // one wavefront per vertex.
 constexpr int kLocalMax = 512;
 constexpr int kLocalHash = 2 * kLocalMax;
-constexpr int kLocalThreads = 256;
+constexpr int kLocalThreads = 1024;
 constexpr int kLocalUnroll = 4;
 
 __device__ __forceinline__ void member_xy(const DevBuf& d, const double2* list_xy, size_t ob, uint32_t base, int32_t m,
