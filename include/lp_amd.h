@@ -24,6 +24,11 @@
  *   lp_oriented_boxes replaces a loop of Polygonizer::boundingBoxRotatingCalipers calls
  *                     (lidar_processing_lib/include/lidar_processing_lib/polygonizer.hpp:115,
  *                      lidar_processing_lib/src/polygonizer.cpp:93-278)
+ *   lp_pca_boxes      replaces a loop of Polygonizer::boundingBoxPrincipalComponentAnalysis calls
+ *                     (lidar_processing_lib/include/lidar_processing_lib/polygonizer.hpp:118-119,
+ *                      lidar_processing_lib/src/polygonizer.cpp:280-362)
+ *   lp_frame_image    replaces Segmenter::image
+ *                     (lidar_processing_lib/include/lidar_processing_lib/segmenter.hpp:163-166)
  *
  * Conventions (mirroring the reference):
  *   - the caller owns inputs and outputs; every output entry is written
@@ -203,6 +208,14 @@ typedef struct lp_bounding_box {
 int lp_oriented_boxes(lp_context* ctx, const double* x, const double* y, const int32_t* offsets,
                       int32_t num_polygons, lp_bounding_box* boxes);
 
+/* Polygonizer::boundingBoxPrincipalComponentAnalysis on num_polygons point
+ * lists packed by offsets: the box of the points in the frame of the right
+ * singular vectors of their covariance (largest first), corners in the order
+ * (min,min), (max,min), (max,max), (min,max) of that frame.  Lists of fewer
+ * than 3 points give is_valid = 0. */
+int lp_pca_boxes(lp_context* ctx, const double* x, const double* y, const int32_t* offsets,
+                 int32_t num_polygons, lp_bounding_box* boxes);
+
 /* Dataloader::addRingInfo (src/dataloader/src/dataloader.cpp:68-137): ring
  * index per point of a scan-ordered cloud (63 down to 0, one step at each
  * fourth-to-first azimuth quadrant transition). */
@@ -219,6 +232,9 @@ typedef struct lp_frame_outputs {
     int32_t hull_capacity;
     uint8_t* noise_labels;      /* [num_points] with the noise filter on (else unused, may be NULL) */
     lp_bounding_box* boxes;     /* [num_points] (first num_clusters used) with boxes on, else may be NULL */
+    int32_t* ground_index;      /* [num_points] with label clouds on: input index of each GROUND point */
+    int32_t* unknown_index;     /* [num_points] with label clouds on: input index of each UNKNOWN point */
+    int32_t* vehicle_class;     /* [num_points] (first num_clusters used) with vehicle boxes on: -1 or 0..4 */
 } lp_frame_outputs;
 
 typedef struct lp_frame_result {
@@ -233,6 +249,9 @@ typedef struct lp_frame_result {
     int32_t num_noise;          /* points removed by the noise filter */
     int32_t num_boxes;          /* clusters with an oriented box computed */
     int32_t num_noise_searched; /* points the noise filter took past their own grid cell */
+    int32_t num_ground;         /* GROUND points (label clouds on, else 0) */
+    int32_t num_unknown;        /* UNKNOWN points (label clouds on, else 0) */
+    int32_t num_vehicles;       /* clusters classified as a vehicle (vehicle boxes on, else 0) */
 } lp_frame_result;
 
 /* Optional stages of the fused path, kept by the context until changed:
@@ -242,6 +261,22 @@ typedef struct lp_frame_result {
  * box per cluster hull (boundingBoxRotatingCalipers over the hull vertices). */
 int lp_set_noise_filter(lp_context* ctx, const lp_noise_config* cfg);
 int lp_set_oriented_boxes(lp_context* ctx, int32_t enable);
+/* The ground / obstacle / unsegmented split of processor.cpp:557-578 in full:
+ * besides obstacle_index, the input-order indices of the GROUND and of the
+ * UNKNOWN points (ground_cloud_ and unsegmented_cloud_). */
+int lp_set_label_clouds(lp_context* ctx, int32_t enable);
+/* The vehicle test of the processor's polygon simplification
+ * (processor.cpp:680-760 with the tables of processor.hpp:78-192), per
+ * cluster: a hull of >= 3 vertices, more than 150 points, height z_max - z_min
+ * and hull volume inside the loosest vehicle bounds, a valid rotating-calipers
+ * box covering the hull with area ratio > 0.4, then the first vehicle type
+ * (0 compact, 1 sedan, 2 SUV, 3 truck, 4 minivan) whose length, width,
+ * height, volume and area bounds all hold; -1 when none does.  A cluster with
+ * class >= 0 is drawn as its box (boxes[k].corners).  Switches the boxes on
+ * for the clusters it tests.  The reference's branch (disabled there by a
+ * constant, processor.cpp:676) fits the box to the raw cluster points; this
+ * fits it to the cluster's hull, the polygon the calipers are defined on. */
+int lp_set_vehicle_boxes(lp_context* ctx, int32_t enable);
 
 /* Processes num_frames frames as one batch (CSR-packed on the device, one
  * launch sequence).  cfgs may hold one config (shared) or num_frames. */
@@ -283,6 +318,8 @@ typedef struct lp_server_config {
     int32_t noise_filter;          /* 1: noise filter (below) ahead of the segmenter */
     lp_noise_config noise;
     int32_t oriented_boxes;        /* 1: a rotating-calipers box per cluster hull */
+    int32_t label_clouds;          /* 1: ground and unknown index lists (lp_set_label_clouds) */
+    int32_t vehicle_boxes;         /* 1: vehicle classes per cluster (lp_set_vehicle_boxes) */
 } lp_server_config;
 void lp_server_config_default(lp_server_config* cfg);
 
@@ -331,6 +368,15 @@ enum {
     LP_DEBUG_STATE_FINAL = 4    /* uint8[H*W]: 0 empty, 1 ground, 2 obstacle, 3 unresolved */
 };
 int lp_debug_fetch(lp_context* ctx, int32_t frame, int32_t which, void* dst, int32_t capacity);
+
+/* Segmenter::image after segment() for frame `frame` of the last run: H*W
+ * BGR pixels (row-major, row 0 = image row 0), black where no point landed,
+ * red (0,0,255) for obstacle pixels and for empty pixels the 5x5 dilation of
+ * the obstacle pixels reaches, green (0,255,0) for ground, blue (255,0,0) for
+ * queue pixels JCP left unresolved (segmenter.cpp:288-318, 484-513, 626-635).
+ * Built on the device from the run's pixel states.  Returns the byte count
+ * written (or needed, when larger than `capacity`). */
+int lp_frame_image(lp_context* ctx, int32_t frame, uint8_t* bgr, int32_t capacity);
 
 /* The device restatements of the C library functions the reference calls
  * (expf: segmenter.cpp:588; atanf / atan2f: clusterer.cpp:74,86,

@@ -185,6 +185,15 @@ class CloudView {
     std::vector<std::uint8_t> packed_;
 };
 
+// The label image of Segmenter::image() (a CV_8UC3 cv::Mat in the reference,
+// segmenter.hpp:163-166): rows x cols BGR pixels, row-major.
+struct LabelImage {
+    std::int32_t rows = 0;
+    std::int32_t cols = 0;
+    std::vector<std::uint8_t> bgr;
+    const std::uint8_t* at(std::int32_t row, std::int32_t col) const { return bgr.data() + 3 * (row * cols + col); }
+};
+
 class Segmenter {
   public:
     Segmenter() : ctx_(new detail::Context()) {}
@@ -199,6 +208,7 @@ class Segmenter {
         lp_cloud in = detail::make_cloud(cloud, !config_.assume_unorganized_cloud);
         labels.assign(cloud.points.size(), Label::UNKNOWN);
         detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
+        image_valid_ = false;
     }
 
     // A PointCloud2 message as it arrives (Processor::run without the
@@ -210,6 +220,21 @@ class Segmenter {
         if (config_.assume_unorganized_cloud) in.ring = nullptr;
         labels.assign(view.size(), Label::UNKNOWN);
         detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
+        image_valid_ = false;
+    }
+
+    // The label image of the last segment() call, built on the device when
+    // first asked for.
+    const LabelImage& image() const {
+        if (!image_valid_) {
+            image_.rows = config_.image_height;
+            image_.cols = config_.image_width;
+            image_.bgr.resize(3 * static_cast<std::size_t>(image_.rows) * image_.cols);
+            const int got = lp_frame_image(ctx_->get(), 0, image_.bgr.data(), static_cast<std::int32_t>(image_.bgr.size()));
+            if (got < 0) detail::check(got);
+            image_valid_ = true;
+        }
+        return image_;
     }
 
     // The RANSAC plane the reference keeps in locals (segmenter.cpp:363-374).
@@ -240,6 +265,8 @@ class Segmenter {
     SegmenterConfiguration config_;
     std::unique_ptr<detail::Context> ctx_;
     lp_ransac_result ransac_{};
+    mutable LabelImage image_;
+    mutable bool image_valid_ = false;
 };
 
 class Clusterer {
@@ -314,6 +341,27 @@ class Polygonizer {
     // Many polygons in one call.
     void boundingBoxesRotatingCalipers(const std::vector<const std::vector<PointXY>*>& polygons,
                                        std::vector<BoundingBox>& boxes) {
+        boxes_of(polygons, boxes, lp_oriented_boxes);
+    }
+
+    // Principal-component box of a point list (polygonizer.cpp:280-362).
+    BoundingBox boundingBoxPrincipalComponentAnalysis(const std::vector<PointXY>& convex_hull_points) {
+        std::vector<BoundingBox> out;
+        boxes_of(std::vector<const std::vector<PointXY>*>{&convex_hull_points}, out, lp_pca_boxes);
+        return out[0];
+    }
+
+    void boundingBoxesPrincipalComponentAnalysis(const std::vector<const std::vector<PointXY>*>& polygons,
+                                                 std::vector<BoundingBox>& boxes) {
+        boxes_of(polygons, boxes, lp_pca_boxes);
+    }
+
+  private:
+    using BoxFn = int (*)(lp_context*, const double*, const double*, const std::int32_t*, std::int32_t,
+                          lp_bounding_box*);
+
+    void boxes_of(const std::vector<const std::vector<PointXY>*>& polygons, std::vector<BoundingBox>& boxes,
+                  BoxFn fn) {
         std::vector<double> x, y;
         std::vector<std::int32_t> offsets(1, 0);
         for (const auto* poly : polygons) {
@@ -324,8 +372,8 @@ class Polygonizer {
             offsets.push_back(static_cast<std::int32_t>(x.size()));
         }
         std::vector<lp_bounding_box> raw(polygons.size());
-        detail::check(lp_oriented_boxes(ctx_->get(), x.data(), y.data(), offsets.data(),
-                                        static_cast<std::int32_t>(polygons.size()), raw.data()));
+        detail::check(fn(ctx_->get(), x.data(), y.data(), offsets.data(), static_cast<std::int32_t>(polygons.size()),
+                         raw.data()));
         boxes.resize(polygons.size());
         for (std::size_t k = 0; k < polygons.size(); ++k) {
             for (int c = 0; c < 4; ++c) boxes[k].corners[c] = {raw[k].corners[2 * c], raw[k].corners[2 * c + 1]};
@@ -335,7 +383,6 @@ class Polygonizer {
         }
     }
 
-  private:
     std::unique_ptr<detail::Context> ctx_;
 };
 

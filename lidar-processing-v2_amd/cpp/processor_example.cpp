@@ -2,7 +2,7 @@
 // (src/processor/src/processor.cpp:556-670) written against the drop-in
 // lidar_processing_lib classes of lidar_processing_lib_amd.hpp.
 //
-// Usage: processor_example <in.bin> <out.bin> [full | msg]
+// Usage: processor_example <in.bin> <out.bin> [full | msg | image]
 //   in.bin : int32 n, then n records {float x, y, z, pad, intensity; uint16 ring; 10 bytes padding}
 //            (the 32-byte pcl::PointXYZIR layout)
 //   out.bin: uint32 seg_labels[n], int32 n_obs, int32 cluster_labels[n_obs],
@@ -15,6 +15,9 @@
 //   msg    : the records are handed to the segmenter as a PointCloud2 message
 //            (fields x, y, z, intensity, ring; CloudView, no conversion to a
 //            pcl::PointCloud); the outputs are those of the default mode.
+//   image  : the default mode's outputs followed by the segmenter's label
+//            image (Segmenter::image, published by processor.cpp:804-816):
+//            int32 rows, int32 cols, rows*cols BGR pixels.
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -45,12 +48,13 @@ struct Cloud {  // the part of pcl::PointCloud<P> the library uses
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3 && !(argc == 4 && (std::string(argv[3]) == "full" || std::string(argv[3]) == "msg"))) {
-        std::fprintf(stderr, "usage: %s in.bin out.bin [full | msg]\n", argv[0]);
+    const std::string mode = argc == 4 ? argv[3] : "";
+    if (argc != 3 && !(argc == 4 && (mode == "full" || mode == "msg" || mode == "image"))) {
+        std::fprintf(stderr, "usage: %s in.bin out.bin [full | msg | image]\n", argv[0]);
         return 2;
     }
-    const bool full = argc == 4 && std::string(argv[3]) == "full";
-    const bool as_msg = argc == 4 && std::string(argv[3]) == "msg";
+    const bool full = mode == "full";
+    const bool as_msg = mode == "msg";
     std::FILE* fi = std::fopen(argv[1], "rb");
     if (!fi) return 2;
     std::int32_t n_in = 0;
@@ -166,6 +170,12 @@ int main(int argc, char** argv) {
         std::fwrite(&b.area, 4, 1, fo);
         std::fwrite(&b.angle_rad, 4, 1, fo);
         std::fwrite(&valid, 4, 1, fo);
+    }
+    if (mode == "image") {
+        const lidar_processing_lib::LabelImage& img = segmenter.image();
+        std::fwrite(&img.rows, 4, 1, fo);
+        std::fwrite(&img.cols, 4, 1, fo);
+        std::fwrite(img.bgr.data(), 1, img.bgr.size(), fo);
     }
     std::fclose(fo);
     std::printf("points %d obstacles %d clusters %d\n", n, no, k);

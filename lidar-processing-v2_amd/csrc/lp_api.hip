@@ -229,6 +229,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.cc, 1);
     go(x.nc, 1);
     go(x.boxes, B * static_cast<size_t>(L.box_stride));
+    go(x.vclass, B * static_cast<size_t>(L.box_stride));
     return static_cast<size_t>(p - base);
 }
 
@@ -281,7 +282,8 @@ int run_pipeline(lp_context* c, int B, bool segment_only) {
     if (!segment_only) {
         launch_cluster(c, B, true);
         launch_hull(c, B);
-        if (c->boxes_on) launch_boxes(c, B, true);
+        if (c->boxes_on || c->vehicles_on) launch_boxes(c, B, true);
+        if (c->vehicles_on) launch_vehicles(c, B);
     }
     LP_HIP(hipGetLastError());
     return LP_OK;
@@ -457,6 +459,8 @@ int lp_context_destroy(lp_context* c) {
     for (auto e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->arena) (void)hipFree(c->arena);
+    if (c->clouds_mem) (void)hipFree(c->clouds_mem);
+    if (c->image_dev) (void)hipFree(c->image_dev);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -548,7 +552,8 @@ int run_lane(lp_context* c, int32_t use_graph) {
     LP_HIP(hipSetDevice(c->device));
     const int B = c->num_frames;
     if (!use_graph) return run_pipeline(c, B, false);
-    const int options = (c->staged_filtered ? 1 : 0) | (c->boxes_on ? 2 : 0);
+    const int options = (c->staged_filtered ? 1 : 0) | (c->boxes_on ? 2 : 0) | (c->clouds_on ? 4 : 0) |
+                        (c->vehicles_on ? 8 : 0);
     lp_context::GraphEntry* ge = nullptr;
     for (auto& g : c->graphs)
         if (g.frames == B && g.options == options) ge = &g;
@@ -627,12 +632,18 @@ int fetch_lane(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* 
         r.num_points = n;
         r.num_noise = c->staged_filtered ? static_cast<int32_t>(s[kFsNoise]) : 0;
         r.num_noise_searched = c->staged_filtered ? static_cast<int32_t>(s[kFsNoiseHard]) : 0;
-        r.num_boxes = c->boxes_on ? static_cast<int32_t>(s[kFsBoxes]) : 0;
+        const bool boxes = c->boxes_on || c->vehicles_on;
+        r.num_boxes = boxes ? static_cast<int32_t>(s[kFsBoxes]) : 0;
         r.num_obstacles = static_cast<int32_t>(s[kFsObstacles]);
         r.num_voxels = static_cast<int32_t>(s[kFsVoxels]);
         r.num_clusters = static_cast<int32_t>(s[kFsClusters]);
         r.num_hull_vertices = static_cast<int32_t>(s[kFsHullTotal]);
         r.jcp_queue = static_cast<int32_t>(s[kFsQueue]);
+        if (c->clouds_on) {
+            r.num_ground = static_cast<int32_t>(s[kFsGround]);
+            r.num_unknown = n - r.num_ground - r.num_obstacles;
+        }
+        r.num_vehicles = c->vehicles_on ? static_cast<int32_t>(s[kFsVehicles]) : 0;
         r.ransac.candidates = s[kFsCands];
         r.ransac.best_inliers = s[kFsBest];
         r.ransac.applied = static_cast<int32_t>(s[kFsApplied]);
@@ -669,9 +680,18 @@ int fetch_lane(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* 
             }
             LP_HIP(hipMemcpyAsync(o.hull_indices, c->d.hull_local + ob, V * 4, hipMemcpyDeviceToHost, c->stream));
         }
-        if (c->boxes_on && o.boxes && K)
+        if (boxes && o.boxes && K)
             LP_HIP(hipMemcpyAsync(o.boxes, c->d.boxes + static_cast<size_t>(f) * L.box_stride,
                                   K * sizeof(lp_bounding_box), hipMemcpyDeviceToHost, c->stream));
+        if (c->vehicles_on && o.vehicle_class && K)
+            LP_HIP(hipMemcpyAsync(o.vehicle_class, c->d.vclass + static_cast<size_t>(f) * L.box_stride,
+                                  K * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        if (c->clouds_on && o.ground_index && r.num_ground)
+            LP_HIP(hipMemcpyAsync(o.ground_index, c->d.split_idx + ob, r.num_ground * 4, hipMemcpyDeviceToHost,
+                                  c->stream));
+        if (c->clouds_on && o.unknown_index && r.num_unknown)
+            LP_HIP(hipMemcpyAsync(o.unknown_index, c->d.split_idx + ob + r.num_ground, r.num_unknown * 4,
+                                  hipMemcpyDeviceToHost, c->stream));
     }
     LP_HIP(hipStreamSynchronize(c->stream));
     return overall;
@@ -794,6 +814,30 @@ int lp_set_oriented_boxes(lp_context* c, int32_t enable) {
     return LP_OK;
 }
 
+int lp_set_label_clouds(lp_context* c, int32_t enable) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    if (enable && !c->clouds_mem) {
+        LP_HIP(hipSetDevice(c->device));
+        const size_t count = static_cast<size_t>(c->L.B) * c->L.N;
+        LP_HIP(hipMalloc(&c->clouds_mem, 2 * count * sizeof(uint32_t)));
+        c->d.gnd_flag = static_cast<uint32_t*>(c->clouds_mem);
+        c->d.split_idx = reinterpret_cast<int32_t*>(c->d.gnd_flag + count);
+    }
+    c->clouds_on = enable != 0;
+    for (auto* lane : c->lanes) {
+        const int st = lp_set_label_clouds(lane, enable);
+        if (st != LP_OK) return st;
+    }
+    return LP_OK;
+}
+
+int lp_set_vehicle_boxes(lp_context* c, int32_t enable) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    c->vehicles_on = enable != 0;
+    for (auto* lane : c->lanes) lane->vehicles_on = c->vehicles_on;
+    return LP_OK;
+}
+
 int lp_remove_noise(lp_context* c, const lp_noise_config* cfg, const lp_cloud* cloud, uint8_t* labels) {
     if (!c || !cfg || !cloud || (!labels && cloud->num_points > 0))
         return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
@@ -823,8 +867,12 @@ int lp_remove_noise(lp_context* c, const lp_noise_config* cfg, const lp_cloud* c
     return LP_OK;
 }
 
-int lp_oriented_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,
-                      lp_bounding_box* boxes) {
+namespace {
+
+// Stages num_polygons point lists (x, y packed by offsets) as frame 0's hull
+// coordinates and runs `launch` over them; boxes[k] receives list k's box.
+int polygon_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,
+                  lp_bounding_box* boxes, void (*launch)(lp_context*, int)) {
     if (!c || !offsets || num_polygons < 0 || (num_polygons > 0 && !boxes))
         return fail(LP_ERR_INVALID_ARGUMENT, "null argument");
     if (num_polygons == 0) return LP_OK;
@@ -847,7 +895,7 @@ int lp_oriented_boxes(lp_context* c, const double* x, const double* y, const int
     }
     c->num_frames = 0;
     c->L.B_active = 1;
-    launch_boxes(c, 1, false);
+    launch(c, num_polygons);
     LP_HIP(hipGetLastError());
     LP_HIP(hipMemcpyAsync(fs.data(), c->d.fs, kFsCount * 4, hipMemcpyDeviceToHost, c->stream));
     LP_HIP(hipMemcpyAsync(boxes, c->d.boxes, num_polygons * sizeof(lp_bounding_box), hipMemcpyDeviceToHost,
@@ -855,6 +903,20 @@ int lp_oriented_boxes(lp_context* c, const double* x, const double* y, const int
     LP_HIP(hipStreamSynchronize(c->stream));
     if (fs[kFsErr] & kErrBoxCapacity) return fail(LP_ERR_CAPACITY, "a polygon exceeds the box kernel's capacity");
     return LP_OK;
+}
+
+}  // namespace
+
+int lp_oriented_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,
+                      lp_bounding_box* boxes) {
+    return polygon_boxes(c, x, y, offsets, num_polygons, boxes,
+                         [](lp_context* ctx, int) { launch_boxes(ctx, 1, false); });
+}
+
+int lp_pca_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,
+                 lp_bounding_box* boxes) {
+    return polygon_boxes(c, x, y, offsets, num_polygons, boxes,
+                         [](lp_context* ctx, int k) { launch_pca_boxes(ctx, k); });
 }
 
 int lp_add_ring_info(lp_context* c, const float* x, const float* y, int32_t n, uint16_t* ring) {
@@ -961,6 +1023,32 @@ int lp_debug_fetch(lp_context* c, int32_t frame, int32_t which, void* dst, int32
         default:
             return fail(LP_ERR_INVALID_ARGUMENT, "unknown debug buffer");
     }
+}
+
+int lp_frame_image(lp_context* c, int32_t frame, uint8_t* bgr, int32_t capacity) {
+    if (!c || frame < 0 || !bgr) return fail(LP_ERR_INVALID_ARGUMENT, "bad image request");
+    // The lane that ran the frame: lane k holds the k-th contiguous share.
+    lp_context* lane = c;
+    int32_t f = frame;
+    if (!c->lanes.empty()) {
+        size_t k = 0;
+        while (k < c->lane_frames.size() && f >= c->lane_frames[k]) f -= c->lane_frames[k++];
+        if (k == c->lane_frames.size()) return fail(LP_ERR_INVALID_ARGUMENT, "frame not in the last batch");
+        lane = k == 0 ? c : c->lanes[k - 1];
+    }
+    if (f >= lane->num_frames || f >= static_cast<int32_t>(lane->frames.size()))
+        return fail(LP_ERR_INVALID_ARGUMENT, "frame not in the last batch");
+    const SegConst& sc = lane->frames[f].sc;
+    const int32_t pixels = sc.H * sc.W;
+    const int32_t bytes = 3 * pixels;
+    if (bytes > capacity) return bytes;
+    LP_HIP(hipSetDevice(lane->device));
+    if (!lane->image_dev) LP_HIP(hipMalloc(&lane->image_dev, 3 * static_cast<size_t>(lane->L.P)));
+    launch_label_image(lane, f, pixels, lane->image_dev);
+    LP_HIP(hipGetLastError());
+    LP_HIP(hipMemcpyAsync(bgr, lane->image_dev, bytes, hipMemcpyDeviceToHost, lane->stream));
+    LP_HIP(hipStreamSynchronize(lane->stream));
+    return bytes;
 }
 
 int lp_stage_names(lp_context* c, char* buf, int32_t buf_len) {

@@ -43,6 +43,28 @@ __global__ void k_obs_gather(DevBuf d, Layout L) {
     d.oz[ob + k] = d.z[gi];
 }
 
+// Ground and unsegmented clouds of the same split (processor.cpp:565-577):
+// with gnd_flag scanned into obs_flag and cl_flag still the obstacle scan,
+// an UNKNOWN point's position among the unknowns is its index less the
+// ground and obstacle points before it.  split_idx holds the ground indices
+// followed by the unknown ones.
+__global__ void k_label_split(DevBuf d, Layout L) {
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fd.n) return;
+    const uint32_t lab = d.labels[fd.pt_off + i];
+    if (lab == LP_LABEL_OBSTACLE) return;
+    const size_t ob = (size_t)f * L.N;
+    const uint32_t g = d.obs_flag[ob + i];
+    if (lab == LP_LABEL_GROUND) {
+        d.split_idx[ob + g] = i;
+    } else {
+        const uint32_t ng = d.fs[f * kFsCount + kFsGround];
+        d.split_idx[ob + ng + (i - g - d.cl_flag[ob + i])] = i;
+    }
+}
+
 // Standalone Clusterer::cluster input: the staged cloud is the obstacle cloud.
 __global__ void k_cluster_input(DevBuf d, Layout L) {
     const int f = blockIdx.y;
@@ -418,6 +440,10 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     if (from_split) {
         launch_scan(c, B, d.obs_flag, d.cl_flag, L.N, kCountPoints, 0, kFsObstacles);
         hipLaunchKernelGGL(k_obs_gather, dim3(nb, B), dim3(256), 0, st, d, L);
+        if (c->clouds_on) {
+            launch_scan(c, B, d.gnd_flag, d.obs_flag, L.N, kCountPoints, 0, kFsGround);
+            hipLaunchKernelGGL(k_label_split, dim3(nb, B), dim3(256), 0, st, d, L);
+        }
     } else {
         hipLaunchKernelGGL(k_cluster_input, dim3(nb, B), dim3(256), 0, st, d, L);
     }

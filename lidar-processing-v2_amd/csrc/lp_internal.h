@@ -175,6 +175,11 @@ struct DevBuf {
     CluConst* cc;            // [1]
     NoiseConst* nc;          // [1]
     lp_bounding_box* boxes;  // [B][box_stride]
+    int32_t* vclass;         // [B][box_stride] vehicle class per cluster
+    // label clouds (allocated when switched on): ground flags and the
+    // ground-then-unknown index lists (frame stride N)
+    uint32_t* gnd_flag;
+    int32_t* split_idx;
 };
 
 // Per-frame scalar slots in DevBuf::fs.
@@ -196,6 +201,8 @@ enum FsSlot : int {
     kFsErr,
     kFsBoxes,          // clusters with a box
     kFsCellsUsed,      // non-empty polar cells
+    kFsGround,         // GROUND points (label clouds on)
+    kFsVehicles,       // clusters classified as vehicles
     // kept across the per-run clear: written by the noise filter before it
     kFsKept,           // points kept by the noise filter
     kFsNoise,          // points labelled NOISE
@@ -234,6 +241,10 @@ struct lp_context {
     bool noise_on = false;
     lp::NoiseConst nc{};
     bool boxes_on = false;
+    bool clouds_on = false;
+    bool vehicles_on = false;
+    void* clouds_mem = nullptr;     // gnd_flag + split_idx
+    uint8_t* image_dev = nullptr;   // one frame's label image (allocated on first use)
     bool staged_filtered = false;   // the staged batch went through the noise filter
     // pinned host buffers for inputs / outputs
     void* pinned = nullptr;

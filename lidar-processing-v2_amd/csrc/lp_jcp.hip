@@ -425,6 +425,49 @@ __global__ void k_seg_labels(DevBuf d, Layout L) {
     }
     d.labels[gi] = lab;
     d.obs_flag[(size_t)f * L.N + i] = lab == LP_LABEL_OBSTACLE ? 1u : 0u;
+    if (d.gnd_flag) d.gnd_flag[(size_t)f * L.N + i] = lab == LP_LABEL_GROUND ? 1u : 0u;
+}
+
+// Segmenter::image after segment() (segmenter.hpp:163-166), as BGR bytes.
+// The winners paint ground green and obstacles red (segmenter.cpp:288-318);
+// the red channel is dilated with a 5x5 rectangle whose out-of-image part
+// never contributes (:484-489), which turns empty pixels near an obstacle
+// red and ground pixels near one yellow; yellow pixels with a point become
+// the blue JCP queue (:491-513) and take the JCP colour where the weights
+// resolve (:626-635).  A yellow pixel without a point (white, :510) cannot
+// occur: only a winner paints green.
+__global__ void k_label_image(DevBuf d, Layout L, int f, uint8_t* img) {
+    const FrameDesc& fd = d.frames[f];
+    const int H = fd.sc.H, W = fd.sc.W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= H * W) return;
+    const uint8_t* st = d.state + fd.pix_off;
+    const int qm = d.qmap[(size_t)f * L.P + p];
+    uint8_t b = 0, g = 0, r = 0;
+    const uint8_t s = qm >= 0 ? d.fin[(size_t)f * L.P + qm] : st[p];
+    if (s == kGround) {
+        g = 255;
+    } else if (s == kObstacle) {
+        r = 255;
+    } else if (s == kPending) {
+        b = 255;
+    } else {
+        const int row = p / W, col = p % W;
+        for (int dr = -2; dr <= 2 && !r; ++dr) {
+            const int rr = row + dr;
+            if (rr < 0 || rr >= H) continue;
+            for (int dc = -2; dc <= 2; ++dc) {
+                const int cc = col + dc;
+                if (cc >= 0 && cc < W && st[rr * W + cc] == kObstacle) {
+                    r = 255;
+                    break;
+                }
+            }
+        }
+    }
+    img[3 * p + 0] = b;
+    img[3 * p + 1] = g;
+    img[3 * p + 2] = r;
 }
 
 void launch_jcp(lp_context* c, int B) {
@@ -442,6 +485,10 @@ void launch_jcp(lp_context* c, int B) {
     hipLaunchKernelGGL(k_jcp_solve, dim3(B), dim3(kSolveThreads), 0, st, d, L);
     stage_mark(c, "labels");
     hipLaunchKernelGGL(k_seg_labels, dim3((L.N + 255) / 256, B), dim3(256), 0, st, d, L);
+}
+
+void launch_label_image(lp_context* c, int f, int pixels, uint8_t* img) {
+    hipLaunchKernelGGL(k_label_image, dim3((pixels + 255) / 256), dim3(256), 0, c->stream, c->d, c->L, f, img);
 }
 
 }  // namespace lp

@@ -39,6 +39,13 @@ void launch_noise(lp_context* c, int B, bool compact);
 // obstacle coordinates) or, for lists, of polygons staged by the caller.
 void launch_boxes(lp_context* c, int B, bool from_hulls);
 
+// Segmenter::image of frame f of the last run into img (H*W*3 bytes).
+void launch_label_image(lp_context* c, int f, int pixels, uint8_t* img);
+// Vehicle classes of the clusters of the last box stage.
+void launch_vehicles(lp_context* c, int B);
+// PCA boxes of polygons staged like lp_oriented_boxes's.
+void launch_pca_boxes(lp_context* c, int num_polygons);
+
 // Standalone hull entry (lp_convex_hulls): `num_lists` lists staged in the
 // round-0 hull buffers of frame 0 with offsets in cl_off.
 void launch_hull_lists(lp_context* c, int num_lists, int total_points);

@@ -20,6 +20,8 @@
 // the candidates in the reference's order (64 at a time, with a ballot for
 // those below the running minimum) and recomputes the chosen candidate's
 // extents in its own direction for the corners.
+#include <algorithm>
+
 #include "lp_device.h"
 #include "lp_internal.h"
 #include "lp_kernels.h"
@@ -334,7 +336,297 @@ __global__ void k_box_gather(DevBuf d, Layout L) {
     }
 }
 
+// Vehicle size classes of the processor's polygon simplification
+// (src/processor/src/processor.hpp:78-192): the five base dimension ranges
+// widened by the length / width / height tolerances, their volume and area
+// bounds, and the loosest height and volume bounds over all five.
+struct VehicleConst {
+    double dims[5][6];    // min/max length, width, height
+    double bounds[5][4];  // min/max volume, min/max area
+    double min_h, max_h, min_vol, max_vol;
+};
+
+VehicleConst vehicle_const() {
+    static const double base[5][6] = {{4.3, 4.6, 1.6, 1.9, 1.4, 1.5},
+                                      {4.6, 5.0, 1.6, 1.9, 1.4, 1.5},
+                                      {4.6, 5.2, 1.7, 2.1, 1.7, 1.8},
+                                      {5.2, 5.8, 1.9, 2.2, 1.8, 2.0},
+                                      {4.9, 5.2, 1.7, 2.1, 1.7, 1.8}};
+    const double tol[3] = {0.8, 0.5, 0.5};
+    VehicleConst v{};
+    v.min_h = 1.7976931348623157e308;
+    v.max_h = -1.7976931348623157e308;
+    v.min_vol = 1.7976931348623157e308;
+    v.max_vol = -1.7976931348623157e308;
+    for (int i = 0; i < 5; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            v.dims[i][2 * k] = base[i][2 * k] - tol[k];
+            v.dims[i][2 * k + 1] = base[i][2 * k + 1] + tol[k];
+        }
+        const double* a = v.dims[i];
+        v.bounds[i][0] = a[0] * a[2] * a[4];
+        v.bounds[i][1] = a[1] * a[3] * a[5];
+        v.bounds[i][2] = a[0] * a[2];
+        v.bounds[i][3] = a[1] * a[3];
+        if (a[4] < v.min_h) v.min_h = a[4];
+        if (a[5] > v.max_h) v.max_h = a[5];
+        if (v.bounds[i][0] < v.min_vol) v.min_vol = v.bounds[i][0];
+        if (v.bounds[i][1] > v.max_vol) v.max_vol = v.bounds[i][1];
+    }
+    return v;
+}
+
+__device__ __forceinline__ double dist_xy(double ax, double ay, double bx, double by) {
+    const double dx = ax - bx;
+    const double dy = ay - by;
+    return sqrt(dx * dx + dy * dy);
+}
+
+// Processor::run's polygon simplification test (processor.cpp:680-757), one
+// wavefront per cluster: the cluster's z range over its points, then on the
+// first lane the shoelace area of the hull in hull order (polygonArea,
+// polygonizer.hpp:185-198), the volume and box tests and the first vehicle
+// class whose strict bounds all hold.  The box is the cluster hull's
+// rotating-calipers box of the preceding stage.
+__global__ __launch_bounds__(256) void k_vehicles(DevBuf d, Layout L, VehicleConst vc) {
+    const int f = blockIdx.y;
+    const uint32_t* fs = d.fs + f * kFsCount;
+    const int K = min(static_cast<int>(fs[kFsClusters]), L.box_stride);
+    const size_t ob = (size_t)f * L.N;
+    const size_t BN = (size_t)L.B * L.N;
+    const int lane = lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    for (int c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < K; c += waves) {
+        const uint32_t size = d.cl_size[ob + c];
+        const uint32_t base = d.cl_off[ob + c];
+        float zlo = 3.402823466e38f, zhi = -3.402823466e38f;
+        for (uint32_t j = lane; j < size; j += kWave) {
+            const float z = d.oz[ob + d.cl_members[ob + base + j]];
+            zlo = z < zlo ? z : zlo;
+            zhi = zhi < z ? z : zhi;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float a = __shfl_xor(zlo, o, kWave), b = __shfl_xor(zhi, o, kWave);
+            zlo = a < zlo ? a : zlo;
+            zhi = zhi < b ? b : zhi;
+        }
+        if (lane == 0) {
+            int32_t cls = -1;
+            const int nv = static_cast<int>(d.hull_count[ob + c]);
+            const double height = static_cast<double>(zhi) - static_cast<double>(zlo);
+            if (nv >= 3 && size > 150u && height > vc.min_h && height < vc.max_h) {
+                const double* X = d.hcx + BN + ob + d.hull_off[ob + c];
+                const double* Y = d.hcy + BN + ob + d.hull_off[ob + c];
+                double area = 0.0;
+                for (int i = 0; i < nv - 1; ++i) area += X[i] * Y[i + 1] - X[i + 1] * Y[i];
+                area += X[nv - 1] * Y[0] - X[0] * Y[nv - 1];
+                area = fabs(area) * 0.5;
+                const double volume = area * height;
+                const lp_bounding_box& box = d.boxes[(size_t)f * L.box_stride + c];
+                if (volume > vc.min_vol && volume < vc.max_vol && box.is_valid &&
+                    area / static_cast<double>(box.area) > 0.4) {
+                    const double* q = box.corners;
+                    const double e1 = dist_xy(q[0], q[1], q[2], q[3]);
+                    const double e2 = dist_xy(q[2], q[3], q[4], q[5]);
+                    const double len = e1 < e2 ? e2 : e1;  // std::max
+                    const double wid = e2 < e1 ? e2 : e1;  // std::min
+                    for (int i = 0; i < 5; ++i) {
+                        const double* v = vc.dims[i];
+                        const double* b = vc.bounds[i];
+                        if (len > v[0] && len < v[1] && wid > v[2] && wid < v[3] && height > v[4] && height < v[5] &&
+                            volume > b[0] && volume < b[1] && area > b[2] && area < b[3]) {
+                            cls = i;
+                            break;
+                        }
+                    }
+                }
+            }
+            d.vclass[(size_t)f * L.box_stride + c] = cls;
+            if (cls >= 0) atomicAdd(&d.fs[f * kFsCount + kFsVehicles], 1u);
+        }
+    }
+}
+
+// One Jacobi rotation (c, s) of the 2x2 SVD sweep: the rotation that
+// diagonalises the symmetric 2x2 [[x, y], [y, z]] (Golub & Van Loan's
+// symmetric Schur step, with the sign choice t = 1 / (tau +- sqrt(tau^2 + 1))).
+__device__ __forceinline__ void jacobi_rot(double x, double y, double z, double& c, double& s) {
+    const double deno = 2.0 * fabs(y);
+    if (deno < 2.2250738585072014e-308) {
+        c = 1.0;
+        s = 0.0;
+        return;
+    }
+    const double tau = (x - z) / deno;
+    const double w = sqrt(tau * tau + 1.0);
+    const double t = tau > 0.0 ? 1.0 / (tau + w) : 1.0 / (tau - w);
+    const double sign_t = t > 0.0 ? 1.0 : -1.0;
+    const double n = 1.0 / sqrt(t * t + 1.0);
+    s = -sign_t * (y / fabs(y)) * fabs(t) * n;
+    c = n;
+}
+
+// Right singular vectors of the 2x2 covariance by two-sided Jacobi sweeps
+// (the real 2x2 step: a rotation making the block symmetric, then the
+// symmetric Schur rotation), largest singular value first.  v = [v00 v01;
+// v10 v11], columns are the vectors.  False for a non-finite matrix.
+__device__ bool svd2_v(const double (&a)[2][2], double (&v)[2][2]) {
+    double scale = fmax(fmax(fabs(a[0][0]), fabs(a[0][1])), fmax(fabs(a[1][0]), fabs(a[1][1])));
+    if (!isfinite(scale)) return false;
+    if (scale == 0.0) scale = 1.0;
+    double w[2][2] = {{a[0][0] / scale, a[0][1] / scale}, {a[1][0] / scale, a[1][1] / scale}};
+    v[0][0] = 1.0;
+    v[0][1] = 0.0;
+    v[1][0] = 0.0;
+    v[1][1] = 1.0;
+    const double precision = 2.0 * 2.220446049250313e-16;
+    double max_diag = fmax(fabs(w[0][0]), fabs(w[1][1]));
+    // p = 1, q = 0 throughout (the only off-diagonal pair of a 2x2).
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        const double thr = fmax(2.2250738585072014e-308, precision * max_diag);
+        if (!(fabs(w[1][0]) > thr || fabs(w[0][1]) > thr)) break;
+        // Block m = [[w11, w10], [w01, w00]] in (p, q) order.
+        const double m00 = w[1][1], m01 = w[1][0], m10 = w[0][1], m11 = w[0][0];
+        double c1 = 1.0, s1 = 0.0;
+        const double tt = m00 + m11, dd = m10 - m01;
+        if (!(fabs(dd) < 2.2250738585072014e-308)) {
+            const double u = tt / dd;
+            const double tmp = sqrt(1.0 + u * u);
+            s1 = 1.0 / tmp;
+            c1 = u / tmp;
+        }
+        // m <- rot1 applied on the left (rows 0, 1 of m).
+        double n00 = m00, n01 = m01, n10 = m10, n11 = m11;
+        if (!(c1 == 1.0 && s1 == 0.0)) {
+            n00 = c1 * m00 + s1 * m10;
+            n01 = c1 * m01 + s1 * m11;
+            n10 = -s1 * m00 + c1 * m10;
+            n11 = -s1 * m01 + c1 * m11;
+        }
+        double cr, sr;
+        jacobi_rot(n00, n01, n11, cr, sr);
+        // left = rot1 * right^T, right^T = (cr, -sr).
+        const double cl = c1 * cr - s1 * (-sr);
+        const double sl = c1 * (-sr) + s1 * cr;
+        // w.applyOnTheLeft(p=1, q=0, left): x = row 1, y = row 0.
+        if (!(cl == 1.0 && sl == 0.0)) {
+            for (int k = 0; k < 2; ++k) {
+                const double xi = w[1][k], yi = w[0][k];
+                w[1][k] = cl * xi + sl * yi;
+                w[0][k] = -sl * xi + cl * yi;
+            }
+        }
+        // w and v .applyOnTheRight(p=1, q=0, right): x = column 1, y = column 0,
+        // rotated by right^T = (cr, -sr).
+        if (!(cr == 1.0 && -sr == 0.0)) {
+            for (int k = 0; k < 2; ++k) {
+                const double xi = w[k][1], yi = w[k][0];
+                w[k][1] = cr * xi + (-sr) * yi;
+                w[k][0] = -(-sr) * xi + cr * yi;
+                const double vx = v[k][1], vy = v[k][0];
+                v[k][1] = cr * vx + (-sr) * vy;
+                v[k][0] = -(-sr) * vx + cr * vy;
+            }
+        }
+        max_diag = fmax(max_diag, fmax(fabs(w[1][1]), fabs(w[0][0])));
+    }
+    const double sv0 = fabs(w[0][0]) * scale, sv1 = fabs(w[1][1]) * scale;
+    if (sv0 != 0.0 || sv1 != 0.0) {
+        if (sv1 > sv0) {
+            const double t0 = v[0][0], t1 = v[1][0];
+            v[0][0] = v[0][1];
+            v[1][0] = v[1][1];
+            v[0][1] = t0;
+            v[1][1] = t1;
+        }
+    }
+    return true;
+}
+
+// Polygonizer::boundingBoxPrincipalComponentAnalysis (polygonizer.cpp:280-360),
+// one wavefront per list.  Every lane evaluates the mean and the covariance
+// in the same sequential order (so all lanes hold identical values with no
+// exchange), the lanes share the rotation of the centred points and the
+// extents, and lane 0 writes the box.
+__global__ __launch_bounds__(256) void k_pca_boxes(DevBuf d, Layout L, const double* VX, const double* VY,
+                                                  const uint32_t* off) {
+    const uint32_t* fs = d.fs;
+    const int K = static_cast<int>(fs[kFsClusters]);
+    const uint32_t total = fs[kFsHullTotal];
+    const int lane = lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    for (int k = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; k < K; k += waves) {
+        const uint32_t beg = off[k];
+        const uint32_t end = k + 1 < K ? off[k + 1] : total;
+        const int n = static_cast<int>(end - beg);
+        const double* X = VX + beg;
+        const double* Y = VY + beg;
+        lp_bounding_box box{};
+        double v[2][2] = {{1.0, 0.0}, {0.0, 1.0}};
+        bool ok = n >= 3;
+        double mx = 0.0, my = 0.0;
+        if (ok) {
+            double sx = X[0], sy = Y[0];
+            for (int i = 1; i < n; ++i) {
+                sx += X[i];
+                sy += Y[i];
+            }
+            mx = sx / static_cast<double>(n);
+            my = sy / static_cast<double>(n);
+            double c00 = 0.0, c01 = 0.0, c11 = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const double cx = X[i] - mx, cy = Y[i] - my;
+                c00 += cx * cx;
+                c01 += cx * cy;
+                c11 += cy * cy;
+            }
+            const double dn = static_cast<double>(static_cast<uint32_t>(n - 1));
+            const double a[2][2] = {{c00 / dn, c01 / dn}, {c01 / dn, c11 / dn}};
+            ok = svd2_v(a, v);
+        }
+        if (ok) {
+            Extent e = extent_init();
+            for (int i = lane; i < n; i += kWave) {
+                const double cx = X[i] - mx, cy = Y[i] - my;
+                extent_add(e, cx * v[0][0] + cy * v[1][0], cx * v[0][1] + cy * v[1][1]);
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const double a0 = __shfl_xor(e.mnx, o, kWave), a1 = __shfl_xor(e.mxx, o, kWave);
+                const double a2 = __shfl_xor(e.mny, o, kWave), a3 = __shfl_xor(e.mxy, o, kWave);
+                e.mnx = a0 < e.mnx ? a0 : e.mnx;
+                e.mxx = e.mxx < a1 ? a1 : e.mxx;
+                e.mny = a2 < e.mny ? a2 : e.mny;
+                e.mxy = e.mxy < a3 ? a3 : e.mxy;
+            }
+            const double cxs[4] = {e.mnx, e.mxx, e.mxx, e.mnx};
+            const double cys[4] = {e.mny, e.mny, e.mxy, e.mxy};
+            for (int j = 0; j < 4; ++j) {
+                box.corners[2 * j] = (cxs[j] * v[0][0] + cys[j] * v[0][1]) + mx;
+                box.corners[2 * j + 1] = (cxs[j] * v[1][0] + cys[j] * v[1][1]) + my;
+            }
+            box.area = static_cast<float>((e.mxx - e.mnx) * (e.mxy - e.mny));
+            box.angle_rad = static_cast<float>(atan2(v[1][0], v[0][0]));
+            box.is_valid = 1;
+        }
+        if (lane == 0) d.boxes[k] = box;
+    }
+}
+
 }  // namespace
+
+void launch_vehicles(lp_context* c, int B) {
+    stage_mark(c, "vehicles");
+    hipLaunchKernelGGL(k_vehicles, dim3(64, B), dim3(256), 0, c->stream, c->d, c->L, vehicle_const());
+}
+
+void launch_pca_boxes(lp_context* c, int num_polygons) {
+    const size_t BN = (size_t)c->L.B * c->L.N;
+    const int blocks = std::max(1, std::min(1024, (num_polygons + 3) / 4));
+    hipLaunchKernelGGL(k_pca_boxes, dim3(blocks), dim3(256), 0, c->stream, c->d, c->L, c->d.hcx + BN,
+                       c->d.hcy + BN, c->d.hull_off);
+}
 
 void launch_boxes(lp_context* c, int B, bool from_hulls) {
     DevBuf& d = c->d;

@@ -118,6 +118,8 @@ void lp_server_config_default(lp_server_config* cfg) {
     cfg->noise_filter = 0;
     lp_noise_config_default(&cfg->noise);
     cfg->oriented_boxes = 0;
+    cfg->label_clouds = 0;
+    cfg->vehicle_boxes = 0;
 }
 
 int lp_server_create(const lp_server_config* cfg, lp_server** out) {
@@ -144,6 +146,13 @@ int lp_server_create(const lp_server_config* cfg, lp_server** out) {
         s->slot_bytes = c->arena_bytes;
         if (cfg->noise_filter) lp_set_noise_filter(c, &cfg->noise);
         lp_set_oriented_boxes(c, cfg->oriented_boxes);
+        lp_set_vehicle_boxes(c, cfg->vehicle_boxes);
+        const int cl = lp_set_label_clouds(c, cfg->label_clouds);
+        if (cl != LP_OK) {
+            lp_context_destroy(c);
+            lp_server_destroy(s);
+            return cl;
+        }
         Slot sl;
         sl.ctx = c;
         s->slots.push_back(std::move(sl));

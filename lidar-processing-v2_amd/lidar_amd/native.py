@@ -29,7 +29,8 @@ EXPORTED = [
     "lp_math_eval", "lp_math_check_range", "lp_add_ring_info", "lp_query", "lp_noise_config_default",
     "lp_remove_noise", "lp_oriented_boxes", "lp_set_noise_filter", "lp_set_oriented_boxes",
     "lp_server_config_default", "lp_server_create", "lp_server_destroy", "lp_server_submit", "lp_server_poll",
-    "lp_server_stats_get", "lp_cloud_from_pointcloud2",
+    "lp_server_stats_get", "lp_cloud_from_pointcloud2", "lp_pca_boxes", "lp_set_label_clouds",
+    "lp_set_vehicle_boxes", "lp_frame_image",
 ]
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
@@ -152,6 +153,9 @@ class FrameOutputs(ctypes.Structure):
         ("hull_capacity", ctypes.c_int32),
         ("noise_labels", ctypes.c_void_p),
         ("boxes", ctypes.c_void_p),
+        ("ground_index", ctypes.c_void_p),
+        ("unknown_index", ctypes.c_void_p),
+        ("vehicle_class", ctypes.c_void_p),
     ]
 
 
@@ -168,6 +172,9 @@ class FrameResult(ctypes.Structure):
         ("num_noise", ctypes.c_int32),
         ("num_boxes", ctypes.c_int32),
         ("num_noise_searched", ctypes.c_int32),
+        ("num_ground", ctypes.c_int32),
+        ("num_unknown", ctypes.c_int32),
+        ("num_vehicles", ctypes.c_int32),
     ]
 
 
@@ -181,6 +188,8 @@ class ServerConfig(ctypes.Structure):
         ("noise_filter", ctypes.c_int32),
         ("noise", NoiseConfig),
         ("oriented_boxes", ctypes.c_int32),
+        ("label_clouds", ctypes.c_int32),
+        ("vehicle_boxes", ctypes.c_int32),
     ]
 
 
@@ -239,6 +248,10 @@ def lib():
         L.lp_oriented_boxes.argtypes = [P, P, P, P, I, ctypes.POINTER(BoundingBox)]
         L.lp_set_noise_filter.argtypes = [P, ctypes.POINTER(NoiseConfig)]
         L.lp_set_oriented_boxes.argtypes = [P, I]
+        L.lp_pca_boxes.argtypes = [P, P, P, P, I, ctypes.POINTER(BoundingBox)]
+        L.lp_set_label_clouds.argtypes = [P, I]
+        L.lp_set_vehicle_boxes.argtypes = [P, I]
+        L.lp_frame_image.argtypes = [P, I, P, I]
         L.lp_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
         L.lp_server_config_default.restype = None
         L.lp_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
@@ -373,6 +386,8 @@ class Context:
         self.h = h
         self.cap = cap
         self._boxes_on = False
+        self._clouds_on = False
+        self._vehicles_on = False
 
     def close(self):
         if getattr(self, "h", None):
@@ -447,6 +462,13 @@ class Context:
 
     def oriented_boxes(self, polygons):
         """boundingBoxRotatingCalipers on each (x, y) CCW polygon -> (status, [box dict])."""
+        return self._polygon_boxes(lib().lp_oriented_boxes, polygons)
+
+    def pca_boxes(self, polygons):
+        """boundingBoxPrincipalComponentAnalysis on each (x, y) list -> (status, [box dict])."""
+        return self._polygon_boxes(lib().lp_pca_boxes, polygons)
+
+    def _polygon_boxes(self, fn, polygons):
         offs = np.zeros(len(polygons) + 1, np.int32)
         for k, (px, _) in enumerate(polygons):
             offs[k + 1] = offs[k] + len(px)
@@ -455,7 +477,7 @@ class Context:
         ys = np.ascontiguousarray(np.concatenate([np.asarray(b, np.float64) for _, b in polygons]) if polygons
                                   else np.zeros(0), np.float64)
         boxes = (BoundingBox * max(len(polygons), 1))()
-        st = lib().lp_oriented_boxes(self.h, _p(xs), _p(ys), _p(offs), len(polygons), boxes)
+        st = fn(self.h, _p(xs), _p(ys), _p(offs), len(polygons), boxes)
         if st != OK:
             return st, None
         return st, [box_dict(boxes[k]) for k in range(len(polygons))]
@@ -468,6 +490,25 @@ class Context:
     def set_oriented_boxes(self, enable):
         check(lib().lp_set_oriented_boxes(self.h, 1 if enable else 0))
         self._boxes_on = bool(enable)
+
+    def set_label_clouds(self, enable):
+        """Ground and unknown index lists beside the obstacle split."""
+        check(lib().lp_set_label_clouds(self.h, 1 if enable else 0))
+        self._clouds_on = bool(enable)
+
+    def set_vehicle_boxes(self, enable):
+        """Vehicle class per cluster (boxes are computed for it)."""
+        check(lib().lp_set_vehicle_boxes(self.h, 1 if enable else 0))
+        self._vehicles_on = bool(enable)
+
+    def image(self, frame=0):
+        """Segmenter::image of a frame of the last run: uint8 [H, W, 3] BGR."""
+        need = lib().lp_frame_image(self.h, frame, ctypes.c_void_p(1), 0)
+        check(min(need, 0))
+        out = np.zeros(need, np.uint8)
+        got = lib().lp_frame_image(self.h, frame, _p(out), need)
+        check(min(got, 0))
+        return out
 
     # -- fused frames --------------------------------------------------------
     def _clouds(self, frames):
@@ -512,8 +553,13 @@ class Context:
             b = dict(seg=np.zeros(n, np.uint32), obs=np.zeros(max(n, 1), np.int32),
                      clab=np.zeros(max(n, 1), np.int32), hoff=np.zeros(n + 2, np.int32),
                      hidx=np.zeros(2 * n + 16, np.int32), noise=np.zeros(max(n, 1), np.uint8),
-                     boxes=(BoundingBox * (max(n, 1) if self._boxes_on else 1))())
+                     boxes=(BoundingBox * (max(n, 1) if self._boxes_on or self._vehicles_on else 1))(),
+                     gidx=np.zeros(max(n, 1), np.int32), uidx=np.zeros(max(n, 1), np.int32),
+                     vcls=np.zeros(max(n, 1), np.int32))
             bufs.append(b)
+            outs[i].ground_index = b["gidx"].ctypes.data
+            outs[i].unknown_index = b["uidx"].ctypes.data
+            outs[i].vehicle_class = b["vcls"].ctypes.data
             outs[i].noise_labels = b["noise"].ctypes.data
             outs[i].boxes = ctypes.addressof(b["boxes"])
             outs[i].seg_labels = b["seg"].ctypes.data
@@ -531,7 +577,11 @@ class Context:
             out.append(dict(status=r.status, seg_labels=b["seg"][:r.num_points], obstacle_index=b["obs"][:no],
                             num_points=r.num_points, num_noise=r.num_noise, noise_labels=b["noise"][:len(frames[i][0])],
                             num_noise_searched=r.num_noise_searched,
-                            boxes=[box_dict(b["boxes"][k]) for k in range(K)] if self._boxes_on else [],
+                            boxes=[box_dict(b["boxes"][k]) for k in range(K)]
+                            if self._boxes_on or self._vehicles_on else [],
+                            ground_index=b["gidx"][:r.num_ground], unknown_index=b["uidx"][:r.num_unknown],
+                            num_ground=r.num_ground, num_unknown=r.num_unknown, num_vehicles=r.num_vehicles,
+                            vehicle_class=b["vcls"][:K] if self._vehicles_on else None,
                             cluster_labels=b["clab"][:no], num_clusters=K, hull_offsets=b["hoff"][:K + 1],
                             hull_indices=b["hidx"][:V], num_voxels=r.num_voxels, jcp_queue=r.jcp_queue,
                             ransac=dict(candidates=r.ransac.candidates, best_inliers=r.ransac.best_inliers,

@@ -430,6 +430,12 @@ struct Seg {
                 dbg->state_recm[p] = image[p] == kGround ? 1 : image[p] == kObstacle ? 2 : 0;
         jcp(X, Y, Z, dbg);
         populate(labels);
+        if (dbg && dbg->image)
+            for (int32_t p = 0; p < P; ++p) {
+                dbg->image[3 * p + 0] = image[p].b;
+                dbg->image[3 * p + 1] = image[p].g;
+                dbg->image[3 * p + 2] = image[p].r;
+            }
         if (dbg && dbg->state_final)
             for (int32_t p = 0; p < P; ++p) {
                 if (mapping[p] == -1)

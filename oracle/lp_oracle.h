@@ -16,6 +16,8 @@
  *   src/dataloader/src/dataloader.cpp:68-137           (addRingInfo)
  *   lidar_processing_lib/src/noise_remover.cpp:38-68 + kdtree.hpp:131-397 (NoiseRemover, DROR)
  *   lidar_processing_lib/src/polygonizer.cpp:93-278    (antipodal pairs, rotating-calipers box)
+ *   lidar_processing_lib/src/polygonizer.cpp:280-362   (PCA box)
+ *   src/processor/src/processor.cpp:672-760 + processor.hpp:78-192 (vehicle test)
  * and mirrors the reference's data-structure choices (vector-of-vector polar
  * grid, per-cell std::sort, FIFO JCP queue, linear-probing voxel map, BFS with
  * a visited set, std::sort hulls) so that its speed is representative.
@@ -73,6 +75,7 @@ typedef struct lpo_seg_debug {
     int32_t ransac_applied;     /* best_inliers > 0 */
     float ransac_plane[4];      /* a,b,c,d after the sign flip */
     uint32_t ransac_draws[180]; /* p2,p3 indices of the 60 hypotheses (+ unused) */
+    uint8_t* image;             /* [H*W*3] BGR image_ after segment() (Segmenter::image), or NULL */
 } lpo_seg_debug;
 
 enum {
@@ -169,6 +172,22 @@ int lpo_full_frame(lpo_processor* p, lpo_noise_remover* nr, const lpo_noise_conf
                    int32_t* kept, uint32_t* seg_labels, int32_t* obstacle_count, int32_t* cluster_labels,
                    int32_t* num_clusters, int32_t* hull_offsets, int32_t hull_offsets_cap, int32_t* hull_indices,
                    int32_t hull_cap, lpo_bounding_box* boxes);
+
+/* boundingBoxPrincipalComponentAnalysis over a point list. */
+int lpo_pca_box(const double* x, const double* y, int32_t n, lpo_bounding_box* out);
+/* Right singular vectors of a 2x2 matrix as the PCA box computes them,
+ * v = {v00, v01, v10, v11} (columns are the vectors, largest value first).
+ * Returns 0, or -1 for a non-finite matrix. */
+int lpo_svd2_v(const double* a, double* v);
+/* Vehicle size tables of processor.hpp:78-192: dims[5*6] (min/max length,
+ * width, height after the tolerances), bounds[5*4] (min/max volume, min/max
+ * area), loose[4] (min/max height, min/max volume over all types). */
+void lpo_vehicle_tables(double* dims, double* bounds, double* loose);
+/* The polygon simplification test of processor.cpp:680-757 for one cluster:
+ * hull vertices (hull order), the cluster's point count and z range, and the
+ * cluster's box.  Returns the vehicle type 0..4 or -1. */
+int32_t lpo_vehicle_class(const double* hx, const double* hy, int32_t nv, int32_t cluster_size, float z_min,
+                          float z_max, const lpo_bounding_box* box);
 
 /* Library math helpers exposed for pinning tests. */
 float lpo_atan2_approx(float y, float x);

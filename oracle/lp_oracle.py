@@ -63,6 +63,7 @@ class SegDebug(ctypes.Structure):
         ("ransac_applied", ctypes.c_int32),
         ("ransac_plane", ctypes.c_float * 4),
         ("ransac_draws", ctypes.c_uint32 * 180),
+        ("image", ctypes.c_void_p),
     ]
 
 
@@ -133,9 +134,9 @@ def segment(x, y, z, ring=None, cfg=None, debug=False):
             np.float32(2 * np.pi) / (np.float32(cfg.grid_slice_resolution_deg) * np.float32(np.pi / 180)))
         out = dict(elevation_map=np.zeros(rings * slices, np.float32), mapping=np.zeros(H * W, np.int32),
                    state_recm=np.zeros(H * W, np.uint8), state_final=np.zeros(H * W, np.uint8),
-                   queue=np.zeros(H * W, np.int32))
+                   queue=np.zeros(H * W, np.int32), image=np.zeros(H * W * 3, np.uint8))
         dbg = SegDebug()
-        for k in ("elevation_map", "mapping", "state_recm", "state_final", "queue"):
+        for k in ("elevation_map", "mapping", "state_recm", "state_final", "queue", "image"):
             setattr(dbg, k, out[k].ctypes.data)
     st = lib().lpo_segment(ctypes.byref(cfg), _p(x), _p(y), _p(z), _p(ring), n, _p(labels),
                            ctypes.byref(dbg) if dbg is not None else None)
@@ -381,6 +382,67 @@ def obb_rotating_calipers(x, y):
         raise ValueError("rotating calipers failed")
     return dict(corners=np.array(b.corners[:], np.float64).reshape(4, 2), area=np.float32(b.area),
                 angle_rad=np.float32(b.angle_rad), is_valid=bool(b.is_valid))
+
+
+def _bind_boxes():
+    L = _bind_ext()
+    if not getattr(L, "_boxes_bound", False):
+        P = ctypes.c_void_p
+        L.lpo_pca_box.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(BoundingBox)]
+        L.lpo_svd2_v.argtypes = [P, P]
+        L.lpo_vehicle_tables.argtypes = [P, P, P]
+        L.lpo_vehicle_tables.restype = None
+        L.lpo_vehicle_class.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                        ctypes.POINTER(BoundingBox)]
+        L._boxes_bound = True
+    return L
+
+
+def _box_dict(b):
+    return dict(corners=np.array(b.corners[:], np.float64).reshape(4, 2), area=np.float32(b.area),
+                angle_rad=np.float32(b.angle_rad), is_valid=bool(b.is_valid))
+
+
+def pca_box(x, y):
+    """boundingBoxPrincipalComponentAnalysis over a point list.  Returns dict."""
+    L = _bind_boxes()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    b = BoundingBox()
+    L.lpo_pca_box(_p(x), _p(y), len(x), ctypes.byref(b))
+    return _box_dict(b)
+
+
+def svd2_v(a):
+    """Right singular vectors (columns, largest first) of a 2x2 matrix."""
+    L = _bind_boxes()
+    a = np.ascontiguousarray(a, np.float64).reshape(4)
+    v = np.zeros(4, np.float64)
+    if L.lpo_svd2_v(_p(a), _p(v)) != OK:
+        raise ValueError("non-finite matrix")
+    return v.reshape(2, 2)
+
+
+def vehicle_tables():
+    """(dims[5,6], bounds[5,4], loose[4]) of processor.hpp:78-192."""
+    L = _bind_boxes()
+    dims, bounds, loose = np.zeros(30), np.zeros(20), np.zeros(4)
+    L.lpo_vehicle_tables(_p(dims), _p(bounds), _p(loose))
+    return dims.reshape(5, 6), bounds.reshape(5, 4), loose
+
+
+def vehicle_class(hx, hy, cluster_size, z_min, z_max, box):
+    """The processor's vehicle test for one cluster (box: a dict like pca_box's)."""
+    L = _bind_boxes()
+    hx = np.ascontiguousarray(hx, np.float64)
+    hy = np.ascontiguousarray(hy, np.float64)
+    b = BoundingBox()
+    b.corners[:] = [float(v) for v in np.asarray(box["corners"], np.float64).reshape(8)]
+    b.area = float(box["area"])
+    b.angle_rad = float(box["angle_rad"])
+    b.is_valid = 1 if box["is_valid"] else 0
+    return int(L.lpo_vehicle_class(_p(hx), _p(hy), len(hx), int(cluster_size), float(z_min), float(z_max),
+                                   ctypes.byref(b)))
 
 
 def _bind_full():

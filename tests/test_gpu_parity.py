@@ -332,3 +332,13 @@ def test_cpp_dropin_processor(tmp_path, oracle):
     dst2 = tmp_path / "out_msg.bin"
     subprocess.run([exe, str(src), str(dst2), "msg"], check=True, timeout=120)
     assert open(dst, "rb").read() == open(dst2, "rb").read()
+    # Segmenter::image() after the same segment call: the oracle's image_.
+    dst3 = tmp_path / "out_image.bin"
+    subprocess.run([exe, str(src), str(dst3), "image"], check=True, timeout=120)
+    body = open(dst, "rb").read()
+    tail = open(dst3, "rb").read()
+    assert tail[:len(body)] == body
+    rows, cols = np.frombuffer(tail[len(body):len(body) + 8], np.int32)
+    assert (rows, cols) == (f.height, f.width)
+    _, _, dbg = oracle.segment(f.x, f.y, f.z, f.ring, debug=True)
+    assert np.array_equal(np.frombuffer(tail[len(body) + 8:], np.uint8), dbg["image"])
