@@ -193,19 +193,59 @@ __global__ void k_jcp_slots(DevBuf d, Layout L) {
     d.qsum_ok[(size_t)f * L.P + j] = ok ? 1 : 0;
 }
 
-// 9-entry transfer map over pair states ps = a*3 + b, 4 bits per entry.
-__device__ __forceinline__ uint64_t map_compose(uint64_t fm, uint64_t gm) {  // f after g
-    uint64_t r = 0;
-#pragma unroll
-    for (int ps = 0; ps < 9; ++ps) {
-        const uint32_t gi = static_cast<uint32_t>(gm >> (4 * ps)) & 0xf;
-        const uint32_t fi = static_cast<uint32_t>(fm >> (4 * gi)) & 0xf;
-        r |= static_cast<uint64_t>(fi) << (4 * ps);
-    }
-    return r;
+// Transfer map over the nine pair states ps = a*3 + b: entry ps (0..8) is
+// byte ps of (lo, hi, x), i.e. entries 0-3, 4-7 and 8 (the upper bytes of x
+// are don't-care values in 0..8).
+struct TMap {
+    uint32_t lo, hi, x;
+};
+constexpr uint32_t kIdLo = 0x03020100u, kIdHi = 0x07060504u, kIdX = 0x08u;
+
+// Entries f[sel_k] for the four selector bytes sel_k (each 0..8) of sel:
+// the byte permute takes entries 0..7 from (hi:lo), and the bytes whose
+// selector is 8 (bit 3 set) are replaced by entry 8 (f8, in every byte).
+// The byte mask is a permute too: selector 12 gives 0x00 and 13 gives 0xff.
+__device__ __forceinline__ uint32_t tmap_pick(const TMap& f, uint32_t f8, uint32_t sel) {
+    const uint32_t v = __builtin_amdgcn_perm(f.hi, f.lo, sel);
+    const uint32_t m = __builtin_amdgcn_perm(0u, 0u, ((sel >> 3) & 0x01010101u) | 0x0c0c0c0cu);
+    return (f8 & m) | (v & ~m);
 }
 
-constexpr uint64_t kIdentityMap = 0x876543210ull;
+// f after g.
+__device__ __forceinline__ TMap tmap_compose(const TMap& f, const TMap& g) {
+    const uint32_t f8 = __builtin_amdgcn_perm(0u, f.x, 0u);
+    return TMap{tmap_pick(f, f8, g.lo), tmap_pick(f, f8, g.hi), tmap_pick(f, f8, g.x)};
+}
+
+__device__ __forceinline__ uint32_t tmap_get(const TMap& m, uint32_t ps) {
+    const uint32_t w = ps < 4 ? m.lo : (ps < 8 ? m.hi : m.x);
+    return (w >> (8 * (ps & 3))) & 0xffu;
+}
+
+// The map of the lane `kCtrl` names (DPP row shift or row broadcast); lanes
+// outside kRowMask or without a source lane read the identity.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ TMap tmap_dpp(const TMap& m) {
+    return TMap{static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(kIdLo), static_cast<int>(m.lo),
+                                                                  kCtrl, kRowMask, 0xf, false)),
+                static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(kIdHi), static_cast<int>(m.hi),
+                                                                  kCtrl, kRowMask, 0xf, false)),
+                static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(kIdX), static_cast<int>(m.x),
+                                                                  kCtrl, kRowMask, 0xf, false))};
+}
+
+// Inclusive scan of the wavefront's maps, m := m_lane o ... o m_0: row
+// shifts by 1, 2, 4 and 8 inside each 16-lane row, then lane 15 of rows 0 / 2
+// into rows 1 / 3 and lane 31 into rows 2 and 3.
+__device__ __forceinline__ TMap tmap_wave_scan(TMap m) {
+    m = tmap_compose(m, tmap_dpp<0x111, 0xf>(m));
+    m = tmap_compose(m, tmap_dpp<0x112, 0xf>(m));
+    m = tmap_compose(m, tmap_dpp<0x114, 0xf>(m));
+    m = tmap_compose(m, tmap_dpp<0x118, 0xf>(m));
+    m = tmap_compose(m, tmap_dpp<0x142, 0xa>(m));
+    m = tmap_compose(m, tmap_dpp<0x143, 0xc>(m));
+    return m;
+}
 constexpr int kSolveThreads = 256;
 constexpr int kSolveWaves = kSolveThreads / kWave;
 constexpr int kFinLds = 65536;
@@ -246,7 +286,7 @@ __device__ __forceinline__ void load_slots(const DevBuf& d, size_t qo, int j, Sl
 
 // Transfer map of queue pixel j (pair state (state(j-2), state(j-1)) ->
 // (state(j-1), state(j))).  Labels of earlier rows come from `fin`.
-__device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs, const uint8_t* fin) {
+__device__ __forceinline__ TMap pixel_map(const SlotData& sd, int j, int rs, const uint8_t* fin) {
     const bool ok = sd.ok != 0;
     float wv[24];
     int32_t src[24];
@@ -282,10 +322,10 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
     const int sel10 = src[10] < rs ? 0 : (src[10] == j - 2 ? 1 : (src[10] == j - 1 ? 2 : 0));  // 1: j-2, 2: j-1
     const bool sel11 = src[11] >= rs && src[11] == j - 1;
     if (!ok) {
-        uint64_t m = 0;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) m |= static_cast<uint64_t>((q % 3) * 3 + kSU) << (4 * q);
-        return m;
+        // Every pair state (a, b) goes to (b, UNKNOWN).
+        return TMap{(0u * 3 + kSU) | ((1u * 3 + kSU) << 8) | ((2u * 3 + kSU) << 16) | ((0u * 3 + kSU) << 24),
+                    (1u * 3 + kSU) | ((2u * 3 + kSU) << 8) | ((0u * 3 + kSU) << 16) | ((1u * 3 + kSU) << 24),
+                    2u * 3 + kSU};
     }
     // Ordered sums of segmenter.cpp:613-626: slots 0..9 are fixed, 10 and 11
     // may depend on the row's previous states, 12..23 are fixed again.  The
@@ -322,7 +362,7 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
         tg[q] = g;
         to[q] = o;
     }
-    uint64_t m = 0;
+    uint32_t e[3] = {0u, 0u, 0u};
 #pragma unroll
     for (int ps = 0; ps < 9; ++ps) {
         const uint32_t a = ps / 3, b = ps % 3;
@@ -333,18 +373,18 @@ __device__ __forceinline__ uint64_t pixel_map(const SlotData& sd, int j, int rs,
         const float g = l11 == kSG ? g23 : g01;
         const float o = l11 == kSO ? o23 : o01;
         const uint32_t t = (o > g) ? kSO : kSG;
-        m |= static_cast<uint64_t>(b * 3 + t) << (4 * ps);
+        e[ps / 4] |= (b * 3 + t) << (8 * (ps % 4));
     }
-    return m;
+    return TMap{e[0], e[1], e[2]};
 }
 
 // Row-ordered exact solve, one workgroup per frame: per row, transfer maps
-// of its queue pixels, a wavefront shuffle scan of the maps, one LDS step
-// across the four wavefronts.
+// of its queue pixels, a wavefront DPP scan of the maps, one LDS step across
+// the four wavefronts.
 __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L) {
     __shared__ uint8_t fin[kFinBase + kFinLds];
     __shared__ int row_start[1025];
-    __shared__ uint64_t wtot[kSolveWaves];
+    __shared__ TMap wtot[kSolveWaves];
     const int f = blockIdx.x;
     const FrameDesc& fd = d.frames[f];
     const SegConst& s = fd.sc;
@@ -377,23 +417,22 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
             const int j = base + tid;
             const int jn = ((base + kSolveThreads < re) ? base + kSolveThreads : re) + tid;
             load_slots(d, qo, min(jn, nq - 1), nxt);
-            uint64_t m = j < re ? pixel_map(cur, j, rs, fin) : kIdentityMap;
-            // Inclusive scan inside the wavefront: m := m_j o ... o m_first.
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint64_t prev = __shfl_up(m, o, kWave);
-                if (lane >= o) m = map_compose(m, prev);
-            }
+            TMap m = j < re ? pixel_map(cur, j, rs, fin) : TMap{kIdLo, kIdHi, kIdX};
+            m = tmap_wave_scan(m);  // m := m_j o ... o m_(first of this wavefront)
             if (lane == kWave - 1) wtot[w] = m;
             __syncthreads();
-            uint64_t pre = kIdentityMap;
-            for (int q = 0; q < w; ++q) pre = map_compose(wtot[q], pre);
-            m = map_compose(m, pre);
-            const uint32_t ps = static_cast<uint32_t>(m >> (4 * carry)) & 0xf;
+            // Prefix of the earlier wavefronts (pre) and of all four (chain).
+            TMap chain = wtot[0];
+            TMap pre = TMap{kIdLo, kIdHi, kIdX};
+#pragma unroll
+            for (int q = 1; q < kSolveWaves; ++q) {
+                if (q == w) pre = chain;
+                chain = tmap_compose(wtot[q], chain);
+            }
+            if (w > 0) m = tmap_compose(m, pre);
+            const uint32_t ps = tmap_get(m, carry);
             if (j < re) fin[kFinBase + j] = static_cast<uint8_t>(ps % 3);
-            uint64_t all = kIdentityMap;
-            for (int q = 0; q < kSolveWaves; ++q) all = map_compose(wtot[q], all);
-            carry = static_cast<uint32_t>(all >> (4 * carry)) & 0xf;
+            carry = tmap_get(chain, carry);
             cur = nxt;  // the prefetch has had the whole chunk to land
             __syncthreads();
         }

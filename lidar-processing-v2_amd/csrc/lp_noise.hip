@@ -7,6 +7,13 @@
 //   dist_sqr = (t0-p0)^2 + ((t1-p1)^2 + ((t2-p2)^2 + 0))   (kdtree.hpp:131-143)
 // all in float except the widened product.
 //
+// A first pass, one thread per point in input order, counts the points
+// within the radius among its 8 input-order neighbours on each side: scan
+// order keeps a point's azimuth neighbours of the same beam next to it, so
+// this decides almost every VALID point of a surface with 17 coalesced
+// distance tests (any min_neighbours points within the radius prove VALID).
+// The rest go through the grid.
+//
 // Layout: the frame's points are bucketed into 1 m grid cells through an
 // open-addressing hash of cell keys (one slot per occupied cell, load <= 0.5),
 // counted, scanned and scattered into a cell-sorted float4 array (w holds the
@@ -90,6 +97,44 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
         d.fs[f * kFsCount + kFsNoiseHard] = 0;
         d.fs[f * kFsCount + kFsNoiseCells] = 0;
     }
+}
+
+constexpr int kNearWindow = 8;  // input-order neighbours tested on each side
+
+// Input-order pass: noise[i] = 0 (VALID) when at least min_neighbours of the
+// points i - 8 .. i + 8 (i itself included) lie within the radius, else 2
+// (undecided, labelled by the grid passes).
+__global__ __launch_bounds__(256) void k_noise_near(DevBuf d, Layout L) {
+    __shared__ float4 tile_pts[256 + 2 * kNearWindow];
+    int tile, f;
+    xcd_tile_frame(tile, f);
+    const FrameDesc& fd = d.frames[f];
+    const int n = fd.n_raw;
+    const int base = tile * blockDim.x;
+    if (base >= n) return;
+    for (int q = threadIdx.x; q < 256 + 2 * kNearWindow; q += blockDim.x) {
+        const int j = min(max(base - kNearWindow + q, 0), n - 1);
+        const int gj = fd.pt_off + j;
+        tile_pts[q] = make_float4(d.rx[gj], d.ry[gj], d.rz[gj], 0.0f);
+    }
+    __syncthreads();
+    const int i = base + threadIdx.x;
+    if (i >= n) return;
+    const NoiseConst nc = *d.nc;
+    const float4 t = tile_pts[threadIdx.x + kNearWindow];
+    const float r2 = noise_radius_sqr(nc, t.x, t.y);
+    const int lo = max(i - kNearWindow, 0), hi = min(i + kNearWindow, n - 1);
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int o = -kNearWindow; o <= kNearWindow; ++o) {
+        const int j = i + o;
+        const bool in = j >= lo && j <= hi;
+        cnt += (in && noise_dist(t, tile_pts[threadIdx.x + kNearWindow + o]) <= r2) ? 1u : 0u;
+    }
+    const bool valid = cnt >= nc.k;
+    const size_t po = (size_t)f * L.N;
+    d.noise[po + i] = valid ? 0 : 2;
+    if (valid) d.obs_flag[po + i] = 1u;
 }
 
 __global__ void k_noise_insert(DevBuf d, Layout L) {
@@ -186,8 +231,9 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
     const NoiseConst nc = *d.nc;
     const size_t po = (size_t)f * L.N;
     const size_t ho = (size_t)f * L.hash_cap;
-    const float4 t = d.dsort[po + pos];
     const int i = d.didx[po + pos];
+    if (d.noise[po + i] == 0) return;  // decided by the input-order pass
+    const float4 t = d.dsort[po + pos];
     const float r2 = noise_radius_sqr(nc, t.x, t.y);
     const uint32_t k = nc.k;
     int state;  // 0 VALID, 1 NOISE, 2 listed
@@ -380,6 +426,7 @@ void launch_noise(lp_context* c, int B, bool compact) {
     const int nb = (L.N + 255) / 256;
     stage_mark(c, "noise");
     hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_noise_near, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_noise_insert, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan_gather(c, B, d.hcount, L.hash_cap, d.ncell_slot, d.ncell_start, L.N, kCountSlot, kFsNoiseCells, -1);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
