@@ -10,13 +10,12 @@
 // minimum is the box's float `area` member, so near-ties are decided against
 // the float-rounded value exactly as the reference does.
 //
-// One workgroup per polygon.  The first wavefront walks the antipodal pairs
-// (the outer step is sequential; each inner run of j advances is tested on
-// 64 lanes at once) while the second sums the centroid in the reference's
-// order.  The candidates (i, -1) and (i - 1, +1) share one edge with opposite
-// directions, which negates every rotated coordinate exactly and leaves the
-// area bit-identical, so the workgroup evaluates one area per edge, O(n)
-// each; the first wavefront then applies the sequential selection rule over
+// One workgroup per polygon.  One lane walks the antipodal pairs while
+// another sums the centroid in the reference's order.  The candidates
+// (i, -1) and (i - 1, +1) share one edge with opposite directions, which
+// negates every rotated coordinate exactly and leaves the area
+// bit-identical, so the workgroup evaluates one area per edge, O(n) each;
+// the first wavefront then applies the sequential selection rule over
 // the candidates in the reference's order (64 at a time, with a ballot for
 // those below the running minimum) and recomputes the chosen candidate's
 // extents in its own direction for the corners.
@@ -33,8 +32,12 @@ using namespace lpd;
 namespace {
 
 constexpr int kBoxThreads = 256;
-constexpr int kBoxVerts = 1024;       // polygon vertices held in LDS (larger: capacity error)
-constexpr int kBoxPairs = 2 * kBoxVerts;  // antipodal pairs (at most 3n/2)
+// Polygons of up to kBoxVerts vertices are held in LDS (larger: capacity
+// error).  The main launch takes those of up to kBoxSmall vertices with half
+// of the LDS, so more than twice as many workgroups fit a CU; a second
+// launch takes the larger ones.  Antipodal pairs: at most 3n/2, room for 2n.
+constexpr int kBoxVerts = 1024;
+constexpr int kBoxSmall = 512;
 
 // areaOfTriangle (polygonizer.hpp:228-231), same operand order.
 __device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, double by, double cx, double cy) {
@@ -43,74 +46,65 @@ __device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, d
 
 __device__ __forceinline__ int wrap(int k, int n) { return k >= n ? k - n : k; }
 
-// Number of consecutive steps j -> j+1 -> ... the reference's loop
-// `while (area(i, i+1, j+1) > area(i, i+1, j)) ++j` takes from j, counted
-// up to `cap` (one wavefront tests 64 steps at a time).
-__device__ int advance_run(const double* X, const double* Y, int n, int i, int j, int cap) {
-    const int lane = lane_id();
-    const int i1 = wrap(i + 1, n);
-    const double xi = X[i], yi = Y[i], xi1 = X[i1], yi1 = Y[i1];
-    int t = 0;
-    while (t < cap) {
-        const int s = t + lane;
-        const int a = (j + s) % n, b = (j + s + 1) % n;
-        const bool go = tri_area_xy(xi, yi, xi1, yi1, X[b], Y[b]) > tri_area_xy(xi, yi, xi1, yi1, X[a], Y[a]);
-        const uint64_t stop = __ballot(!go);
-        if (stop) return min(cap, t + __ffsll(static_cast<unsigned long long>(stop)) - 1);
-        t += kWave;
-    }
-    return cap;
-}
-
 // Antipodal pairs of polygon (X, Y, n >= 3) in the reference's order
-// (polygonizer.cpp:93-161), by one wavefront; returns the count, or -1 when
-// the walk exceeds 4n + 8 steps or kBoxPairs.
-__device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pairs) {
-    const int lane = lane_id();
+// (polygonizer.cpp:93-161), walked by one lane: the areas of the running
+// pair are carried from step to step and the next vertex is loaded while the
+// current comparison runs.  Returns the count, or -1 when the walk exceeds
+// 4n + 8 steps or `cap` pairs (the reference's loop need not end on
+// degenerate input).
+__device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pairs, int cap) {
     const int i0 = n - 1;
     const int limit = 4 * n + 8;
     int guard = 0, count = 0;
     int i = 0, j = 1;
-    {
-        // Initial advance: every step counts against the guard.
-        const int T = advance_run(X, Y, n, i, j, limit - guard + 1);
-        if (guard + T > limit) return -1;
-        guard += T;
-        j = (j + T) % n;
+    double xi = X[0], yi = Y[0], xi1 = X[1], yi1 = Y[1];
+    double xj = X[1], yj = Y[1];
+    int jn = wrap(2, n);
+    double xn = X[jn], yn = Y[jn];
+    double a_cur = tri_area_xy(xi, yi, xi1, yi1, xj, yj);
+    // while (area(i, i+1, j+1) > area(i, i+1, j)) ++j
+    for (;;) {
+        const double a_nx = tri_area_xy(xi, yi, xi1, yi1, xn, yn);
+        if (!(a_nx > a_cur)) break;
+        j = jn;
+        xj = xn;
+        yj = yn;
+        a_cur = a_nx;
+        jn = wrap(j + 1, n);
+        xn = X[jn];
+        yn = Y[jn];
+        if (++guard > limit) return -1;
     }
     const int j0 = j;
     while (i != j0) {
         i = wrap(i + 1, n);
-        if (count >= kBoxPairs) return -1;
-        if (lane == 0) pairs[count] = make_int2(i, j);
-        ++count;
-        // Inner advance, step t = 1..T: j + t; stop (returning the pairs so
-        // far) when i == j0 and j + t == i0, fail past the guard.
-        const int T = advance_run(X, Y, n, i, j, limit - guard + 1);
-        int t_term = T + 1;
-        if (i == j0) {
-            const int d = ((i0 - j) % n + n) % n;  // first step t >= 1 with j + t == i0 (mod n)
-            const int t_first = d > 0 ? d : n;
-            if (t_first <= T) t_term = t_first;
+        xi = xi1;
+        yi = yi1;
+        const int i1 = wrap(i + 1, n);
+        xi1 = X[i1];
+        yi1 = Y[i1];
+        if (count >= cap) return -1;
+        pairs[count++] = make_int2(i, j);
+        a_cur = tri_area_xy(xi, yi, xi1, yi1, xj, yj);
+        for (;;) {
+            const double a_nx = tri_area_xy(xi, yi, xi1, yi1, xn, yn);
+            if (!(a_nx > a_cur)) break;
+            j = jn;
+            xj = xn;
+            yj = yn;
+            a_cur = a_nx;
+            jn = wrap(j + 1, n);
+            xn = X[jn];
+            yn = Y[jn];
+            if (i == j0 && j == i0) return count;
+            if (count >= cap) return -1;
+            pairs[count++] = make_int2(i, j);
+            if (++guard > limit) return -1;
         }
-        const int t_guard = limit - guard + 1;  // first step that trips the guard
-        if (t_term <= T && t_term <= t_guard) {
-            if (count + t_term - 1 > kBoxPairs) return -1;
-            for (int t = 1 + lane; t < t_term; t += kWave) pairs[count + t - 1] = make_int2(i, (j + t) % n);
-            return count + t_term - 1;
-        }
-        if (t_guard <= T) return -1;
-        if (count + T > kBoxPairs) return -1;
-        for (int t = 1 + lane; t <= T; t += kWave) pairs[count + t - 1] = make_int2(i, (j + t) % n);
-        count += T;
-        guard += T;
-        j = (j + T) % n;
         // areaOfTriangle(j, i+1, j+1) == areaOfTriangle(i, i+1, j)
-        const int i1 = wrap(i + 1, n), j1 = wrap(j + 1, n);
-        if (tri_area_xy(X[j], Y[j], X[i1], Y[i1], X[j1], Y[j1]) == tri_area_xy(X[i], Y[i], X[i1], Y[i1], X[j], Y[j])) {
-            if (count >= kBoxPairs) return -1;
-            if (lane == 0) pairs[count] = (i == j0 && j == i0) ? make_int2(i1, j) : make_int2(i, j1);
-            ++count;
+        if (tri_area_xy(xj, yj, xi1, yi1, xn, yn) == a_cur) {
+            if (count >= cap) return -1;
+            pairs[count++] = (i == j0 && j == i0) ? make_int2(i1, j) : make_int2(i, jn);
         }
         if (++guard > limit) return -1;
     }
@@ -155,12 +149,16 @@ __device__ __forceinline__ bool cand_dir(const double* X, const double* Y, int n
 
 // Polygon k of frame f: vertices VX/VY[ob + off[k] .. off[k+1]) with the
 // offsets' end given by `total`.
+// This instantiation boxes the polygons of kMin < n <= kVerts vertices (and,
+// with kMin = 0, the degenerate ones); kVerts = kBoxVerts also flags larger
+// polygons as a capacity error.
+template <int kMin, int kVerts>
 __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const double* VX, const double* VY,
                                                       const uint32_t* off) {
-    __shared__ double X[kBoxVerts];
-    __shared__ double Y[kBoxVerts];
-    __shared__ double area[kBoxVerts];  // per edge e (e -> e+1); NaN when too short
-    __shared__ int2 pairs[kBoxPairs];
+    __shared__ double X[kVerts];
+    __shared__ double Y[kVerts];
+    __shared__ double area[kVerts];  // per edge e (e -> e+1); NaN when too short
+    __shared__ int2 pairs[2 * kVerts];
     __shared__ int np_s;
     __shared__ double cen_s[2];
     int tile, f;
@@ -180,13 +178,14 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
         const uint32_t v0 = off[ob + k];
         const uint32_t v1 = k + 1 < K ? off[ob + k + 1] : total;
         const int n = static_cast<int>(v1 - v0);
+        if ((kMin > 0 && n <= kMin) || (n > kVerts && kVerts < kBoxVerts)) continue;  // the other launch's
         lp_bounding_box box;
         for (int q = 0; q < 8; ++q) box.corners[q] = 0.0;
         box.area = 0.0f;
         box.angle_rad = 0.0f;
         box.is_valid = 0;
         box.reserved = 0;
-        if (n > kBoxVerts) {
+        if (n > kVerts) {
             if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
         } else if (n >= 3) {
             for (int q = tid; q < n; q += kBoxThreads) {
@@ -194,9 +193,8 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
                 Y[q] = VY[ob + v0 + q];
             }
             __syncthreads();
-            if (w == 0) {
-                const int np = antipodal_walk(X, Y, n, pairs);
-                if (lane == 0) np_s = np;
+            if (tid == 0) {
+                np_s = antipodal_walk(X, Y, n, pairs, 2 * kVerts);
             } else if (w == 1 && lane == 0) {
                 double cx = 0.0, cy = 0.0;
                 for (int q = 0; q < n; ++q) {
@@ -496,12 +494,12 @@ __device__ bool svd2_v(const double (&a)[2][2], double (&v)[2][2]) {
             s1 = 1.0 / tmp;
             c1 = u / tmp;
         }
-        // m <- rot1 applied on the left (rows 0, 1 of m).
-        double n00 = m00, n01 = m01, n10 = m10, n11 = m11;
+        // m <- rot1 applied on the left (rows 0, 1 of m); the symmetric
+        // rotation reads m00, m01 and m11 of the result.
+        double n00 = m00, n01 = m01, n11 = m11;
         if (!(c1 == 1.0 && s1 == 0.0)) {
             n00 = c1 * m00 + s1 * m10;
             n01 = c1 * m01 + s1 * m11;
-            n10 = -s1 * m00 + c1 * m10;
             n11 = -s1 * m01 + c1 * m11;
         }
         double cr, sr;
@@ -635,7 +633,10 @@ void launch_boxes(lp_context* c, int B, bool from_hulls) {
     const size_t BN = (size_t)L.B * L.N;
     stage_mark(c, "boxes");
     if (from_hulls) hipLaunchKernelGGL(k_box_gather, dim3(256, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_boxes, dim3(128, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN, d.hcy + BN, d.hull_off);
+    hipLaunchKernelGGL((k_boxes<0, kBoxSmall>), dim3(128, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN,
+                       d.hcy + BN, d.hull_off);
+    hipLaunchKernelGGL((k_boxes<kBoxSmall, kBoxVerts>), dim3(4, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN,
+                       d.hcy + BN, d.hull_off);
 }
 
 }  // namespace lp
