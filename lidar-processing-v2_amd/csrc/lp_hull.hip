@@ -137,20 +137,23 @@ __device__ __forceinline__ bool pt_less(const double* X, const double* Y, const 
 // One half of Andrew's chain (polygonizer.cpp:66-88) over the n sorted points
 // P: the lower chain visits P[0..n), the upper chain P[n-1..0] starting from
 // P[n-1], which the lower chain always ends with, so the two halves are
-// independent and run on two lanes at once.  The two top stack points live in
-// registers and the next point's coordinates are loaded ahead of the pop
-// loop.  Pops while the exact orientation is <= 0.  Writes the chain to S
+// independent and run on two lanes at once.  The top four stack points live
+// in registers (a pop shifts them and refills the fourth from the stack, two
+// pops before it is needed) and the points two ahead are loaded before the
+// pop loop.  Pops while the exact orientation is <= 0.  Writes the chain to S
 // and returns its length.
 __device__ int half_chain(const double* X, const double* Y, const uint16_t* P, int n, bool upper, uint16_t* S,
                           uint32_t* inexact) {
     int kk = 0;
-    double x1 = 0.0, y1 = 0.0, x2 = 0.0, y2 = 0.0;
+    double x1 = 0.0, y1 = 0.0, x2 = 0.0, y2 = 0.0, x3 = 0.0, y3 = 0.0, x4 = 0.0, y4 = 0.0;
     int ns;
     int p = upper ? P[n - 1] : P[0];
     double px = X[p], py = Y[p];
+    int pn = (n > 1) ? (upper ? P[n - 2] : P[1]) : p;
+    double pxn = X[pn], pyn = Y[pn];
     for (int t = 0; t < n; ++t) {
-        const int pn = (t + 1 < n) ? (upper ? P[n - 2 - t] : P[t + 1]) : p;
-        const double pxn = X[pn], pyn = Y[pn];
+        const int pnn = (t + 2 < n) ? (upper ? P[n - 3 - t] : P[t + 2]) : pn;
+        const double pxnn = X[pnn], pynn = Y[pnn];
         while (kk > 1) {
             const int o = orient_exact(x2, y2, x1, y1, px, py, &ns);
             *inexact += (o != ns) ? 1u : 0u;
@@ -158,13 +161,21 @@ __device__ int half_chain(const double* X, const double* Y, const uint16_t* P, i
             --kk;
             x1 = x2;
             y1 = y2;
-            if (kk > 1) {
-                const int q = S[kk - 2];
-                x2 = X[q];
-                y2 = Y[q];
+            x2 = x3;
+            y2 = y3;
+            x3 = x4;
+            y3 = y4;
+            if (kk > 3) {
+                const int q = S[kk - 4];
+                x4 = X[q];
+                y4 = Y[q];
             }
         }
         S[kk++] = static_cast<uint16_t>(p);
+        x4 = x3;
+        y4 = y3;
+        x3 = x2;
+        y3 = y2;
         x2 = x1;
         y2 = y1;
         x1 = px;
@@ -172,6 +183,9 @@ __device__ int half_chain(const double* X, const double* Y, const uint16_t* P, i
         p = pn;
         px = pxn;
         py = pyn;
+        pn = pnn;
+        pxn = pxnn;
+        pyn = pynn;
     }
     return kk;
 }
@@ -380,6 +394,43 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[
 // identity.  Returns the new count; *ns_in receives the number of points the
 // cull kept.
 template <int S>
+__device__ int sort_dedupe_chunk(double* X, double* Y, int32_t* I, uint16_t* P, uint64_t (&key)[S],
+                                 uint32_t (&val)[S], int ns);
+
+// The ns real keys (key != ~0) of S slots moved into the first of S2 < S
+// slots through LDS (X holds the keys, P the values: neither is read again
+// before the sorted write), then sorted there.
+template <int S, int S2>
+__device__ int compact_sort_chunk(double* X, double* Y, int32_t* I, uint16_t* P, uint64_t (&key)[S],
+                                  uint32_t (&val)[S], int ns) {
+    const int lane = lane_id();
+    uint64_t* ks = reinterpret_cast<uint64_t*>(X);
+    int before = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const bool real = key[s] != ~0ull;
+        const uint64_t m = __ballot(real);
+        if (real) {
+            const int q = before + static_cast<int>(lanes_below(m));
+            ks[q] = key[s];
+            P[q] = static_cast<uint16_t>(val[s]);
+        }
+        before += static_cast<int>(__popcll(m));
+    }
+    wave_sync();
+    uint64_t key2[S2];
+    uint32_t val2[S2];
+#pragma unroll
+    for (int s = 0; s < S2; ++s) {
+        const int e = s * kWave + lane;
+        key2[s] = e < ns ? ks[e] : ~0ull;
+        val2[s] = e < ns ? static_cast<uint32_t>(P[e]) : static_cast<uint32_t>(kHullChunk + e);
+    }
+    wave_sync();
+    return sort_dedupe_chunk<S2>(X, Y, I, P, key2, val2, ns);
+}
+
+template <int S>
 __device__ int cull_sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t* P, int n, int* ns_in) {
     const int lane = lane_id();
     uint64_t key[S];
@@ -447,6 +498,25 @@ __device__ int cull_sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t*
         ns += static_cast<int>(__popcll(__ballot(keep)));
     }
     *ns_in = ns;
+    // Fewer survivors than slots: sort them in fewer slots (the network's
+    // cost grows with the slot count).
+    if constexpr (S == 4) {
+        if (ns <= kWave) return compact_sort_chunk<4, 1>(X, Y, I, P, key, val, ns);
+        if (ns <= 2 * kWave) return compact_sort_chunk<4, 2>(X, Y, I, P, key, val, ns);
+    }
+    if constexpr (S == 2) {
+        if (ns <= kWave) return compact_sort_chunk<2, 1>(X, Y, I, P, key, val, ns);
+    }
+    return sort_dedupe_chunk<S>(X, Y, I, P, key, val, ns);
+}
+
+// Sort of the (key, value) slots (culled and padding keys are ~0 and sort
+// last), the duplicate drop and the write of X, Y, I in sorted order with P
+// the identity; ns is the number of real keys.
+template <int S>
+__device__ int sort_dedupe_chunk(double* X, double* Y, int32_t* I, uint16_t* P, uint64_t (&key)[S],
+                                 uint32_t (&val)[S], int ns) {
+    const int lane = lane_id();
     const bool keep_all = ns < 3;
     reg_bitonic<S>(key, val);
     int32_t idx[S];
