@@ -162,13 +162,14 @@ def run_gpu(args, frames):
                                note="host arrays in, labels/clusters/hulls out (PCIe included)"))
 
 
-def run_batched(frames, steps, noise=False, boxes=False, streams=2):
+def run_batched(frames, steps, noise=False, boxes=False, streams=2, load=0.5):
     """Frames/s of one batch of `frames` (any sizes) replayed as graphs on
     `streams` streams."""
     from lidar_amd import native as N
     maxn = max(f.n for f in frames)
     H = max(f.height for f in frames)
-    ctx = N.Context(max_frames=len(frames), max_points=maxn, max_height=H, max_width=2048, streams=streams)
+    ctx = N.Context(max_frames=len(frames), max_points=maxn, max_height=H, max_width=2048, streams=streams,
+                    hash_load_factor=load)
     if noise:
         ctx.set_noise_filter(N.noise_config())
     ctx.set_oriented_boxes(boxes)
@@ -189,7 +190,8 @@ def run_batched(frames, steps, noise=False, boxes=False, streams=2):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ctx.close()
-    return dict(frames=len(frames), streams=streams, frames_per_s=round(len(frames) * steps / dt, 2),
+    return dict(frames=len(frames), streams=streams, hash_load_factor=load,
+                frames_per_s=round(len(frames) * steps / dt, 2),
                 points_per_s=round(sum(f.n for f in frames) * steps / dt, 1),
                 ms_per_batch=round(dt / steps * 1e3, 4))
 
@@ -330,8 +332,17 @@ def main():
     extra = {}
     if not args.no_extra:
         from lidar_amd import synth
-        log("batched c3 (mixed widths)")
-        extra["batched_c3"] = run_batched([synth.workload_c3(i) for i in range(32)], 5)
+        from lidar_amd import autotune
+        from lidar_amd import native as N
+        c3 = [synth.workload_c3(i) for i in range(32)]
+        log("autotune c3 (streams x hash load factor)")
+        c3_cfgs = [N.segmenter_config(image_height=f.height, image_width=f.width, elevation_up_deg=f.fov_up_deg,
+                                      elevation_down_deg=f.fov_down_deg) for f in c3]
+        tuned = autotune.tune(c3, c3_cfgs, streams=(1, 2, 3), loads=(0.5, 0.7), steps=5, noise=False, boxes=False)
+        extra["c3_autotune"] = tuned
+        log("batched c3 (mixed widths, tuned)")
+        extra["batched_c3"] = run_batched(c3, 5, streams=tuned["best"]["streams"],
+                                          load=tuned["best"]["hash_load_factor"])
         log("batched c4 (128x2048)")
         extra["batched_c4"] = run_batched([synth.workload_c4(i) for i in range(16)], 5)
         if args.serve_seconds > 0:

@@ -112,6 +112,10 @@ typedef struct lp_capacity {
                                       sub-batches, each on its own stream and graph, so the
                                       latency-bound kernels of one overlap the others'
                                       work; max_frames is the total over all of them */
+    float hash_load_factor;        /* worst-case load of the voxel and noise-cell hash tables
+                                      (capacity = next power of two of points / factor):
+                                      0.5 (default) .. 0.9; a smaller table is cleared and
+                                      probed faster, a larger one has shorter probe runs */
 } lp_capacity;
 void lp_capacity_default(lp_capacity* cap);
 

@@ -350,6 +350,7 @@ void lp_capacity_default(lp_capacity* cap) {
     cap->max_voxels = 200000;            // clusterer.cpp:36
     cap->device = 0;
     cap->streams = 1;
+    cap->hash_load_factor = 0.5F;
 }
 
 const char* lp_last_error(void) { return g_last_error.c_str(); }
@@ -401,6 +402,8 @@ int create_lane(const lp_capacity* cap, lp_context** out) {
     if (cap->max_frames < 1 || cap->max_points_per_frame < 1 || cap->max_image_height < 1 ||
         cap->max_image_width < 1 || cap->max_image_height > 1024 || cap->max_voxels < 1)
         return fail(LP_ERR_INVALID_ARGUMENT, "invalid capacity");
+    if (!(cap->hash_load_factor >= 0.5F && cap->hash_load_factor <= 0.9F))
+        return fail(LP_ERR_INVALID_ARGUMENT, "hash_load_factor must be in [0.5, 0.9]");
     int ndev = 0;
     LP_HIP(hipGetDeviceCount(&ndev));
     if (cap->device < 0 || cap->device >= ndev) return fail(LP_ERR_INVALID_ARGUMENT, "no such HIP device");
@@ -422,7 +425,9 @@ int create_lane(const lp_capacity* cap, lp_context** out) {
     }
     L.cell_stride = 32768;
     L.cand_stride = 4 * 4096;
-    L.hash_cap = 1 << ceil_log2(2 * static_cast<int64_t>(L.N));
+    // Every point could occupy its own cell or voxel: the table holds N keys
+    // at the chosen worst-case load.
+    L.hash_cap = 1 << ceil_log2(static_cast<int64_t>(std::ceil(L.N / static_cast<double>(cap->hash_load_factor))));
     L.blk_stride = (std::max({L.N, L.P, L.hash_cap}) + 2047) / 2048 + 1;
     L.box_stride = L.N / 3 + 1024;
     const size_t bytes = layout_bytes(L, nullptr, nullptr);

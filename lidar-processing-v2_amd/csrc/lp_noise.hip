@@ -15,9 +15,9 @@
 // The rest go through the grid.
 //
 // Layout: the frame's points are bucketed into 1 m grid cells through an
-// open-addressing hash of cell keys (one slot per occupied cell, load <= 0.5),
-// counted, scanned and scattered into a cell-sorted float4 array (w holds the
-// cell's hash slot).  The count runs in two passes.  The first, one thread per
+// open-addressing hash of cell keys (one slot per occupied cell, load at most
+// the context's hash load factor), counted, scanned and scattered into a
+// cell-sorted float4 array (w holds the cell's hash slot).  The count runs in two passes.  The first, one thread per
 // point in cell order, checks the point's own cell outward from its own
 // position (points next to each other in scan order are mostly neighbours in
 // space), so almost every point of a dense surface reaches min_neighbours

@@ -98,6 +98,7 @@ class Capacity(ctypes.Structure):
         ("max_voxels", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("streams", ctypes.c_int32),
+        ("hash_load_factor", ctypes.c_float),
     ]
 
 
@@ -371,10 +372,11 @@ class Context:
     """One device context: a HIP stream plus preallocated buffers."""
 
     def __init__(self, max_frames=1, max_points=200000, max_height=64, max_width=2048, max_voxels=200000, device=0,
-                 streams=1):
+                 streams=1, hash_load_factor=0.5):
         cap = Capacity()
         lib().lp_capacity_default(ctypes.byref(cap))
         cap.streams = streams
+        cap.hash_load_factor = hash_load_factor
         cap.max_frames = max_frames
         cap.max_points_per_frame = max_points
         cap.max_image_height = max_height
