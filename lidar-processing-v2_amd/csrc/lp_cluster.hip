@@ -460,7 +460,7 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     hipLaunchKernelGGL(k_component_flag, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.cl_flag, d.cl_flag, L.N, kCountSlot, kFsObstacles, kFsClusters);
     hipLaunchKernelGGL(k_component_label, dim3(nb, B), dim3(256), 0, st, d, L);
-    launch_scan(c, B, d.cl_size, d.cl_off, L.N, kCountSlot, kFsClusters, -1);
+    launch_scan_small(c, B, d.cl_size, d.cl_off, L.N, kCountSlot, kFsClusters, -1);
     hipLaunchKernelGGL(k_cluster_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
 }
 

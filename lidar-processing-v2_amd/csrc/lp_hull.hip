@@ -1205,12 +1205,12 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     stage_mark(c, "hull");
     for (int r = 0; r < kHullRounds; ++r) {
         hipLaunchKernelGGL(k_hull_chunks, dim3(nb, B), dim3(256), 0, st, d, L, r);
-        launch_scan(c, B, d.hc_chunks, d.hc_choff, L.N, kCountSlot, kFsClusters, kFsHullChunks);
+        launch_scan_small(c, B, d.hc_chunks, d.hc_choff, L.N, kCountSlot, kFsClusters, kFsHullChunks);
         hipLaunchKernelGGL(k_hull_round, dim3(512, B), dim3(256), 0, st, d, L, r);
     }
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
     hipLaunchKernelGGL(k_hull_check, dim3(nb, B), dim3(256), 0, st, d, L);
-    launch_scan(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
+    launch_scan_small(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
     hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(kLocalThreads), 0, st, d, L, list_xy);
 }
 

@@ -17,6 +17,9 @@ enum ScanCount : int { kCountPoints = 0, kCountPixels = 1, kCountSlot = 2, kCoun
 // fs[f][total_slot] (if >= 0).
 void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
                  int count_slot, int total_slot);
+// The same in one launch, one workgroup per frame: for cluster-sized counts.
+void launch_scan_small(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
+                       int count_slot, int total_slot);
 // The same with element i of frame f read as in[f * in_stride + gather[f * stride + i]].
 void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, const int32_t* gather,
                         uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot);

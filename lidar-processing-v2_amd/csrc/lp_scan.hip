@@ -89,6 +89,48 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, 
         d.fs[f * kFsCount + total_slot] = pre_total + total;
 }
 
+// Scans whose counts are cluster-sized (a few hundred to a few thousand):
+// one 1024-thread workgroup per frame walks the frame in tiles of 4096 with
+// a running carry, one launch instead of a reduce and a down pass.  In-place
+// use (in == out) is safe: a tile is loaded before the block sum's barriers.
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallItems = 4;
+
+__global__ __launch_bounds__(kSmallThreads) void k_scan_small(DevBuf d, Layout L, ScanIn si, uint32_t* out,
+                                                              size_t stride, ScanCount kind, int slot,
+                                                              int total_slot) {
+    __shared__ uint32_t lds[kSmallThreads / kWave + 1];
+    const int f = blockIdx.x;
+    const int n = scan_count(d, L, f, kind, slot);
+    uint32_t* o = out + (size_t)f * stride;
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += kSmallThreads * kSmallItems) {
+        uint32_t v[kSmallItems];
+        uint32_t s = 0;
+        const int b = base + threadIdx.x * kSmallItems;
+#pragma unroll
+        for (int k = 0; k < kSmallItems; ++k) {
+            v[k] = (b + k < n) ? scan_value(si, stride, f, b + k) : 0u;
+            s += v[k];
+        }
+        uint32_t total;
+        uint32_t ex = block_exclusive_sum<kSmallThreads>(s, lds, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < kSmallItems; ++k) {
+            if (b + k < n) o[b + k] = ex;
+            ex += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_slot >= 0) d.fs[f * kFsCount + total_slot] = carry;
+}
+
+void launch_scan_small(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
+                       int count_slot, int total_slot) {
+    hipLaunchKernelGGL(k_scan_small, dim3(B), dim3(kSmallThreads), 0, c->stream, c->d, c->L,
+                       ScanIn{in, stride, nullptr}, out, stride, kind, count_slot, total_slot);
+}
+
 static void launch_scan_in(lp_context* c, int B, ScanIn si, uint32_t* out, size_t stride, ScanCount kind,
                            int count_slot, int total_slot) {
     const Layout& L = c->L;
