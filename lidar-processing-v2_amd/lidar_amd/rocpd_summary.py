@@ -3,6 +3,10 @@
   python -m lidar_amd.rocpd_summary trace <run_results.db> [frames_per_launch]
       kernel durations (calls, total, average, min, max), optionally only the
       launches whose grid covers that many frames (grid y / block y)
+  python -m lidar_amd.rocpd_summary trace_wg <run_results.db> frames
+      the same for launches of one workgroup per frame (grid y = 1 and grid
+      x / block x = frames: the row solve, the per-frame scans and
+      union-find, ...)
   python -m lidar_amd.rocpd_summary pmc <run_results.db>
       per-kernel counter values averaged per dispatch
   python -m lidar_amd.rocpd_summary traffic <fetch.db> <write.db> <frames> <copy_bytes>
@@ -24,12 +28,16 @@ def _short(name):
     return name.replace("void ", "")[:40]
 
 
-def trace(db, frames=None):
+def trace(db, frames=None, per_frame_grid=False):
     con = sqlite3.connect(db)
     agg = {}
-    for name, dur, gy, wy in con.execute("select name, duration, grid_y, workgroup_y from kernels"):
+    rows = con.execute("select name, duration, grid_x, workgroup_x, grid_y, workgroup_y from kernels")
+    for name, dur, gx, wx, gy, wy in rows:
         nb = gy // max(wy, 1) if gy else 1
-        if frames is not None and nb != frames:
+        if per_frame_grid:
+            if nb != 1 or gx // max(wx, 1) != frames:
+                continue
+        elif frames is not None and nb != frames:
             continue
         a = agg.setdefault(_short(name), [0, 0.0, float("inf"), 0.0])
         a[0] += 1
@@ -105,6 +113,8 @@ if __name__ == "__main__":
     mode, db = sys.argv[1], sys.argv[2]
     if mode == "trace":
         print(trace(db, int(sys.argv[3]) if len(sys.argv) > 3 else None))
+    elif mode == "trace_wg":
+        print(trace(db, int(sys.argv[3]), per_frame_grid=True))
     elif mode == "traffic":
         print(json.dumps(traffic(db, sys.argv[3], int(sys.argv[4]), int(sys.argv[5])), indent=1))
     else:
