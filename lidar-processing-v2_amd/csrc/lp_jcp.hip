@@ -234,6 +234,13 @@ __device__ __forceinline__ TMap tmap_dpp(const TMap& m) {
                                                                   kCtrl, kRowMask, 0xf, false))};
 }
 
+// Lane `src` (wavefront-uniform) of a map.
+__device__ __forceinline__ TMap tmap_readlane(const TMap& m, int src) {
+    return TMap{static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m.lo), src)),
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m.hi), src)),
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m.x), src))};
+}
+
 // Inclusive scan of the wavefront's maps, m := m_lane o ... o m_0: row
 // shifts by 1, 2, 4 and 8 inside each 16-lane row, then lane 15 of rows 0 / 2
 // into rows 1 / 3 and lane 31 into rows 2 and 3.
@@ -246,7 +253,7 @@ __device__ __forceinline__ TMap tmap_wave_scan(TMap m) {
     m = tmap_compose(m, tmap_dpp<0x143, 0xc>(m));
     return m;
 }
-constexpr int kSolveThreads = 256;
+constexpr int kSolveThreads = 512;
 constexpr int kSolveWaves = kSolveThreads / kWave;
 constexpr int kFinLds = 65536;
 // LDS label table of the solve: entries 0, 1, 2 are the static labels of
@@ -380,7 +387,8 @@ __device__ __forceinline__ TMap pixel_map(const SlotData& sd, int j, int rs, con
 
 // Row-ordered exact solve, one workgroup per frame: per row, transfer maps
 // of its queue pixels, a wavefront DPP scan of the maps, one LDS step across
-// the four wavefronts.
+// the eight wavefronts (two per SIMD, so a row of up to 512 queue pixels is
+// one step and the SIMDs interleave two wavefronts' map evaluation).
 __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L) {
     __shared__ uint8_t fin[kFinBase + kFinLds];
     __shared__ int row_start[1025];
@@ -421,15 +429,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
             m = tmap_wave_scan(m);  // m := m_j o ... o m_(first of this wavefront)
             if (lane == kWave - 1) wtot[w] = m;
             __syncthreads();
-            // Prefix of the earlier wavefronts (pre) and of all four (chain).
-            TMap chain = wtot[0];
-            TMap pre = TMap{kIdLo, kIdHi, kIdX};
-#pragma unroll
-            for (int q = 1; q < kSolveWaves; ++q) {
-                if (q == w) pre = chain;
-                chain = tmap_compose(wtot[q], chain);
-            }
-            if (w > 0) m = tmap_compose(m, pre);
+            // Prefixes of the wavefront totals: lane q scans wtot[q] o ... o
+            // wtot[0] with DPP row shifts (the totals sit in lanes 0..7 of
+            // row 0); pre is the prefix of the earlier wavefronts, chain
+            // that of all of them.
+            TMap t = lane < kSolveWaves ? wtot[lane] : TMap{kIdLo, kIdHi, kIdX};
+            t = tmap_compose(t, tmap_dpp<0x111, 0xf>(t));
+            t = tmap_compose(t, tmap_dpp<0x112, 0xf>(t));
+            t = tmap_compose(t, tmap_dpp<0x114, 0xf>(t));
+            static_assert(kSolveWaves <= 8, "three row-shift steps scan at most 8 wavefront totals");
+            const TMap chain = tmap_readlane(t, kSolveWaves - 1);
+            if (w > 0) m = tmap_compose(m, tmap_readlane(t, w - 1));
             const uint32_t ps = tmap_get(m, carry);
             if (j < re) fin[kFinBase + j] = static_cast<uint8_t>(ps % 3);
             carry = tmap_get(chain, carry);
