@@ -190,13 +190,6 @@ __global__ void k_voxel_insert(DevBuf d, Layout L) {
     d.vslot[ob + k] = slot;
 }
 
-__global__ void k_voxel_check(DevBuf d, Layout L) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= L.B_active) return;
-    if (static_cast<int32_t>(d.fs[f * kFsCount + kFsVoxels]) > d.cc->max_voxels)
-        atomicOr(&d.fs[f * kFsCount + kFsErr], kErrVoxelOverflow);
-}
-
 __device__ __forceinline__ int32_t hash_find(const int32_t* hkey, uint32_t mask, int32_t key) {
     uint32_t h = hash_slot(key, mask);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
@@ -266,6 +259,9 @@ __device__ __forceinline__ void lds_union(int32_t* par, int32_t a, int32_t b) {
 __global__ void k_voxel_neighbours(DevBuf d, Layout L) {
     int tile, f;
     xcd_tile_frame(tile, f);
+    // Voxel capacity (the reference's StaticUnorderedMap throws past it).
+    if (tile == 0 && threadIdx.x == 0 && static_cast<int32_t>(d.fs[f * kFsCount + kFsVoxels]) > d.cc->max_voxels)
+        atomicOr(&d.fs[f * kFsCount + kFsErr], kErrVoxelOverflow);
     const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
     const uint32_t v = tile * blockDim.x + threadIdx.x;
     if (v >= U) return;
@@ -348,18 +344,11 @@ __global__ void k_voxel_cc_global(DevBuf d, Layout L) {
     }
 }
 
-__global__ void k_voxel_compress_global(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
-    if (!d.fs[f * kFsCount + kFsCcGlobal]) return;
-    const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= U) return;
-    int32_t* par = d.vpar + (size_t)f * L.N;
-    par[v] = g_find(par, static_cast<int32_t>(v));
-}
-
 // Component minimum point index and point count (removeSmallClusters counts
-// points, clusterer.cpp:214-220): one atomic per voxel.
+// points, clusterer.cpp:214-220): one atomic per voxel.  After the
+// global-memory union-find every voxel also compresses its own parent to the
+// root (all unions are done, so concurrent finds only shorten paths within
+// one tree); the LDS union-find leaves the parents compressed already.
 __global__ void k_voxel_stats(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const uint32_t U = min(d.fs[f * kFsCount + kFsVoxels], static_cast<uint32_t>(L.N));
@@ -367,7 +356,11 @@ __global__ void k_voxel_stats(DevBuf d, Layout L) {
     if (v >= U) return;
     const size_t ob = (size_t)f * L.N;
     const size_t ho = (size_t)f * L.hash_cap;
-    const int32_t r = d.vpar[ob + v];
+    int32_t r = d.vpar[ob + v];
+    if (d.fs[f * kFsCount + kFsCcGlobal]) {
+        r = g_find(d.vpar + ob, static_cast<int32_t>(v));
+        d.vpar[ob + v] = r;
+    }
     const int32_t h = d.vlist[ob + v];
     atomicMin(&d.vmin[ob + r], d.hmin[ho + h]);
     atomicAdd(&d.vcnt[ob + r], d.hcount[ho + h]);
@@ -450,12 +443,10 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     stage_mark(c, "voxels");
     hipLaunchKernelGGL(k_spherical, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_insert, dim3(nb, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_voxel_check, dim3((B + 63) / 64), dim3(64), 0, st, d, L);
     stage_mark(c, "components");
     hipLaunchKernelGGL(k_voxel_neighbours, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_cc, dim3(B), dim3(1024), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_cc_global, dim3(nb, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_voxel_compress_global, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_stats, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_component_flag, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.cl_flag, d.cl_flag, L.N, kCountSlot, kFsObstacles, kFsClusters);

@@ -230,6 +230,29 @@ def test_standalone_cluster(ctx, oracle, seed):
     assert np.array_equal(lab, labo)
 
 
+def test_standalone_cluster_many_voxels(oracle):
+    """More occupied voxels than the LDS union-find holds (32768): the
+    global-memory union-find and its in-place compression run."""
+    rng = np.random.default_rng(7)
+    blobs = rng.uniform(-50, 50, (40, 2))
+    idx = rng.integers(0, 40, 30000)
+    bx = blobs[idx, 0] + rng.normal(0, 1.5, 30000)
+    by = blobs[idx, 1] + rng.normal(0, 1.5, 30000)
+    sx, sy = rng.uniform(-70, 70, 60000), rng.uniform(-70, 70, 60000)
+    x = np.concatenate([bx, sx]).astype(np.float32)
+    y = np.concatenate([by, sy]).astype(np.float32)
+    z = rng.uniform(-1.5, 3.0, len(x)).astype(np.float32)
+    perm = rng.permutation(len(x))
+    x, y, z = x[perm], y[perm], z[perm]
+    print(f"many voxels: {len(x)} points", flush=True)
+    with N.Context(max_points=131072) as c:
+        st, lab = c.cluster(x, y, z)
+    sto, labo = oracle.cluster(x, y, z)
+    assert st == 0 and sto == 0
+    assert np.array_equal(lab, labo)
+    assert lab.max() > 10
+
+
 def test_standalone_cluster_wrap_and_overflow(oracle):
     x = np.array([-10.0, -10.0, -10.0, 10.0, 10.0, 10.0], np.float32)
     y = np.array([0.0, 0.0, 0.0, 0.0873, 0.0873, 0.0873], np.float32)
