@@ -282,7 +282,7 @@ int run_pipeline(lp_context* c, int B, bool segment_only) {
     if (!segment_only) {
         launch_cluster(c, B, true);
         launch_hull(c, B);
-        if (c->boxes_on || c->vehicles_on) launch_boxes(c, B, true);
+        if (c->boxes_on || c->vehicles_on) launch_boxes(c, B);
         if (c->vehicles_on) launch_vehicles(c, B);
     }
     LP_HIP(hipGetLastError());
@@ -915,7 +915,7 @@ int polygon_boxes(lp_context* c, const double* x, const double* y, const int32_t
 int lp_oriented_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,
                       lp_bounding_box* boxes) {
     return polygon_boxes(c, x, y, offsets, num_polygons, boxes,
-                         [](lp_context* ctx, int) { launch_boxes(ctx, 1, false); });
+                         [](lp_context* ctx, int) { launch_boxes(ctx, 1); });
 }
 
 int lp_pca_boxes(lp_context* c, const double* x, const double* y, const int32_t* offsets, int32_t num_polygons,

@@ -952,14 +952,6 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
     }
 }
 
-__global__ void k_hull_check(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
-    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= K) return;
-    if (d.hull_count[(size_t)f * L.N + c] == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
-}
-
 // Hull vertices as cluster-local indices: rank of the obstacle index among
 // the cluster's members (the gather order of processor.cpp:640-658).  Among
 // members with exactly a vertex's coordinates the smallest index is reported:
@@ -968,6 +960,9 @@ __global__ void k_hull_check(DevBuf d, Layout L) {
 // std::sort), so the coordinates are the defined result and the index is made
 // canonical.  Coordinates come from the obstacle points, or for standalone
 // lists from the copy staged in `list_xy` (list offset + list-local index).
+// For the obstacle clusters the vertex coordinates (PointXY doubles, in hull
+// order) also go to the second candidate buffer, free once the hulls are
+// done, where the box stage reads its polygons.
 //
 // One workgroup per cluster: the hull (up to kLocalMax vertices) is staged in
 // LDS with a hash of its coordinates; one pass over the members lowers each
@@ -1016,12 +1011,16 @@ __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L
     const int w = tid / kWave;
     const int lane = lane_id();
     const size_t ob = (size_t)f * L.N;
+    const size_t BN = (size_t)L.B * L.N;
     for (int c = tile; c < K; c += gridDim.x) {
         const int V = static_cast<int>(d.hull_count[ob + c]);
         const uint32_t hoff = d.hull_off[ob + c];
         const uint32_t base = d.cl_off[ob + c];
         const int size = static_cast<int>(d.cl_size[ob + c]);
         const int32_t* members = d.cl_members + ob + base;
+        // A cluster left without a hull ran out of the chunk rounds' and the
+        // final workgroup's capacity.
+        if (V == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
         if (V > kLocalMax) {
             for (int i = w; i < V; i += kLocalThreads / kWave) {
                 const int32_t idx = d.hull_idx[ob + base + i];
@@ -1043,6 +1042,10 @@ __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L
                 if (lane == 0) {
                     d.hull_idx[ob + base + i] = cn;
                     d.hull_local[ob + hoff + i] = static_cast<int32_t>(cnt);
+                    if (!list_xy) {
+                        d.hcx[BN + ob + hoff + i] = px;
+                        d.hcy[BN + ob + hoff + i] = py;
+                    }
                 }
             }
             continue;
@@ -1058,6 +1061,10 @@ __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L
             canon[i] = idx;
             vx[i] = x;
             vy[i] = y;
+            if (!list_xy) {
+                d.hcx[BN + ob + hoff + i] = x;
+                d.hcy[BN + ob + hoff + i] = y;
+            }
             uint32_t h = xy_hash(x, y) & hmask;  // load <= 1/2: a free slot exists
             while (atomicCAS(&table[h], -1, i) != -1) h = (h + 1) & hmask;
         }
@@ -1209,7 +1216,6 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
         hipLaunchKernelGGL(k_hull_round, dim3(512, B), dim3(256), 0, st, d, L, r);
     }
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
-    hipLaunchKernelGGL(k_hull_check, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan_small(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
     hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(kLocalThreads), 0, st, d, L, list_xy);
 }

@@ -38,9 +38,10 @@ void launch_hull(lp_context* c, int B, const double2* list_xy = nullptr);
 // Noise filter over the staged raw inputs; with `compact`, the kept points
 // become the segmenter's inputs.
 void launch_noise(lp_context* c, int B, bool compact);
-// Oriented boxes of the hulls of the last hull stage (gathered from the
-// obstacle coordinates) or, for lists, of polygons staged by the caller.
-void launch_boxes(lp_context* c, int B, bool from_hulls);
+// Oriented boxes of the polygons in the second hull candidate buffer: the
+// hulls of the last hull stage (written there by the local-rank kernel) or,
+// for lists, the polygons staged by the caller.
+void launch_boxes(lp_context* c, int B);
 
 // Segmenter::image of frame f of the last run into img (H*W*3 bytes).
 void launch_label_image(lp_context* c, int f, int pixels, uint8_t* img);

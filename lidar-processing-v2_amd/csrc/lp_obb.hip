@@ -308,32 +308,6 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
     }
 }
 
-// Hull vertex coordinates (PointXY doubles of the obstacle points, in hull
-// order) into the second hull candidate buffer, free once the hulls are done.
-__global__ void k_box_gather(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
-    const uint32_t* fs = d.fs + f * kFsCount;
-    const int K = static_cast<int>(fs[kFsClusters]);
-    const uint32_t total = fs[kFsHullTotal];
-    const size_t ob = (size_t)f * L.N;
-    const size_t BN = (size_t)L.B * L.N;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < total; v += gridDim.x * blockDim.x) {
-        int lo = 0, hi = K;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (d.hull_off[ob + mid] <= v)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        const int c = lo - 1;
-        const uint32_t i = v - d.hull_off[ob + c];
-        const int32_t idx = d.hull_idx[ob + d.cl_off[ob + c] + i];
-        d.hcx[BN + ob + v] = static_cast<double>(d.ox[ob + idx]);
-        d.hcy[BN + ob + v] = static_cast<double>(d.oy[ob + idx]);
-    }
-}
-
 // Vehicle size classes of the processor's polygon simplification
 // (src/processor/src/processor.hpp:78-192): the five base dimension ranges
 // widened by the length / width / height tolerances, their volume and area
@@ -626,13 +600,12 @@ void launch_pca_boxes(lp_context* c, int num_polygons) {
                        c->d.hcy + BN, c->d.hull_off);
 }
 
-void launch_boxes(lp_context* c, int B, bool from_hulls) {
+void launch_boxes(lp_context* c, int B) {
     DevBuf& d = c->d;
     const Layout& L = c->L;
     hipStream_t st = c->stream;
     const size_t BN = (size_t)L.B * L.N;
     stage_mark(c, "boxes");
-    if (from_hulls) hipLaunchKernelGGL(k_box_gather, dim3(256, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL((k_boxes<0, kBoxSmall>), dim3(128, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN,
                        d.hcy + BN, d.hull_off);
     hipLaunchKernelGGL((k_boxes<kBoxSmall, kBoxVerts>), dim3(4, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN,
