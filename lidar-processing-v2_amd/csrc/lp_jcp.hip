@@ -43,30 +43,35 @@ constexpr uint32_t kSG = 0, kSO = 1, kSU = 2;
 // Queue membership: a ground pixel with an obstacle pixel in its clipped 5x5
 // window (cv::dilate of the red channel, then the raster scan of
 // segmenter.cpp:491-514).
-__global__ void k_jcp_mark(DevBuf d, Layout L) {
+// Queue flags: a ground pixel with an obstacle pixel in its 5x5 window
+// clipped to the image (the dilated red channel of segmenter.cpp:484-513).
+// Each workgroup takes 256 columns of one row and stages rows r-2..r+2 of
+// columns c0-2..c0+257 in LDS (out-of-image pixels read as empty).
+__global__ __launch_bounds__(256) void k_jcp_mark(DevBuf d, Layout L) {
+    constexpr int kSpan = 256 + 4;
+    __shared__ uint8_t win[5][kSpan];
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
     const int H = fd.sc.H, W = fd.sc.W;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= H * W) return;
+    const int tpr = (W + 255) / 256;
+    const int r = static_cast<int>(blockIdx.x) / tpr, c0 = (static_cast<int>(blockIdx.x) % tpr) * 256;
+    if (r >= H) return;
     const uint8_t* st = d.state + fd.pix_off;
-    uint32_t q = 0;
-    if (st[p] == kGround) {
-        const int r = p / W, c = p % W;
-        for (int dr = -2; dr <= 2 && !q; ++dr) {
-            const int rr = r + dr;
-            if (rr < 0 || rr >= H) continue;
-            for (int dc = -2; dc <= 2; ++dc) {
-                const int cc = c + dc;
-                if (cc < 0 || cc >= W) continue;
-                if (st[rr * W + cc] == kObstacle) {
-                    q = 1;
-                    break;
-                }
-            }
-        }
+    for (int q = threadIdx.x; q < 5 * kSpan; q += blockDim.x) {
+        const int rr = r + q / kSpan - 2, cc = c0 - 2 + q % kSpan;
+        win[q / kSpan][q % kSpan] = (rr >= 0 && rr < H && cc >= 0 && cc < W) ? st[rr * W + cc] : kEmpty;
     }
-    d.qflag[(size_t)f * L.P + p] = q;
+    __syncthreads();
+    const int c = c0 + static_cast<int>(threadIdx.x);
+    if (c >= W) return;
+    uint32_t q = 0;
+    if (win[2][threadIdx.x + 2] == kGround) {
+#pragma unroll
+        for (int dr = 0; dr < 5; ++dr)
+#pragma unroll
+            for (int dc = 0; dc < 5; ++dc) q |= win[dr][threadIdx.x + dc] == kObstacle ? 1u : 0u;
+    }
+    d.qflag[(size_t)f * L.P + r * W + c] = q;
 }
 
 // qmap holds the exclusive scan of qflag on entry.
@@ -525,7 +530,7 @@ void launch_jcp(lp_context* c, int B) {
     hipStream_t st = c->stream;
     const int pb = (L.P + 255) / 256;
     stage_mark(c, "jcp_queue");
-    hipLaunchKernelGGL(k_jcp_mark, dim3(pb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_jcp_mark, dim3(L.H * ((L.W + 255) / 256), B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.qflag, reinterpret_cast<uint32_t*>(d.qmap), L.P, kCountPixels, 0, kFsQueue);
     hipLaunchKernelGGL(k_jcp_fill, dim3(pb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "jcp_slots");
