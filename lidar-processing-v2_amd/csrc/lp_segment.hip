@@ -376,16 +376,16 @@ __global__ void k_recm(DevBuf d, Layout L) {
     }
 }
 
-// RANSAC candidate count per near cell, one wave per cell, in the
-// reference's iteration order: k = slice*4 + radial (segmenter.cpp:328-351).
+// RANSAC candidate count per near cell, one wave per cell (a grid-stride
+// loop over the frame's slices x 4 near cells), in the reference's iteration
+// order: k = slice*4 + radial (segmenter.cpp:328-351).
 __global__ __launch_bounds__(256) void k_cand_count(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
     const SegConst& s = fd.sc;
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
-    const int k = blockIdx.x * 4 + w;
-    if (k >= s.slices * kRansacRadialBins) return;
+    for (int k = blockIdx.x * 4 + w; k < s.slices * kRansacRadialBins; k += gridDim.x * 4) {
     const int cell = (k / kRansacRadialBins) * s.rings + (k % kRansacRadialBins);
     const size_t co = (size_t)f * L.cell_stride;
     const uint32_t n = d.cell_count[co + cell];
@@ -399,6 +399,7 @@ __global__ __launch_bounds__(256) void k_cand_count(DevBuf d, Layout L) {
     }
     c = wave_sum(c);
     if (lane == 0) d.cand_count[(size_t)f * L.cand_stride + k] = c;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_cand_scan(DevBuf d, Layout L) {
@@ -431,15 +432,14 @@ __global__ __launch_bounds__(1024) void k_cand_scan(DevBuf d, Layout L) {
 }
 
 // Candidate compaction: each wave writes its cell's candidates in input
-// order at the cell's scanned offset.
+// order at the cell's scanned offset (grid-stride over the near cells).
 __global__ __launch_bounds__(256) void k_cand_fill(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
     const SegConst& s = fd.sc;
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
-    const int k = blockIdx.x * 4 + w;
-    if (k >= s.slices * kRansacRadialBins) return;
+    for (int k = blockIdx.x * 4 + w; k < s.slices * kRansacRadialBins; k += gridDim.x * 4) {
     const int cell = (k / kRansacRadialBins) * s.rings + (k % kRansacRadialBins);
     const size_t co = (size_t)f * L.cell_stride;
     const uint32_t n = d.cell_count[co + cell];
@@ -467,6 +467,7 @@ __global__ __launch_bounds__(256) void k_cand_fill(DevBuf d, Layout L) {
             d.cz[ob + o] = z;
         }
         pos += static_cast<uint32_t>(__popcll(mask));
+    }
     }
 }
 
@@ -709,9 +710,10 @@ void launch_segment_front(lp_context* c, int B) {
     stage_mark(c, "recm");
     hipLaunchKernelGGL(k_recm, dim3((L.cand_stride / kRansacRadialBins + 255) / 256, B), dim3(256), 0, st, d, L);
     stage_mark(c, "ransac");
-    hipLaunchKernelGGL(k_cand_count, dim3((L.cand_stride + 3) / 4, B), dim3(256), 0, st, d, L);
+    // 128 workgroups (512 wavefronts) per frame walk the slices x 4 near cells.
+    hipLaunchKernelGGL(k_cand_count, dim3(128, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_cand_scan, dim3(B), dim3(1024), 0, st, d, L);
-    hipLaunchKernelGGL(k_cand_fill, dim3((L.cand_stride + 3) / 4, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_cand_fill, dim3(128, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_ransac_hyp, dim3(B), dim3(64), 0, st, d, L);
     hipLaunchKernelGGL(k_ransac_count, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_ransac_best, dim3((B + 63) / 64), dim3(64), 0, st, d, L);
