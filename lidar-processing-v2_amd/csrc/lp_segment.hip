@@ -505,7 +505,11 @@ __global__ void k_mt19937(uint32_t* out, int count) {
 // (segmenter.cpp:389-418).
 __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
     __shared__ uint32_t idx2[kRansacIters], idx3[kRansacIters];
-    __shared__ uint32_t raw[kMtRawCount];
+    // 60 hypotheses draw about 120 values (a few more after rejections or
+    // repeated indices): the first 256 are staged in LDS, later ones are read
+    // from global memory.
+    constexpr int kStaged = 256;
+    __shared__ uint32_t raw[kStaged];
     __shared__ uint32_t ok;
     const int f = blockIdx.x;
     const FrameDesc& fd = d.frames[f];
@@ -517,7 +521,8 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
         if (lane == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrRansacSingle);
         return;
     }
-    for (int i = lane; i < kMtRawCount; i += 64) raw[i] = d.mt_raw[i];
+#pragma unroll
+    for (int i = lane; i < kStaged; i += 64) raw[i] = d.mt_raw[i];
     __syncthreads();
     if (lane == 0) {
         int pos = 0;
@@ -527,7 +532,8 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
                 good = false;
                 return 0;
             }
-            uint64_t prod = static_cast<uint64_t>(raw[pos++]) * n;
+            uint64_t prod = static_cast<uint64_t>(pos < kStaged ? raw[pos] : d.mt_raw[pos]) * n;
+            ++pos;
             uint32_t low = static_cast<uint32_t>(prod);
             if (low < n) {
                 const uint32_t thresh = (0u - n) % n;
@@ -536,7 +542,8 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
                         good = false;
                         break;
                     }
-                    prod = static_cast<uint64_t>(raw[pos++]) * n;
+                    prod = static_cast<uint64_t>(pos < kStaged ? raw[pos] : d.mt_raw[pos]) * n;
+                    ++pos;
                     low = static_cast<uint32_t>(prod);
                 }
             }
