@@ -99,7 +99,7 @@ __global__ void k_project(DevBuf d, Layout L) {
 // Cell starts (exclusive scan of the counts), the list of non-empty cells
 // for k_cell_sort, and the empty marker of every empty cell's minimum.
 __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
-    __shared__ uint32_t lds[1024 / kWave + 1];
+    __shared__ uint64_t lds[1024 / kWave + 1];
     const int f = blockIdx.x;
     const int cells = d.frames[f].sc.cells;
     const size_t co = (size_t)f * L.cell_stride;
@@ -118,9 +118,13 @@ __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
             local += v[k];
             nz += v[k] ? 1u : 0u;
         }
-        uint32_t total, nz_total;
-        uint32_t ex = block_exclusive_sum<1024>(local, lds, &total) + carry;
-        uint32_t lx = block_exclusive_sum<1024>(nz, lds, &nz_total) + used;
+        // Both sums in one block scan: the point count in the high word, the
+        // non-empty cell count in the low one (each frame total < 2^32).
+        uint64_t both_total;
+        const uint64_t both = block_exclusive_sum<1024>((static_cast<uint64_t>(local) << 32) | nz, lds, &both_total);
+        const uint32_t total = static_cast<uint32_t>(both_total >> 32), nz_total = static_cast<uint32_t>(both_total);
+        uint32_t ex = static_cast<uint32_t>(both >> 32) + carry;
+        uint32_t lx = static_cast<uint32_t>(both) + used;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (b + k < cells) {
