@@ -366,17 +366,26 @@ __global__ void k_recm(DevBuf d, Layout L) {
     const float* zm = d.zmin + co;
     E[off] = s.e0;
     float prev = s.e0;
-    for (int r = 1; r < s.rings; ++r) {
-        const float z = zm[off + r];
-        float e;
-        if (f2u(z) == kEmptyZ) {
-            e = prev + s.dz;
-        } else {
-            const float b = prev + s.dz;
-            e = (b < z) ? b : z;  // std::min(curr_z_min, prev + dz)
+    // Eight rings' minima are loaded together ahead of their eight
+    // recurrence steps (the loads do not depend on the recurrence).
+    for (int r0 = 1; r0 < s.rings; r0 += 8) {
+        float zz[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) zz[k] = (r0 + k < s.rings) ? zm[off + r0 + k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (r0 + k >= s.rings) break;
+            const float z = zz[k];
+            float e;
+            if (f2u(z) == kEmptyZ) {
+                e = prev + s.dz;
+            } else {
+                const float b = prev + s.dz;
+                e = (b < z) ? b : z;  // std::min(curr_z_min, prev + dz)
+            }
+            E[off + r0 + k] = e;
+            prev = e;
         }
-        E[off + r] = e;
-        prev = e;
     }
 }
 
