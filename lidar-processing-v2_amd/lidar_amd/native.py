@@ -631,6 +631,9 @@ class FrameSlot:
         self.hidx = np.zeros(2 * n + 16, np.int32)
         self.noise = np.zeros(n, np.uint8)
         self.boxes = (BoundingBox * (n if boxes else 1))()
+        self.gidx = np.zeros(n, np.int32)
+        self.uidx = np.zeros(n, np.int32)
+        self.vcls = np.zeros(n, np.int32)
         self.out = FrameOutputs()
         self.out.seg_labels = self.seg.ctypes.data
         self.out.obstacle_index = self.obs.ctypes.data
@@ -640,26 +643,31 @@ class FrameSlot:
         self.out.hull_capacity = len(self.hidx)
         self.out.noise_labels = self.noise.ctypes.data
         self.out.boxes = ctypes.addressof(self.boxes)
+        self.out.ground_index = self.gidx.ctypes.data
+        self.out.unknown_index = self.uidx.ctypes.data
+        self.out.vehicle_class = self.vcls.ctypes.data
         self.res = FrameResult()
         self.cloud = None
         self.cfg = None
         self.inputs = None
 
-    def result(self, boxes=False):
+    def result(self, boxes=False, vehicles=False):
         r = self.res
         no, K, V = r.num_obstacles, r.num_clusters, r.num_hull_vertices
         return dict(status=r.status, num_points=r.num_points, num_noise=r.num_noise,
                     seg_labels=self.seg[:r.num_points].copy(), obstacle_index=self.obs[:no].copy(),
                     cluster_labels=self.clab[:no].copy(), num_clusters=K, hull_offsets=self.hoff[:K + 1].copy(),
                     hull_indices=self.hidx[:V].copy(), boxes=[box_dict(self.boxes[k]) for k in range(K)] if boxes
-                    else [], num_voxels=r.num_voxels)
+                    else [], num_voxels=r.num_voxels, ground_index=self.gidx[:r.num_ground].copy(),
+                    unknown_index=self.uidx[:r.num_unknown].copy(), num_vehicles=r.num_vehicles,
+                    vehicle_class=self.vcls[:K].copy() if vehicles else None)
 
 
 class Server:
     """lp_server: continuous batching over several contexts."""
 
     def __init__(self, slots=3, max_batch_frames=8, max_points=262144, max_height=128, max_width=2048,
-                 hbm_budget_bytes=0, noise_cfg=None, boxes=False, ccfg=None):
+                 hbm_budget_bytes=0, noise_cfg=None, boxes=False, ccfg=None, label_clouds=False, vehicles=False):
         cfg = ServerConfig()
         lib().lp_server_config_default(ctypes.byref(cfg))
         cfg.slots = slots
@@ -674,10 +682,13 @@ class Server:
             cfg.noise_filter = 1
             cfg.noise = noise_cfg
         cfg.oriented_boxes = 1 if boxes else 0
+        cfg.label_clouds = 1 if label_clouds else 0
+        cfg.vehicle_boxes = 1 if vehicles else 0
         h = ctypes.c_void_p()
         check(lib().lp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
-        self.boxes = boxes
+        self.boxes = boxes or vehicles
+        self.vehicles = vehicles
         self._done = (ctypes.c_uint64 * 1024)()
 
     def close(self):

@@ -71,7 +71,7 @@ def serve(server, pools, schedule, keep_results=False, on_done=None):
             lat[tag] = t_done - schedule[tag][0]
             slot = busy.pop(tag)
             if keep_results:
-                results[tag] = slot.result(server.boxes)
+                results[tag] = slot.result(server.boxes, server.vehicles)
             if on_done is not None:
                 on_done(tag, slot)
             free.append(slot)

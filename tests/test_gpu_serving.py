@@ -11,28 +11,32 @@ from tests.test_gpu_noise_boxes import box_equal
 pytestmark = pytest.mark.gpu
 
 
-def alone(pools, noise, boxes):
+def alone(pools, noise, boxes, extras=False):
     """Each pool frame processed on its own context, one frame per batch."""
     ref = {}
     with N.Context(max_frames=1, max_points=262144, max_height=128, max_width=2048) as ctx:
         if noise:
             ctx.set_noise_filter(N.noise_config())
         ctx.set_oriented_boxes(boxes)
+        if extras:
+            ctx.set_label_clouds(True)
+            ctx.set_vehicle_boxes(True)
         for s, p in enumerate(pools):
             for k, f in enumerate(p["frames"]):
                 ref[s, k] = ctx.process([(f.x, f.y, f.z, f.ring)], p["cfg"])[0]
     return ref
 
 
-@pytest.mark.parametrize("noise,boxes", [(False, False), (True, True)])
-def test_alone_equals_in_mix(noise, boxes):
+@pytest.mark.parametrize("noise,boxes,extras", [(False, False, False), (True, True, False), (True, True, True)])
+def test_alone_equals_in_mix(noise, boxes, extras):
     pools = serving.sensor_pools(pool_frames=2)
-    ref = alone(pools, noise, boxes)
+    ref = alone(pools, noise, boxes, extras)
     # 200 Hz per sensor for 0.4 s: frames pile up while batches run, so the
     # slots take mixed batches of all three sensor geometries.
     schedule = serving.arrival_schedule(0.4, rate_hz=200.0, jitter_ms=2.0, seed=3)
-    srv = N.Server(slots=3, max_batch_frames=8, noise_cfg=N.noise_config() if noise else None, boxes=boxes)
-    print(f"serving {len(schedule)} frames noise={noise} boxes={boxes}", flush=True)
+    srv = N.Server(slots=3, max_batch_frames=8, noise_cfg=N.noise_config() if noise else None, boxes=boxes,
+                   label_clouds=extras, vehicles=extras)
+    print(f"serving {len(schedule)} frames noise={noise} boxes={boxes} extras={extras}", flush=True)
     out = serving.serve(srv, pools, schedule, keep_results=True)
     st = srv.stats()
     srv.close()
@@ -47,6 +51,10 @@ def test_alone_equals_in_mix(noise, boxes):
         if boxes:
             assert len(got["boxes"]) == len(want["boxes"])
             assert all(box_equal(a, b) for a, b in zip(got["boxes"], want["boxes"]))
+        if extras:
+            for key in ("ground_index", "unknown_index", "vehicle_class"):
+                assert np.array_equal(got[key], want[key]), (tag, key)
+            assert got["num_vehicles"] == want["num_vehicles"]
 
 
 def test_hbm_budget_admission():
