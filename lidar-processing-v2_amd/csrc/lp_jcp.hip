@@ -188,12 +188,41 @@ __global__ void k_jcp_slots(DevBuf d, Layout L) {
         }
     }
     const bool ok = fabsf(sum) > 1.1920928955078125e-07f;  // FLT_EPSILON
+    float w[24];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) w[i] = ok ? u[i] / sum : 0.0f;
+    // Head slots 0..11 (rows -2, -1 and the two left neighbours): sources
+    // and weights as they are.  Tail slots 12..23 (right neighbours and rows
+    // +1, +2) name a neighbour raster-after their writer (the pixel itself
+    // or, stale, an earlier queue pixel), so their source is static or a
+    // pending queue pixel: their addends are known here.  They are stored as
+    // the ground list and the obstacle list of nonzero weights in slot order,
+    // zero-padded; dropping +0 addends leaves the ordered sums bit-identical
+    // (every partial sum is >= +0).
+    float tg[12], to[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) tg[k] = to[k] = 0.0f;
+    int ng = 0, no = 0;
+#pragma unroll
+    for (int i = 12; i < 24; ++i) {
+        const bool g = src[i] == kSrcGround && w[i] != 0.0f;
+        const bool o = src[i] == kSrcObstacle && w[i] != 0.0f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            tg[k] = (g && ng == k) ? w[i] : tg[k];
+            to[k] = (o && no == k) ? w[i] : to[k];
+        }
+        ng += g ? 1 : 0;
+        no += o ? 1 : 0;
+    }
     float* wo = d.slot_w + ((size_t)f * L.P + j) * 24;
     int32_t* so = d.slot_src + ((size_t)f * L.P + j) * 24;
 #pragma unroll
-    for (int i = 0; i < 24; ++i) {
-        wo[i] = ok ? u[i] / sum : 0.0f;
-        so[i] = src[i];
+    for (int k = 0; k < 12; ++k) {
+        wo[k] = tg[k];
+        wo[12 + k] = to[k];
+        so[k] = src[k];
+        so[12 + k] = __float_as_int(w[k]);
     }
     d.qsum_ok[(size_t)f * L.P + j] = ok ? 1 : 0;
 }
@@ -267,18 +296,11 @@ constexpr int kFinBase = 3;
 static_assert(kSrcObstacle + kFinBase == 0 && kSrcGround + kFinBase == 1 && kSrcUnknown + kFinBase == 2,
               "static sources map to the first three label entries");
 
-// wg += w for a GROUND slot, wo += w for an OBSTACLE slot (segmenter.cpp:
-// 613-626).  Written as two selects: every weight is +0 or positive and the
-// sums start at +0, so adding +0.0f to the other sum leaves it bit-identical,
-// and the select form keeps both sums in registers.
-__device__ __forceinline__ void add_slot(uint32_t l, float w, float& wg, float& wo) {
-    wg += (l == kSG) ? w : 0.0f;
-    wo += (l == kSO) ? w : 0.0f;
-}
-
-// Slot weights, label sources and the sum test of one queue pixel, as
-// written by k_jcp_slots.  They do not depend on solved labels, so the solve
-// loads the next chunk's while it solves the current one.
+// Slot data of one queue pixel, as written by k_jcp_slots: w holds the
+// tail's ground addend list (entries 0..11) and obstacle addend list (12..23),
+// s the head slots' sources (0..11) and weight bit patterns (12..23), ok the
+// sum test.  None of it depends on solved labels, so the solve loads the next
+// chunk's while it solves the current one.
 struct SlotData {
     float4 w[6];
     int4 s[6];
@@ -296,98 +318,105 @@ __device__ __forceinline__ void load_slots(const DevBuf& d, size_t qo, int j, Sl
     sd.ok = d.qsum_ok[qo + j];
 }
 
+// Byte vectors over the nine pair states ps = a*3 + b, laid out like a TMap
+// (bytes 0-3 of lo, 4-7 of hi, byte 0 of x): 3a, 3b and b.
+constexpr uint32_t kA3Lo = 0x03000000u, kA3Hi = 0x06060303u, kA3X = 0x06u;
+constexpr uint32_t kB3Lo = 0x00060300u, kB3Hi = 0x03000603u, kB3X = 0x06u;
+constexpr uint32_t kB1Lo = 0x00020100u, kB1Hi = 0x01000201u, kB1X = 0x02u;
+
+__device__ __forceinline__ uint32_t obstacle_wins(float o, float g) { return o > g ? kSO : kSG; }
+
 // Transfer map of queue pixel j (pair state (state(j-2), state(j-1)) ->
 // (state(j-1), state(j))).  Labels of earlier rows come from `fin`.
 __device__ __forceinline__ TMap pixel_map(const SlotData& sd, int j, int rs, const uint8_t* fin) {
-    const bool ok = sd.ok != 0;
-    float wv[24];
-    int32_t src[24];
+    float tw[24], hw[12];
+    int32_t hs[12];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        const float4 a = sd.w[q];
-        const int4 b = sd.s[q];
-        wv[4 * q] = a.x;
-        wv[4 * q + 1] = a.y;
-        wv[4 * q + 2] = a.z;
-        wv[4 * q + 3] = a.w;
-        src[4 * q] = b.x;
-        src[4 * q + 1] = b.y;
-        src[4 * q + 2] = b.z;
-        src[4 * q + 3] = b.w;
+        tw[4 * q] = sd.w[q].x;
+        tw[4 * q + 1] = sd.w[q].y;
+        tw[4 * q + 2] = sd.w[q].z;
+        tw[4 * q + 3] = sd.w[q].w;
     }
-    // Static and earlier-row labels.  A source in this row can only be the
-    // queue pixel before (j-1) or two before (j-2), through slots (0,-2) and
-    // (0,-1) (indices 10, 11): stale slots always read earlier rows.
-    // `fin` holds the three static labels below the queue states (entry
-    // sv + kFinBase), so every slot is one unconditional LDS read; sources in
-    // this row (sv >= rs) read as pending here and come from the pair state.
-    uint32_t lab[24];
 #pragma unroll
-    for (int i = 0; i < 24; ++i) {
-        const int32_t sv = src[i];
-        const uint32_t fv = fin[sv + kFinBase];
-        lab[i] = sv < rs ? fv : kSU;
+    for (int q = 0; q < 3; ++q) {
+        hs[4 * q] = sd.s[q].x;
+        hs[4 * q + 1] = sd.s[q].y;
+        hs[4 * q + 2] = sd.s[q].z;
+        hs[4 * q + 3] = sd.s[q].w;
+        hw[4 * q] = __int_as_float(sd.s[q + 3].x);
+        hw[4 * q + 1] = __int_as_float(sd.s[q + 3].y);
+        hw[4 * q + 2] = __int_as_float(sd.s[q + 3].z);
+        hw[4 * q + 3] = __int_as_float(sd.s[q + 3].w);
     }
-    // Only sources inside this row come from the scan's pair state; a stale
-    // column slot at column 0 or 1 can name the last queue pixels of the
-    // previous row, whose labels are already final in `fin`.
-    const int sel10 = src[10] < rs ? 0 : (src[10] == j - 2 ? 1 : (src[10] == j - 1 ? 2 : 0));  // 1: j-2, 2: j-1
-    const bool sel11 = src[11] >= rs && src[11] == j - 1;
-    if (!ok) {
-        // Every pair state (a, b) goes to (b, UNKNOWN).
-        return TMap{(0u * 3 + kSU) | ((1u * 3 + kSU) << 8) | ((2u * 3 + kSU) << 16) | ((0u * 3 + kSU) << 24),
-                    (1u * 3 + kSU) | ((2u * 3 + kSU) << 8) | ((0u * 3 + kSU) << 16) | ((1u * 3 + kSU) << 24),
-                    2u * 3 + kSU};
-    }
-    // Ordered sums of segmenter.cpp:613-626: slots 0..9 are fixed, 10 and 11
-    // may depend on the row's previous states, 12..23 are fixed again.  The
-    // ground sum only sees which of slots 10, 11 are GROUND and the obstacle
-    // sum which are OBSTACLE, so four chains of each (q bit 0: slot 10
-    // counts, bit 1: slot 11 counts), each in the reference's order, cover
-    // all nine pair states; the chains are independent and interleave.
-    // Per-slot addends (the weight or +0) of the two sums, formed once.
-    float ag[24], ao[24];
-#pragma unroll
-    for (int i = 0; i < 24; ++i) {
-        ag[i] = lab[i] == kSG ? wv[i] : 0.0f;
-        ao[i] = lab[i] == kSO ? wv[i] : 0.0f;
-    }
+    // Ordered sums of segmenter.cpp:613-626 (wg += w for a GROUND slot, wo +=
+    // w for an OBSTACLE slot), written as selects of w or +0: every weight
+    // is +0 or positive and the sums start at +0, so adding +0 leaves a sum
+    // bit-identical.  Slots 0..9 (rows -2, -1) name
+    // static labels or queue pixels of earlier rows, all final in `fin`
+    // (which holds the three static labels below the queue states, entry
+    // sv + kFinBase).
     float wg0 = 0.0f, wo0 = 0.0f;
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
-        wg0 += ag[i];
-        wo0 += ao[i];
+        const uint32_t l = fin[hs[i] + kFinBase];
+        wg0 += l == kSG ? hw[i] : 0.0f;
+        wo0 += l == kSO ? hw[i] : 0.0f;
     }
-    float tg[4], to[4];
+    // Slots (0,-2), (0,-1) (10, 11): a source in this row is the queue pixel
+    // before (j-1) or two before (j-2) and comes from the pair state; other
+    // sources are static or earlier rows (a stale slot at column 0 or 1 can
+    // name the last queue pixels of the previous row).
+    const int32_t s10 = hs[10], s11 = hs[11];
+    const uint32_t lab10 = s10 < rs ? fin[s10 + kFinBase] : kSU;
+    const uint32_t lab11 = s11 < rs ? fin[s11 + kFinBase] : kSU;
+    const int sel10 = s10 < rs ? 0 : (s10 == j - 2 ? 1 : (s10 == j - 1 ? 2 : 0));  // 1: j-2, 2: j-1
+    const bool sel11 = s11 >= rs && s11 == j - 1;
+    // Each sum with slot 10 / slot 11 counted or not (q bit 0 / bit 1): the
+    // ground sum only sees which of them are GROUND and the obstacle sum which
+    // are OBSTACLE, so four values of each cover all nine pair states.
+    float g0 = wg0, g1 = wg0 + hw[10], g2 = wg0 + hw[11];
+    float g3 = g1 + hw[11];
+    float o0 = wo0, o1 = wo0 + hw[10], o2 = wo0 + hw[11];
+    float o3 = o1 + hw[11];
+    // The tail (slots 12..23) in slot order: the lists hold the nonzero
+    // addends only, so the loops end once every lane's list has ended.
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float g = wg0, o = wo0;
-        g += (q & 1) ? wv[10] : 0.0f;
-        o += (q & 1) ? wv[10] : 0.0f;
-        g += (q & 2) ? wv[11] : 0.0f;
-        o += (q & 2) ? wv[11] : 0.0f;
-#pragma unroll
-        for (int i = 12; i < 24; ++i) {
-            g += ag[i];
-            o += ao[i];
-        }
-        tg[q] = g;
-        to[q] = o;
+    for (int k = 0; k < 12; ++k) {
+        const float a = tw[k];
+        if (__builtin_amdgcn_ballot_w64(a != 0.0f) == 0) break;
+        g0 += a;
+        g1 += a;
+        g2 += a;
+        g3 += a;
     }
-    uint32_t e[3] = {0u, 0u, 0u};
 #pragma unroll
-    for (int ps = 0; ps < 9; ++ps) {
-        const uint32_t a = ps / 3, b = ps % 3;
-        const uint32_t l10 = sel10 == 1 ? a : (sel10 == 2 ? b : lab[10]);
-        const uint32_t l11 = sel11 ? b : lab[11];
-        const float g01 = l10 == kSG ? tg[1] : tg[0], g23 = l10 == kSG ? tg[3] : tg[2];
-        const float o01 = l10 == kSO ? to[1] : to[0], o23 = l10 == kSO ? to[3] : to[2];
-        const float g = l11 == kSG ? g23 : g01;
-        const float o = l11 == kSO ? o23 : o01;
-        const uint32_t t = (o > g) ? kSO : kSG;
-        e[ps / 4] |= (b * 3 + t) << (8 * (ps % 4));
+    for (int k = 0; k < 12; ++k) {
+        const float a = tw[12 + k];
+        if (__builtin_amdgcn_ballot_w64(a != 0.0f) == 0) break;
+        o0 += a;
+        o1 += a;
+        o2 += a;
+        o3 += a;
     }
-    return TMap{e[0], e[1], e[2]};
+    // Decision table over the label pairs (l10, l11), byte l10 * 3 + l11:
+    // ground sum g[(l10 == G) | (l11 == G) << 1] against obstacle sum
+    // o[(l10 == O) | (l11 == O) << 1].  A pixel whose weight sum fails the
+    // test goes to PENDING for every pair state.
+    TMap T{obstacle_wins(o0, g3) | obstacle_wins(o2, g1) << 8 | obstacle_wins(o0, g1) << 16 |
+               obstacle_wins(o1, g2) << 24,
+           obstacle_wins(o3, g0) | obstacle_wins(o1, g0) << 8 | obstacle_wins(o0, g2) << 16 |
+               obstacle_wins(o2, g0) << 24,
+           obstacle_wins(o0, g0)};
+    if (sd.ok == 0) T = TMap{kSU * 0x01010101u, kSU * 0x01010101u, kSU};
+    // Per pair state (a, b): l10 is a, b or static, l11 is b or static; the
+    // entry is (b, t(l10, l11)).
+    const uint32_t r10 = lab10 * 3u * 0x01010101u, r11 = lab11 * 0x01010101u;
+    const uint32_t ilo = (sel10 == 1 ? kA3Lo : (sel10 == 2 ? kB3Lo : r10)) + (sel11 ? kB1Lo : r11);
+    const uint32_t ihi = (sel10 == 1 ? kA3Hi : (sel10 == 2 ? kB3Hi : r10)) + (sel11 ? kB1Hi : r11);
+    const uint32_t ix = (sel10 == 1 ? kA3X : (sel10 == 2 ? kB3X : r10)) + (sel11 ? kB1X : r11);
+    const uint32_t t8 = T.x * 0x01010101u;
+    return TMap{tmap_pick(T, t8, ilo) + kB3Lo, tmap_pick(T, t8, ihi) + kB3Hi, tmap_pick(T, t8, ix) + kB3X};
 }
 
 // Row-ordered exact solve, one workgroup per frame: per row, transfer maps
@@ -414,43 +443,57 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
     const int32_t* ql = d.qlist + qo;
     for (int r = tid; r <= s.H; r += kSolveThreads) row_start[r] = lower_bound_i32(ql, nq, r * s.W);
     if (tid < kFinBase) fin[tid] = static_cast<uint8_t>(tid == 0 ? kSO : (tid == 1 ? kSG : kSU));
-    // Chunk bases run rs, rs + T, ... below re, then the next non-empty row
-    // starts at re: the first chunk starts at 0 and each next base is known.
-    // Slot loads are unconditional (index clamped to the last queue pixel):
-    // with the same number of loads in flight on every path the compiler
-    // waits for the current chunk's data only, not for the prefetch.
-    SlotData cur, nxt;
-    load_slots(d, qo, min(tid, nq - 1), cur);
     __syncthreads();
-    for (int r = 0; r < s.H; ++r) {
-        const int rs = row_start[r], re = row_start[r + 1];
-        if (rs == re) continue;
-        uint32_t carry = kSU * 3 + kSU;
-        for (int base = rs; base < re; base += kSolveThreads) {
-            const int j = base + tid;
-            const int jn = ((base + kSolveThreads < re) ? base + kSolveThreads : re) + tid;
-            load_slots(d, qo, min(jn, nq - 1), nxt);
-            TMap m = j < re ? pixel_map(cur, j, rs, fin) : TMap{kIdLo, kIdHi, kIdX};
-            m = tmap_wave_scan(m);  // m := m_j o ... o m_(first of this wavefront)
-            if (lane == kWave - 1) wtot[w] = m;
-            __syncthreads();
-            // Prefixes of the wavefront totals: lane q scans wtot[q] o ... o
-            // wtot[0] with DPP row shifts (the totals sit in lanes 0..7 of
-            // row 0); pre is the prefix of the earlier wavefronts, chain
-            // that of all of them.
-            TMap t = lane < kSolveWaves ? wtot[lane] : TMap{kIdLo, kIdHi, kIdX};
-            t = tmap_compose(t, tmap_dpp<0x111, 0xf>(t));
-            t = tmap_compose(t, tmap_dpp<0x112, 0xf>(t));
-            t = tmap_compose(t, tmap_dpp<0x114, 0xf>(t));
-            static_assert(kSolveWaves <= 8, "three row-shift steps scan at most 8 wavefront totals");
-            const TMap chain = tmap_readlane(t, kSolveWaves - 1);
-            if (w > 0) m = tmap_compose(m, tmap_readlane(t, w - 1));
-            const uint32_t ps = tmap_get(m, carry);
-            if (j < re) fin[kFinBase + j] = static_cast<uint8_t>(ps % 3);
-            carry = tmap_get(chain, carry);
-            cur = nxt;  // the prefetch has had the whole chunk to land
-            __syncthreads();
+    auto row_at = [&](int r) { return __builtin_amdgcn_readfirstlane(row_start[r]); };
+    // Chunks run in queue order: a row's chunks start at rs, rs + T, ... below
+    // its end re, and the next chunk after the row's last starts at re, the
+    // first queue pixel of the next non-empty row (row_start[H] = nq).
+    int r = 0;
+    while (row_at(r + 1) == 0) ++r;
+    int base = 0;
+    uint32_t carry = kSU * 3 + kSU;
+    // One chunk with the slot data `cur`, prefetching the next chunk's into
+    // `nxt` (its index is clamped to the last queue pixel, so every path has
+    // the same loads in flight and the compiler waits for the current chunk's
+    // data only).  Returns false after the last chunk.
+    auto chunk = [&](const SlotData& cur, SlotData& nxt) -> bool {
+        const int rs = row_at(r), re = row_at(r + 1);
+        const int j = base + tid;
+        const int nb = base + kSolveThreads < re ? base + kSolveThreads : re;
+        load_slots(d, qo, min(nb + tid, nq - 1), nxt);
+        TMap m = j < re ? pixel_map(cur, j, rs, fin) : TMap{kIdLo, kIdHi, kIdX};
+        m = tmap_wave_scan(m);  // m := m_j o ... o m_(first of this wavefront)
+        if (lane == kWave - 1) wtot[w] = m;
+        __syncthreads();
+        // Prefixes of the wavefront totals: lane q scans wtot[q] o ... o
+        // wtot[0] with DPP row shifts (the totals sit in lanes 0..7 of row
+        // 0); pre is the prefix of the earlier wavefronts, chain that of all
+        // of them.
+        TMap t = lane < kSolveWaves ? wtot[lane] : TMap{kIdLo, kIdHi, kIdX};
+        t = tmap_compose(t, tmap_dpp<0x111, 0xf>(t));
+        t = tmap_compose(t, tmap_dpp<0x112, 0xf>(t));
+        t = tmap_compose(t, tmap_dpp<0x114, 0xf>(t));
+        static_assert(kSolveWaves <= 8, "three row-shift steps scan at most 8 wavefront totals");
+        const TMap chain = tmap_readlane(t, kSolveWaves - 1);
+        if (w > 0) m = tmap_compose(m, tmap_readlane(t, w - 1));
+        const uint32_t ps = tmap_get(m, carry);
+        if (j < re) fin[kFinBase + j] = static_cast<uint8_t>(ps % 3);
+        carry = tmap_get(chain, carry);
+        __syncthreads();
+        if (nb < re) {
+            base = nb;
+            return true;
         }
+        if (nb >= nq) return false;
+        ++r;
+        while (row_at(r + 1) == nb) ++r;
+        base = nb;
+        carry = kSU * 3 + kSU;
+        return true;
+    };
+    SlotData sa, sb;
+    load_slots(d, qo, min(tid, nq - 1), sa);
+    while (chunk(sa, sb) && chunk(sb, sa)) {
     }
     for (int j = tid; j < nq; j += kSolveThreads) {
         const uint32_t v = fin[kFinBase + j];
