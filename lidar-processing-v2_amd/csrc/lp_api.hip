@@ -195,6 +195,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.qflag, B * P);
     go(x.qmap, B * P);
     go(x.qlist, B * P);
+    go(x.qrow, B * (L.H + 1));
     go(x.slot_w, B * P * 24);
     go(x.slot_src, B * P * 24);
     go(x.qsum_ok, B * P);

@@ -135,6 +135,7 @@ struct DevBuf {
     int32_t* qmap;
     // JCP queue (frame stride pix_stride)
     int32_t* qlist;
+    int32_t* qrow;           // [frame][H + 1] first queue index of each row
     float* slot_w;           // [q][24]
     int32_t* slot_src;       // [q][24]
     uint8_t* qsum_ok;

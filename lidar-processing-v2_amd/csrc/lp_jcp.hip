@@ -74,7 +74,9 @@ __global__ __launch_bounds__(256) void k_jcp_mark(DevBuf d, Layout L) {
     d.qflag[(size_t)f * L.P + r * W + c] = q;
 }
 
-// qmap holds the exclusive scan of qflag on entry.
+// qmap holds the exclusive scan of qflag on entry.  The scan value at a
+// row's first pixel is the row's first queue index (queue order is raster
+// order), kept for the row solve.
 __global__ void k_jcp_fill(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
@@ -83,6 +85,7 @@ __global__ void k_jcp_fill(DevBuf d, Layout L) {
     if (p >= P) return;
     const size_t o = (size_t)f * L.P + p;
     const int pos = d.qmap[o];
+    if (p % fd.sc.W == 0) d.qrow[(size_t)f * (L.H + 1) + p / fd.sc.W] = pos;
     if (d.qflag[o]) {
         d.qlist[(size_t)f * L.P + pos] = p;
         d.qmap[o] = pos;
@@ -195,32 +198,14 @@ __global__ void k_jcp_slots(DevBuf d, Layout L) {
     // and weights as they are.  Tail slots 12..23 (right neighbours and rows
     // +1, +2) name a neighbour raster-after their writer (the pixel itself
     // or, stale, an earlier queue pixel), so their source is static or a
-    // pending queue pixel: their addends are known here.  They are stored as
-    // the ground list and the obstacle list of nonzero weights in slot order,
-    // zero-padded; dropping +0 addends leaves the ordered sums bit-identical
-    // (every partial sum is >= +0).
-    float tg[12], to[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) tg[k] = to[k] = 0.0f;
-    int ng = 0, no = 0;
-#pragma unroll
-    for (int i = 12; i < 24; ++i) {
-        const bool g = src[i] == kSrcGround && w[i] != 0.0f;
-        const bool o = src[i] == kSrcObstacle && w[i] != 0.0f;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            tg[k] = (g && ng == k) ? w[i] : tg[k];
-            to[k] = (o && no == k) ? w[i] : to[k];
-        }
-        ng += g ? 1 : 0;
-        no += o ? 1 : 0;
-    }
+    // pending queue pixel and their addends are known here: the ground
+    // addend (w or +0) and the obstacle addend of each tail slot.
     float* wo = d.slot_w + ((size_t)f * L.P + j) * 24;
     int32_t* so = d.slot_src + ((size_t)f * L.P + j) * 24;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        wo[k] = tg[k];
-        wo[12 + k] = to[k];
+        wo[k] = src[12 + k] == kSrcGround ? w[12 + k] : 0.0f;
+        wo[12 + k] = src[12 + k] == kSrcObstacle ? w[12 + k] : 0.0f;
         so[k] = src[k];
         so[12 + k] = __float_as_int(w[k]);
     }
@@ -297,7 +282,7 @@ static_assert(kSrcObstacle + kFinBase == 0 && kSrcGround + kFinBase == 1 && kSrc
               "static sources map to the first three label entries");
 
 // Slot data of one queue pixel, as written by k_jcp_slots: w holds the
-// tail's ground addend list (entries 0..11) and obstacle addend list (12..23),
+// tail slots' ground addends (entries 0..11) and obstacle addends (12..23),
 // s the head slots' sources (0..11) and weight bit patterns (12..23), ok the
 // sum test.  None of it depends on solved labels, so the solve loads the next
 // chunk's while it solves the current one.
@@ -379,12 +364,10 @@ __device__ __forceinline__ TMap pixel_map(const SlotData& sd, int j, int rs, con
     float g3 = g1 + hw[11];
     float o0 = wo0, o1 = wo0 + hw[10], o2 = wo0 + hw[11];
     float o3 = o1 + hw[11];
-    // The tail (slots 12..23) in slot order: the lists hold the nonzero
-    // addends only, so the loops end once every lane's list has ended.
+    // The tail (slots 12..23) in slot order, its addends precomputed.
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
         const float a = tw[k];
-        if (__builtin_amdgcn_ballot_w64(a != 0.0f) == 0) break;
         g0 += a;
         g1 += a;
         g2 += a;
@@ -393,7 +376,6 @@ __device__ __forceinline__ TMap pixel_map(const SlotData& sd, int j, int rs, con
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
         const float a = tw[12 + k];
-        if (__builtin_amdgcn_ballot_w64(a != 0.0f) == 0) break;
         o0 += a;
         o1 += a;
         o2 += a;
@@ -440,8 +422,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
     }
     if (nq == 0) return;
     const size_t qo = (size_t)f * L.P;
-    const int32_t* ql = d.qlist + qo;
-    for (int r = tid; r <= s.H; r += kSolveThreads) row_start[r] = lower_bound_i32(ql, nq, r * s.W);
+    for (int r = tid; r <= s.H; r += kSolveThreads) row_start[r] = r < s.H ? d.qrow[(size_t)f * (L.H + 1) + r] : nq;
     if (tid < kFinBase) fin[tid] = static_cast<uint8_t>(tid == 0 ? kSO : (tid == 1 ? kSG : kSU));
     __syncthreads();
     auto row_at = [&](int r) { return __builtin_amdgcn_readfirstlane(row_start[r]); };
