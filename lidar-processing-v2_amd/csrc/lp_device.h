@@ -280,6 +280,29 @@ __device__ __forceinline__ uint32_t wave_atomic_inc(uint32_t* ctr) {
 // Bitonic sort (ascending) of 64*S u32 keys held in registers, element
 // e = s * 64 + lane in slot s: partners within a lane take min / max of two
 // registers, others exchange through lane shuffles.
+// Lane (lane ^ j) of v.  Partners inside a 16-lane row move by DPP (no LDS
+// round trip): quad permutes for j = 1, 2, the rotation by 8 for j = 8, and
+// for j = 4 the rotations by 4 and 12 (lane i reads lane (i - 4) mod 16 and
+// (i + 4) mod 16), picked by lane bit 2.  Larger j go through a shuffle.
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int j) {
+    const int x = static_cast<int>(v);
+    switch (j) {
+        case 1:
+            return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xf, 0xf, false));
+        case 2:
+            return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xf, 0xf, false));
+        case 4: {
+            const int a = __builtin_amdgcn_update_dpp(x, x, 0x124, 0xf, 0xf, false);
+            const int b = __builtin_amdgcn_update_dpp(x, x, 0x12C, 0xf, 0xf, false);
+            return static_cast<uint32_t>((lane_id() & 4) ? a : b);
+        }
+        case 8:
+            return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x128, 0xf, 0xf, false));
+        default:
+            return __shfl_xor(v, j, kWave);
+    }
+}
+
 template <int S>
 __device__ __forceinline__ void reg_sort_u32(uint32_t (&key)[S]) {
     const int lane = lane_id();
@@ -301,7 +324,7 @@ __device__ __forceinline__ void reg_sort_u32(uint32_t (&key)[S]) {
             } else {
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const uint32_t other = __shfl_xor(key[s], j, kWave);
+                    const uint32_t other = lane_xor(key[s], j);
                     const bool asc = ((s * kWave + lane) & k) == 0;
                     const bool lower = (lane & j) == 0;
                     key[s] = (lower == asc) ? min(key[s], other) : max(key[s], other);
