@@ -225,6 +225,17 @@ int lp_pca_boxes(lp_context* ctx, const double* x, const double* y, const int32_
  * fourth-to-first azimuth quadrant transition). */
 int lp_add_ring_info(lp_context* ctx, const float* x, const float* y, int32_t n, uint16_t* ring);
 
+/* Dataloader::addRingInfo without allocating anything: x, y (n floats) and
+ * ring (n uint16) are the caller's device arrays; the work is queued on
+ * `stream` (a hipStream_t, NULL = the default stream) and not waited for.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops that forbid hipMalloc).  The
+ * per-tile transition counts live in a fixed device array of 16384 entries
+ * (64 KiB on each device, held until the process ends), so n is limited to
+ * 16777216 points (else LP_ERR_CAPACITY); calls that overlap on one device
+ * race on that array.  Host pointers give LP_ERR_INVALID_ARGUMENT. */
+int lp_add_ring_info_noalloc(const float* x, const float* y, int32_t n, uint16_t* ring, void* stream);
+
 /* Fused per-frame path of the processor: segment -> obstacle split (input
  * order, processor.cpp:562-579) -> cluster -> per-cluster gather -> hull. */
 typedef struct lp_frame_outputs {

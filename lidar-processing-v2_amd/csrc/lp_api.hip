@@ -37,6 +37,8 @@ int ceil_log2(int64_t v) {
 
 namespace lp {
 
+int set_last_error(int code, const char* msg) { return fail(code, msg); }
+
 void stage_mark(lp_context* c, const char* name) {
     if (!c->profiling) return;
     hipEvent_t e;

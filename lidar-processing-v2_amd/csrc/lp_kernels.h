@@ -6,6 +6,9 @@
 
 namespace lp {
 
+// Stores msg as the calling thread's lp_last_error() text; returns code.
+int set_last_error(int code, const char* msg);
+
 // Records a named stage boundary on the context stream when profiling.
 void stage_mark(lp_context* c, const char* name);
 

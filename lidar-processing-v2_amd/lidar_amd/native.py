@@ -30,7 +30,7 @@ EXPORTED = [
     "lp_remove_noise", "lp_oriented_boxes", "lp_set_noise_filter", "lp_set_oriented_boxes",
     "lp_server_config_default", "lp_server_create", "lp_server_destroy", "lp_server_submit", "lp_server_poll",
     "lp_server_stats_get", "lp_cloud_from_pointcloud2", "lp_pca_boxes", "lp_set_label_clouds",
-    "lp_set_vehicle_boxes", "lp_frame_image",
+    "lp_set_vehicle_boxes", "lp_frame_image", "lp_add_ring_info_noalloc",
 ]
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
