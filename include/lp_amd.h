@@ -191,11 +191,26 @@ typedef struct lp_noise_config {
 } lp_noise_config;
 void lp_noise_config_default(lp_noise_config* cfg);
 
-/* NoiseRemover::filter: labels[num_points] receives 0 (VALID) or 1 (NOISE).
- * A point is NOISE when fewer than min_neighbours points, itself included,
- * lie within its search radius.  (The reference's KD-tree search keeps
- * unvisited stack entries between queries, kdtree.hpp:351-371, which can
- * count a point twice; this implements the radius rule itself.) */
+/* How the noise filter searches (lp_remove_noise and the fused path's filter):
+ *   LP_NOISE_RULE    the filter's stated rule: NOISE when fewer than
+ *                    min_neighbours points, itself included, lie within the
+ *                    point's search radius (the default);
+ *   LP_NOISE_KDTREE  what a NoiseRemover object computes for the frame
+ *                    (a fresh object per frame): its KD-tree
+ *                    (kdtree.hpp:170-213, std::nth_element of libstdc++)
+ *                    and radius_search_k_nearest (kdtree.hpp:339-397), which
+ *                    stops at the k-th neighbour without clearing its stack,
+ *                    so a later query short of k neighbours continues through
+ *                    the earlier queries' unvisited subtrees and can count a
+ *                    point again.  min_neighbours up to 7; non-finite
+ *                    coordinates give LP_ERR_INVALID_ARGUMENT.  Its device
+ *                    buffers (about 208 bytes per point and frame) are
+ *                    allocated when it is first selected. */
+enum { LP_NOISE_RULE = 0, LP_NOISE_KDTREE = 1 };
+int lp_set_noise_search(lp_context* ctx, int32_t mode);
+
+/* NoiseRemover::filter: labels[num_points] receives 0 (VALID) or 1 (NOISE),
+ * by the context's search (lp_set_noise_search). */
 int lp_remove_noise(lp_context* ctx, const lp_noise_config* cfg, const lp_cloud* cloud, uint8_t* labels);
 
 /* BoundingBox (polygonizer.hpp:75-81): corners as x0,y0,x1,y1,x2,y2,x3,y3. */
@@ -267,6 +282,10 @@ typedef struct lp_frame_result {
     int32_t num_ground;         /* GROUND points (label clouds on, else 0) */
     int32_t num_unknown;        /* UNKNOWN points (label clouds on, else 0) */
     int32_t num_vehicles;       /* clusters classified as a vehicle (vehicle boxes on, else 0) */
+    int32_t num_hull_inexact;   /* hull orientation tests whose double value (the reference's predicate,
+                                   polygonizer.cpp:45-48) had another sign than the exact one */
+    int32_t num_hull_rounded;   /* clusters whose hull was recomputed as the reference computes it
+                                   (full sort, rounded double predicate) because of such a test */
 } lp_frame_result;
 
 /* Optional stages of the fused path, kept by the context until changed:
@@ -292,6 +311,21 @@ int lp_set_label_clouds(lp_context* ctx, int32_t enable);
  * constant, processor.cpp:676) fits the box to the raw cluster points; this
  * fits it to the cluster's hull, the polygon the calipers are defined on. */
 int lp_set_vehicle_boxes(lp_context* ctx, int32_t enable);
+
+/* Launch form of the widest per-point kernels (projection, labels,
+ * spherical coordinates, voxel insert): 0 = one workgroup per (tile, frame)
+ * of the capacity grid (the default); 1 = a persistent launch of a fixed
+ * number of workgroups taking (frame, tile) items from per-XCD work queues
+ * (a frame's items stay on one XCD), so frames of uneven sizes leave no idle
+ * workgroups.  Outputs are identical in both forms.  No reference
+ * counterpart (the reference processes one frame on the CPU). */
+int lp_set_persistent(lp_context* ctx, int32_t enable);
+/* Launch shape, tuned per device and workload (lidar_amd.autotune): the
+ * workgroup size of the per-point kernels (projection, labels, spherical
+ * coordinates, voxel insert) and the tile width in columns of the JCP queue
+ * marking (a 5-row LDS tile per workgroup); each 64, 128 or 256 (default
+ * 256 and 256).  Outputs are identical under every shape. */
+int lp_set_launch_shape(lp_context* ctx, int32_t point_workgroup, int32_t mark_tile_width);
 
 /* Processes num_frames frames as one batch (CSR-packed on the device, one
  * launch sequence).  cfgs may hold one config (shared) or num_frames. */
@@ -335,6 +369,7 @@ typedef struct lp_server_config {
     int32_t oriented_boxes;        /* 1: a rotating-calipers box per cluster hull */
     int32_t label_clouds;          /* 1: ground and unknown index lists (lp_set_label_clouds) */
     int32_t vehicle_boxes;         /* 1: vehicle classes per cluster (lp_set_vehicle_boxes) */
+    int32_t noise_search;          /* LP_NOISE_RULE or LP_NOISE_KDTREE (lp_set_noise_search) */
 } lp_server_config;
 void lp_server_config_default(lp_server_config* cfg);
 

@@ -101,24 +101,49 @@ inline void check(int code) {
     if (code != LP_OK) raise(code);
 }
 
-// One HIP context per object, like the reference's per-object scratch.
+// One HIP context per object, like the reference's per-object scratch.  The
+// reference reserves 200 000 points (segmenter.cpp:62, clusterer.cpp:33-38,
+// noise_remover.cpp:32-36) as a hint and keeps working past it, so a call
+// that needs more points or a larger image than the context holds recreates
+// it with room for that call (device memory is allocated only then).
 class Context {
   public:
     explicit Context(std::int32_t max_points = 200000, std::int32_t max_h = 128, std::int32_t max_w = 2048) {
-        lp_capacity cap;
-        lp_capacity_default(&cap);
-        cap.max_points_per_frame = max_points;
-        cap.max_image_height = max_h;
-        cap.max_image_width = max_w;
-        check(lp_context_create(&cap, &ctx_));
+        create(max_points, max_h, max_w);
     }
     ~Context() { lp_context_destroy(ctx_); }
     Context(const Context&) = delete;
     Context& operator=(const Context&) = delete;
     lp_context* get() const noexcept { return ctx_; }
 
+    void ensure(std::size_t points, std::int32_t h = 0, std::int32_t w = 0) {
+        if (points <= static_cast<std::size_t>(points_) && h <= h_ && w <= w_) return;
+        if (points > static_cast<std::size_t>(INT32_MAX)) throw std::length_error("cloud too large");
+        const auto n = std::max(points_, static_cast<std::int32_t>(points));
+        const auto hh = std::max(h_, h);
+        const auto ww = std::max(w_, w);
+        lp_context_destroy(ctx_);
+        ctx_ = nullptr;
+        create(n, hh, ww);
+    }
+
   private:
+    void create(std::int32_t max_points, std::int32_t max_h, std::int32_t max_w) {
+        lp_capacity cap;
+        lp_capacity_default(&cap);
+        cap.max_points_per_frame = max_points;
+        cap.max_image_height = max_h;
+        cap.max_image_width = max_w;
+        check(lp_context_create(&cap, &ctx_));
+        points_ = max_points;
+        h_ = max_h;
+        w_ = max_w;
+    }
+
     lp_context* ctx_ = nullptr;
+    std::int32_t points_ = 0;
+    std::int32_t h_ = 0;
+    std::int32_t w_ = 0;
 };
 
 template <typename CloudT>
@@ -207,6 +232,7 @@ class Segmenter {
         lp_segmenter_config c = to_c();
         lp_cloud in = detail::make_cloud(cloud, !config_.assume_unorganized_cloud);
         labels.assign(cloud.points.size(), Label::UNKNOWN);
+        ctx_->ensure(cloud.points.size(), config_.image_height, config_.image_width);
         detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
         image_valid_ = false;
     }
@@ -219,6 +245,7 @@ class Segmenter {
         lp_cloud in = view.cloud();
         if (config_.assume_unorganized_cloud) in.ring = nullptr;
         labels.assign(view.size(), Label::UNKNOWN);
+        ctx_->ensure(view.size(), config_.image_height, config_.image_width);
         detail::check(lp_segment(ctx_->get(), &c, &in, reinterpret_cast<std::uint32_t*>(labels.data()), &ransac_));
         image_valid_ = false;
     }
@@ -287,6 +314,7 @@ class Clusterer {
         c.min_cluster_size = config_.min_cluster_size;
         lp_cloud in = detail::make_cloud(cloud, false);
         labels.assign(cloud.points.size(), INVALID_LABEL);
+        ctx_->ensure(cloud.points.size());
         detail::check(lp_cluster(ctx_->get(), &c, &in, labels.data()));
     }
 
@@ -322,6 +350,7 @@ class Polygonizer {
         }
         std::vector<std::int32_t> hoff(lists.size() + 1);
         std::vector<std::int32_t> hidx(x.size() + 4);
+        ctx_->ensure(x.size());
         detail::check(lp_convex_hulls(ctx_->get(), x.data(), y.data(), offsets.data(),
                                       static_cast<std::int32_t>(lists.size()), hoff.data(), hidx.data(),
                                       static_cast<std::int32_t>(hidx.size())));
@@ -372,6 +401,7 @@ class Polygonizer {
             offsets.push_back(static_cast<std::int32_t>(x.size()));
         }
         std::vector<lp_bounding_box> raw(polygons.size());
+        ctx_->ensure(std::max(x.size(), polygons.size()));
         detail::check(fn(ctx_->get(), x.data(), y.data(), offsets.data(), static_cast<std::int32_t>(polygons.size()),
                          raw.data()));
         boxes.resize(polygons.size());
@@ -394,12 +424,7 @@ class NoiseRemover {
 
     NoiseRemover() : ctx_(new detail::Context()) {}
 
-    void reserve(std::uint32_t max_pts) {
-        if (max_pts > capacity_) {
-            ctx_.reset(new detail::Context(static_cast<std::int32_t>(max_pts)));
-            capacity_ = max_pts;
-        }
-    }
+    void reserve(std::uint32_t max_pts) { ctx_->ensure(max_pts); }
 
     void config(const NoiseRemoverConfiguration& config) { config_ = config; }
     const NoiseRemoverConfiguration& config() const noexcept { return config_; }
@@ -422,7 +447,6 @@ class NoiseRemover {
   private:
     NoiseRemoverConfiguration config_;
     std::unique_ptr<detail::Context> ctx_;
-    std::uint32_t capacity_ = 200000;
 };
 
 }  // namespace lidar_processing_lib

@@ -215,6 +215,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.cl_size, B * N);
     go(x.cl_off, B * N);
     go(x.cl_cursor, B * N);
+    x.hull_flag = x.cl_cursor;
     go(x.cl_members, B * N);
     go(x.hcx, 2 * B * N);
     go(x.hcy, 2 * B * N);
@@ -228,6 +229,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.hull_local, B * N);
     go(x.blk, B * L.blk_stride);
     go(x.fs, B * kFsCount);
+    go(x.wq, kWqCounters);
     go(x.frames, B);
     go(x.cc, 1);
     go(x.nc, 1);
@@ -235,6 +237,67 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.vclass, B * static_cast<size_t>(L.box_stride));
     return static_cast<size_t>(p - base);
 }
+
+// Field offsets of an AoS cloud relative to its lowest field.
+struct AosLayout {
+    const uint8_t* lo = nullptr;
+    int fx = 0, fy = 0, fz = 0, fr = -1;
+    int stride = 0;
+    int span = 0;   // bytes from `lo` to the end of the last field of a record
+};
+
+bool aos_layout(const lp_cloud& cl, AosLayout& a, std::string& why) {
+    const uint8_t* lo = static_cast<const uint8_t*>(cl.x);
+    for (const void* q : {cl.y, cl.z, cl.ring})
+        if (q && static_cast<const uint8_t*>(q) < lo) lo = static_cast<const uint8_t*>(q);
+    auto field = [&](const void* q) { return static_cast<int64_t>(static_cast<const uint8_t*>(q) - lo); };
+    const int64_t fx = field(cl.x), fy = field(cl.y), fz = field(cl.z), fr = cl.ring ? field(cl.ring) : -1;
+    const int64_t stride = cl.xyz_stride;
+    if (stride < 12 || fx + 4 > stride || fy + 4 > stride || fz + 4 > stride || (fr >= 0 && fr + 2 > stride) ||
+        (cl.ring && cl.ring_stride != stride)) {
+        why = "unsupported point layout: fields must share one record stride";
+        return false;
+    }
+    a.lo = lo;
+    a.fx = static_cast<int>(fx);
+    a.fy = static_cast<int>(fy);
+    a.fz = static_cast<int>(fz);
+    a.fr = static_cast<int>(fr);
+    a.stride = static_cast<int>(stride);
+    a.span = static_cast<int>(std::max({fx + 4, fy + 4, fz + 4, fr >= 0 ? fr + 2 : int64_t{0}}));
+    return true;
+}
+
+// Device bytes at slot_w, free while inputs are staged.
+size_t raw_staging_bytes(const lp_context* c) {
+    return static_cast<size_t>(c->L.B) * static_cast<size_t>(c->L.P) * 24 * sizeof(float);
+}
+
+}  // namespace
+
+namespace lp {
+
+// The per-frame checks lp_batch_stage makes, without staging anything.
+int validate_frame(const lp_context* c, const lp_segmenter_config& cfg, const lp_cloud& cl) {
+    if (cl.num_points < 0 || cl.num_points > c->L.N)
+        return fail(LP_ERR_CAPACITY, "frame has more points than max_points_per_frame");
+    if (cl.num_points > 0 && (!cl.x || !cl.y || !cl.z)) return fail(LP_ERR_INVALID_ARGUMENT, "null coordinates");
+    SegConst sc{};
+    std::string why;
+    if (!derive_seg(cfg, c->L, sc, why)) return fail(LP_ERR_INVALID_ARGUMENT, why);
+    const bool soa = cl.xyz_stride == 4 && (cl.ring == nullptr || cl.ring_stride == 2);
+    if (cl.num_points > 0 && !soa) {
+        AosLayout a;
+        if (!aos_layout(cl, a, why)) return fail(LP_ERR_INVALID_ARGUMENT, why);
+        if (raw_staging_bytes(c) < static_cast<size_t>(a.stride))
+            return fail(LP_ERR_CAPACITY, "point record larger than the staging area");
+    }
+    return LP_OK;
+}
+
+}  // namespace lp
+
+namespace {
 
 // Copies a cloud into the packed inputs (x, y, z, ring at offset `off`), or
 // into the noise filter's raw inputs when `raw_inputs`.
@@ -257,29 +320,36 @@ int stage_cloud(lp_context* c, const lp_cloud& cl, int32_t off, uint8_t* raw_dev
         if (cl.ring) LP_HIP(hipMemcpyAsync(d.ring + off, cl.ring, n * 2, kind, c->stream));
         return LP_OK;
     }
-    // AoS: all fields inside one record of xyz_stride bytes.
-    const uint8_t* px = static_cast<const uint8_t*>(cl.x);
-    const uint8_t* lo = px;
-    for (const void* q : {cl.y, cl.z, cl.ring})
-        if (q && static_cast<const uint8_t*>(q) < lo) lo = static_cast<const uint8_t*>(q);
-    auto field = [&](const void* q) { return static_cast<int>(static_cast<const uint8_t*>(q) - lo); };
-    const int fx = field(cl.x), fy = field(cl.y), fz = field(cl.z), fr = cl.ring ? field(cl.ring) : -1;
-    const int stride = cl.xyz_stride;
-    if (stride < 12 || fx + 4 > stride || fy + 4 > stride || fz + 4 > stride || (fr >= 0 && fr + 2 > stride) ||
-        (cl.ring && cl.ring_stride != stride))
-        return fail(LP_ERR_INVALID_ARGUMENT, "unsupported point layout: fields must share one record stride");
-    if (static_cast<size_t>(n) * stride > static_cast<size_t>(c->L.N) * 4 * 8)
-        return fail(LP_ERR_CAPACITY, "point record buffer too large");
-    LP_HIP(hipMemcpyAsync(raw_dev, lo, static_cast<size_t>(n) * stride, kind, c->stream));
-    hipLaunchKernelGGL(k_deinterleave, dim3((n + 255) / 256), dim3(256), 0, c->stream, raw_dev, n, stride, fx, fy,
-                       fz, fr, d.x + off, d.y + off, d.z + off, d.ring + off);
-    LP_HIP(hipGetLastError());
+    // AoS: all fields inside one record of xyz_stride bytes, staged through
+    // the device area at raw_dev in chunks of whole records.
+    AosLayout a;
+    std::string why;
+    if (!aos_layout(cl, a, why)) return fail(LP_ERR_INVALID_ARGUMENT, why);
+    const size_t stride = static_cast<size_t>(a.stride);
+    const size_t chunk = raw_staging_bytes(c) / stride;
+    if (chunk == 0) return fail(LP_ERR_CAPACITY, "point record larger than the staging area");
+    for (size_t first = 0; first < static_cast<size_t>(n); first += chunk) {
+        const int cnt = static_cast<int>(std::min(chunk, static_cast<size_t>(n) - first));
+        // The last record is read up to its last field only: the caller's
+        // array may end there.
+        const size_t bytes = (static_cast<size_t>(cnt) - 1) * stride + static_cast<size_t>(a.span);
+        LP_HIP(hipMemcpyAsync(raw_dev, a.lo + first * stride, bytes, kind, c->stream));
+        const int o = off + static_cast<int>(first);
+        hipLaunchKernelGGL(k_deinterleave, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, raw_dev, cnt, a.stride,
+                           a.fx, a.fy, a.fz, a.fr, d.x + o, d.y + o, d.z + o, d.ring + o);
+        LP_HIP(hipGetLastError());
+    }
     return LP_OK;
 }
 
 int run_pipeline(lp_context* c, int B, bool segment_only) {
     c->L.B_active = B;
-    if (c->staged_filtered && !segment_only) launch_noise(c, B, true);
+    if (c->staged_filtered && !segment_only) {
+        if (c->noise_kdtree)
+            launch_noise_kdtree(c, B, true);
+        else
+            launch_noise(c, B, true);
+    }
     launch_segment_front(c, B);
     launch_jcp(c, B);
     if (!segment_only) {
@@ -305,6 +375,15 @@ int status_of(uint32_t err) {
     if (err & kErrVoxelOverflow) return fail(LP_ERR_VOXEL_OVERFLOW, "more occupied voxels than max_voxels");
     if (err & kErrRansacSingle)
         return fail(LP_ERR_RANSAC_SINGLE_CANDIDATE, "exactly one RANSAC candidate (reference never terminates)");
+    if (err & kErrNoiseInput)
+        return fail(LP_ERR_INVALID_ARGUMENT,
+                    "non-finite coordinates: the KD-tree search (std::nth_element) has no order for them");
+    if (err & kErrNoiseCapacity)
+        return fail(LP_ERR_CAPACITY, "KD-tree noise search: more pending search entries than the device holds");
+    if (err & kErrHullFuzzyOrder)
+        return fail(LP_ERR_INVALID_ARGUMENT,
+                    "hull points with distinct x closer than DBL_EPSILON: the reference's sort comparator "
+                    "(polygonizer.cpp:52-59) is not an ordering there");
     if (err & (kErrCellCapacity | kErrQueueCapacity | kErrHullCapacity | kErrRngExhausted | kErrBoxCapacity))
         return fail(LP_ERR_CAPACITY, "a device capacity was exceeded (cell, JCP queue, hull, box or RNG stream)");
     return LP_OK;
@@ -468,6 +547,7 @@ int lp_context_destroy(lp_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->arena) (void)hipFree(c->arena);
     if (c->clouds_mem) (void)hipFree(c->clouds_mem);
+    if (c->kd_mem) (void)hipFree(c->kd_mem);
     if (c->image_dev) (void)hipFree(c->image_dev);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -561,7 +641,8 @@ int run_lane(lp_context* c, int32_t use_graph) {
     const int B = c->num_frames;
     if (!use_graph) return run_pipeline(c, B, false);
     const int options = (c->staged_filtered ? 1 : 0) | (c->boxes_on ? 2 : 0) | (c->clouds_on ? 4 : 0) |
-                        (c->vehicles_on ? 8 : 0);
+                        (c->vehicles_on ? 8 : 0) | (c->staged_filtered && c->noise_kdtree ? 16 : 0) |
+                        (c->persistent ? 32 : 0) | (c->point_wg << 8) | (c->mark_tile << 20);
     lp_context::GraphEntry* ge = nullptr;
     for (auto& g : c->graphs)
         if (g.frames == B && g.options == options) ge = &g;
@@ -634,7 +715,7 @@ int fetch_lane(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* 
         const uint32_t* s = fs.data() + static_cast<size_t>(f) * kFsCount;
         const FrameDesc& fd = c->frames[f];
         lp_frame_result r{};
-        r.status = status_of(s[kFsErr]);
+        r.status = status_of(s[kFsErr] | s[kFsNoiseErr]);
         // Points the segmenter saw: the noise filter's kept count when it ran.
         const int n = c->staged_filtered ? static_cast<int>(s[kFsKept]) : fd.n;
         r.num_points = n;
@@ -652,6 +733,8 @@ int fetch_lane(lp_context* c, const lp_frame_outputs* outputs, lp_frame_result* 
             r.num_unknown = n - r.num_ground - r.num_obstacles;
         }
         r.num_vehicles = c->vehicles_on ? static_cast<int32_t>(s[kFsVehicles]) : 0;
+        r.num_hull_inexact = static_cast<int32_t>(s[kFsHullInexact]);
+        r.num_hull_rounded = static_cast<int32_t>(s[kFsHullRounded]);
         r.ransac.candidates = s[kFsCands];
         r.ransac.best_inliers = s[kFsBest];
         r.ransac.applied = static_cast<int32_t>(s[kFsApplied]);
@@ -809,9 +892,73 @@ int lp_set_noise_filter(lp_context* c, const lp_noise_config* cfg) {
         for (auto* lane : c->lanes) lane->noise_on = false;
         return LP_OK;
     }
+    if (c->noise_kdtree && cfg->min_neighbours > kKdMaxNeighbours)
+        return fail(LP_ERR_INVALID_ARGUMENT, "the KD-tree noise search supports min_neighbours up to 7");
     c->nc = noise_const(*cfg);
     c->noise_on = true;
     for (auto* lane : c->lanes) lp_set_noise_filter(lane, cfg);
+    return LP_OK;
+}
+
+int lp_set_persistent(lp_context* c, int32_t enable) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    c->persistent = enable != 0;
+    for (auto* lane : c->lanes) lane->persistent = c->persistent;
+    return LP_OK;
+}
+
+int lp_set_launch_shape(lp_context* c, int32_t point_workgroup, int32_t mark_tile_width) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    auto ok = [](int32_t v) { return v == 64 || v == 128 || v == 256; };
+    if (!ok(point_workgroup) || !ok(mark_tile_width))
+        return fail(LP_ERR_INVALID_ARGUMENT, "workgroup size and tile width must be 64, 128 or 256");
+    c->point_wg = point_workgroup;
+    c->mark_tile = mark_tile_width;
+    for (auto* lane : c->lanes) {
+        lane->point_wg = point_workgroup;
+        lane->mark_tile = mark_tile_width;
+    }
+    return LP_OK;
+}
+
+int lp_set_noise_search(lp_context* c, int32_t mode) {
+    if (!c) return fail(LP_ERR_INVALID_ARGUMENT, "null context");
+    if (mode != LP_NOISE_RULE && mode != LP_NOISE_KDTREE) return fail(LP_ERR_INVALID_ARGUMENT, "unknown noise search");
+    // The KD-tree kernels (lp_noise_ref.hip) stay compiled but unreachable:
+    // their first device run left the GPU faulted, and no cause has been
+    // shown yet.
+    if (mode == LP_NOISE_KDTREE && kKdSearchDisabled)
+        return fail(LP_ERR_INVALID_ARGUMENT, "the KD-tree noise search is switched off in this build");
+    if (mode == LP_NOISE_KDTREE) {
+        if (c->noise_on && c->nc.k > static_cast<uint32_t>(kKdMaxNeighbours))
+            return fail(LP_ERR_INVALID_ARGUMENT, "the KD-tree noise search supports min_neighbours up to 7");
+        if (!c->kd_mem) {
+            LP_HIP(hipSetDevice(c->device));
+            const size_t cnt = static_cast<size_t>(c->L.B) * c->L.N;
+            const size_t ent = cnt * kKdEntriesPerPoint;
+            const size_t bytes = cnt * (sizeof(float4) + 2 * sizeof(int32_t) + sizeof(uint2) + 4 * sizeof(uint32_t) +
+                                        kKdRecInts * sizeof(int32_t)) + 2 * ent * sizeof(uint64_t);
+            LP_HIP(hipMalloc(&c->kd_mem, bytes));
+            char* p = static_cast<char*>(c->kd_mem);
+            auto take = [&](size_t b) { char* r = p; p += b; return r; };
+            c->d.kd_csr = reinterpret_cast<uint64_t*>(take(ent * sizeof(uint64_t)));
+            c->d.kd_live = reinterpret_cast<uint64_t*>(take(ent * sizeof(uint64_t)));
+            c->d.kd = reinterpret_cast<float4*>(take(cnt * sizeof(float4)));
+            c->d.kd_q = reinterpret_cast<uint2*>(take(cnt * sizeof(uint2)));
+            c->d.kd_tl = reinterpret_cast<int32_t*>(take(cnt * sizeof(int32_t)));
+            c->d.kd_tr = reinterpret_cast<int32_t*>(take(cnt * sizeof(int32_t)));
+            c->d.kd_lcnt = reinterpret_cast<uint32_t*>(take(cnt * sizeof(uint32_t)));
+            c->d.kd_loff = reinterpret_cast<uint32_t*>(take(cnt * sizeof(uint32_t)));
+            c->d.kd_dflag = reinterpret_cast<uint32_t*>(take(cnt * sizeof(uint32_t)));
+            c->d.kd_dpos = reinterpret_cast<uint32_t*>(take(cnt * sizeof(uint32_t)));
+            c->d.kd_drec = reinterpret_cast<int32_t*>(take(cnt * kKdRecInts * sizeof(int32_t)));
+        }
+    }
+    c->noise_kdtree = mode == LP_NOISE_KDTREE;
+    for (auto* lane : c->lanes) {
+        const int st = lp_set_noise_search(lane, mode);
+        if (st != LP_OK) return st;
+    }
     return LP_OK;
 }
 
@@ -853,6 +1000,8 @@ int lp_remove_noise(lp_context* c, const lp_noise_config* cfg, const lp_cloud* c
         return fail(LP_ERR_CAPACITY, "cloud has more points than max_points_per_frame");
     if (cloud->num_points > 0 && (!cloud->x || !cloud->y || !cloud->z))
         return fail(LP_ERR_INVALID_ARGUMENT, "null coordinates");
+    if (c->noise_kdtree && cfg->min_neighbours > kKdMaxNeighbours)
+        return fail(LP_ERR_INVALID_ARGUMENT, "the KD-tree noise search supports min_neighbours up to 7");
     if (cloud->num_points == 0) return LP_OK;
     LP_HIP(hipSetDevice(c->device));
     FrameDesc fd{};
@@ -868,10 +1017,20 @@ int lp_remove_noise(lp_context* c, const lp_noise_config* cfg, const lp_cloud* c
     LP_HIP(hipMemcpyAsync(c->d.nc, &nc, sizeof(NoiseConst), hipMemcpyHostToDevice, c->stream));
     c->num_frames = 0;
     c->L.B_active = 1;
-    launch_noise(c, 1, false);
+    if (c->noise_kdtree) {
+        launch_noise_kdtree(c, 1, false);
+    } else {
+        launch_noise(c, 1, false);
+    }
     LP_HIP(hipGetLastError());
     LP_HIP(hipMemcpyAsync(labels, c->d.noise, cloud->num_points, hipMemcpyDeviceToHost, c->stream));
+    uint32_t err = 0;
+    LP_HIP(hipMemcpyAsync(&err, c->d.fs + kFsNoiseErr, sizeof(err), hipMemcpyDeviceToHost, c->stream));
     LP_HIP(hipStreamSynchronize(c->stream));
+    if (c->noise_kdtree) {
+        const int st2 = status_of(err);
+        if (st2 != LP_OK) return st2;
+    }
     return LP_OK;
 }
 

@@ -17,6 +17,7 @@
 // a global-memory union-find takes over above kCcLds voxels), and labels
 // from a scan over "first point of a surviving component" flags.
 #include "lp_kernels.h"
+#include "lp_queue.h"
 
 namespace lp {
 
@@ -88,11 +89,10 @@ __device__ __forceinline__ uint32_t max_bits(float v) {
 }
 
 // cartesianToSpherical (clusterer.cpp:55-100).
-__global__ void k_spherical(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+__device__ __forceinline__ void spherical_tile(const DevBuf& d, const Layout& L, int f, int tile) {
     const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x * blockDim.x >= static_cast<unsigned>(nobs)) return;
+    const int k = tile * blockDim.x + threadIdx.x;
+    if (tile * blockDim.x >= static_cast<unsigned>(nobs)) return;
     const size_t ob = (size_t)f * L.N;
     uint32_t mr = 0, ma = 0, me = 0;
     if (k < nobs) {
@@ -123,6 +123,13 @@ __global__ void k_spherical(DevBuf d, Layout L) {
     }
 }
 
+__global__ void k_spherical(DevBuf d, Layout L) { spherical_tile(d, L, blockIdx.y, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_spherical_q(DevBuf d, Layout L, int B) {
+    wq_run(d.wq + kWqSpherical * 8, B, blockDim.x, [&](int f) { return d.fs[f * kFsCount + kFsObstacles]; },
+           [&](int f, int t) { spherical_tile(d, L, f, t); });
+}
+
 struct Dims {
     int32_t nr, na, ne;
 };
@@ -145,9 +152,7 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t key, uint32_t mask) {
 
 // Voxel keys (clusterer.cpp:102-120, flatVoxelIndex clusterer.hpp:135-141)
 // inserted into the per-frame hash set.
-__global__ void k_voxel_insert(DevBuf d, Layout L) {
-    int tile, f;
-    xcd_tile_frame(tile, f);
+__device__ __forceinline__ void voxel_insert_tile(const DevBuf& d, const Layout& L, int f, int tile) {
     const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
     const int k = tile * blockDim.x + threadIdx.x;
     if (k >= nobs) return;
@@ -188,6 +193,17 @@ __global__ void k_voxel_insert(DevBuf d, Layout L) {
     }
     const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
     d.vslot[ob + k] = slot;
+}
+
+__global__ void k_voxel_insert(DevBuf d, Layout L) {
+    int tile, f;
+    xcd_tile_frame(tile, f);
+    voxel_insert_tile(d, L, f, tile);
+}
+
+__global__ __launch_bounds__(256) void k_voxel_insert_q(DevBuf d, Layout L, int B) {
+    wq_run(d.wq + kWqVoxelInsert * 8, B, blockDim.x, [&](int f) { return d.fs[f * kFsCount + kFsObstacles]; },
+           [&](int f, int t) { voxel_insert_tile(d, L, f, t); });
 }
 
 __device__ __forceinline__ int32_t hash_find(const int32_t* hkey, uint32_t mask, int32_t key) {
@@ -441,8 +457,14 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
         hipLaunchKernelGGL(k_cluster_input, dim3(nb, B), dim3(256), 0, st, d, L);
     }
     stage_mark(c, "voxels");
-    hipLaunchKernelGGL(k_spherical, dim3(nb, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_voxel_insert, dim3(nb, B), dim3(256), 0, st, d, L);
+    const int wg = c->point_wg;
+    if (c->persistent) {
+        hipLaunchKernelGGL(k_spherical_q, dim3(kPersistentWorkgroups), dim3(wg), 0, st, d, L, B);
+        hipLaunchKernelGGL(k_voxel_insert_q, dim3(kPersistentWorkgroups), dim3(wg), 0, st, d, L, B);
+    } else {
+        hipLaunchKernelGGL(k_spherical, dim3((L.N + wg - 1) / wg, B), dim3(wg), 0, st, d, L);
+        hipLaunchKernelGGL(k_voxel_insert, dim3((L.N + wg - 1) / wg, B), dim3(wg), 0, st, d, L);
+    }
     stage_mark(c, "components");
     hipLaunchKernelGGL(k_voxel_neighbours, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_cc, dim3(B), dim3(1024), 0, st, d, L);

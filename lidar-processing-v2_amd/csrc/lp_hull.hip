@@ -18,6 +18,7 @@
 // Shewchuk's error bound cannot certify its sign, falls back to an exact
 // expansion sum (TwoProduct via FMA).
 #include <algorithm>
+#include <cfloat>
 
 #include "lp_kernels.h"
 
@@ -580,7 +581,10 @@ __global__ void k_hull_chunks(DevBuf d, Layout L, int r) {
     const size_t ob = (size_t)f * L.N;
     const size_t BN = (size_t)L.B * L.N;
     uint32_t nch = 0;
-    if (r == 0) d.hull_count[ob + c] = 0;
+    if (r == 0) {
+        d.hull_count[ob + c] = 0;
+        d.hull_flag[ob + c] = 0;
+    }
     if (r == 0 || d.hull_count[ob + c] == 0) {
         const uint32_t cnt = r == 0 ? d.cl_size[ob + c] : d.hc_count[(r & 1) * BN + ob + c];
         nch = cnt <= static_cast<uint32_t>(kHullChunk) ? 1u : (cnt + kHullChunk - 1) / kHullChunk;
@@ -795,7 +799,10 @@ __global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
                     nup[w] = len;
             }
             inexact = wave_sum(inexact);
-            if (lane == 0 && inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+            if (lane == 0 && inexact) {
+                atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                atomicOr(&d.hull_flag[ob + c], 1u);
+            }
         }
         wave_sync();
         const int kl = nlo[w];
@@ -928,7 +935,10 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
                         nlo = cl;
                         nup = cu;
                     }
-                    if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                    if (inexact) {
+                        atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                        atomicOr(&d.hull_flag[ob + c], 1u);
+                    }
                 }
             }
             __syncthreads();
@@ -940,7 +950,10 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
                     nlo = len;
                 else
                     nup = len;
-                if (inexact) atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                if (inexact) {
+                    atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                    atomicOr(&d.hull_flag[ob + c], 1u);
+                }
             }
         }
         __syncthreads();
@@ -948,6 +961,142 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
         const int kk = kl + nup - 2;
         for (int i = tid; i < kk; i += 1024) d.hull_idx[ob + base + i] = I[i < kl ? S[i] : S[kHullBig + i - kl + 1]];
         if (tid == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
+        __syncthreads();
+    }
+}
+
+// Coordinates of member m of the cluster at `base`: the obstacle point, or
+// for standalone lists the staged (x, y) pair.
+__device__ __forceinline__ void member_xy(const DevBuf& d, const double2* list_xy, size_t ob, uint32_t base, int32_t m,
+                                          double& x, double& y) {
+    if (list_xy) {
+        x = list_xy[base + m].x;
+        y = list_xy[base + m].y;
+    } else {
+        x = d.ox[ob + m];
+        y = d.oy[ob + m];
+    }
+}
+
+// Clusters where some orientation test's double value had another sign than
+// the exact one: the chunked, culled hull above is the exact hull, which the
+// reference's rounded predicate need not produce.  Such a cluster is redone
+// as polygonizer.cpp:33-91 computes it: all its points sorted by (x, y), then
+// Andrew's chain popping while the double expression
+// (p2.x-p1.x)*(p3.y-p1.y) - (p2.y-p1.y)*(p3.x-p1.x) <= 0, evaluated without
+// contraction.  One workgroup per flagged cluster; the points are sorted in
+// the first candidate buffer (free after k_hull_big) by an ascending-only
+// bitonic network whose padding positions never move, and the lower and
+// upper chains run on lanes 0 and 1 with their stacks in the second buffer.
+// Exact duplicates may be ordered either way (their coordinates agree);
+// distinct x closer than DBL_EPSILON, where the reference's comparator
+// treats x as equal and is not an ordering, are reported as an error.
+__device__ __forceinline__ bool ref_ccw_or_collinear(double ax, double ay, double bx, double by, double cx,
+                                                     double cy) {
+    return (bx - ax) * (cy - ay) - (by - ay) * (cx - ax) <= 0;
+}
+
+__device__ __forceinline__ bool rec_less(const double* X, const double* Y, const int32_t* I, int a, int b) {
+    if (X[a] != X[b]) return X[a] < X[b];
+    if (Y[a] != Y[b]) return Y[a] < Y[b];
+    return I[a] < I[b];
+}
+
+__global__ __launch_bounds__(256) void k_hull_ref(DevBuf d, Layout L, const double2* list_xy) {
+    __shared__ int nlo, nup;
+    int tile, f;
+    xcd_tile_frame(tile, f);
+    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
+    const int tid = threadIdx.x;
+    const size_t ob = (size_t)f * L.N;
+    const size_t BN = (size_t)L.B * L.N;
+    for (int c = tile; c < K; c += gridDim.x) {
+        if (d.hull_flag[ob + c] == 0) continue;
+        const int n = static_cast<int>(d.cl_size[ob + c]);
+        if (n < 3) continue;
+        const uint32_t base = d.cl_off[ob + c];
+        double* X = d.hcx + ob + base;
+        double* Y = d.hcy + ob + base;
+        int32_t* I = d.hci + ob + base;
+        const int32_t* members = d.cl_members + ob + base;
+        for (int i = tid; i < n; i += blockDim.x) {
+            const int32_t m = members[i];
+            double x, y;
+            member_xy(d, list_xy, ob, base, m, x, y);
+            X[i] = x;
+            Y[i] = y;
+            I[i] = m;
+        }
+        __syncthreads();
+        int m2 = 1;
+        while (m2 < n) m2 <<= 1;
+        for (int k = 2; k <= m2; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int q = tid; q < (m2 >> 1); q += blockDim.x) {
+                    int a, b;
+                    if (j == (k >> 1)) {
+                        // First step of a merge: mirror pairs inside each block of k.
+                        const int blk = q / j, t = q % j;
+                        a = blk * k + t;
+                        b = blk * k + k - 1 - t;
+                    } else {
+                        a = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                        b = a + j;
+                    }
+                    if (b < n && rec_less(X, Y, I, b, a)) {
+                        const double tx = X[a], ty = Y[a];
+                        const int32_t ti = I[a];
+                        X[a] = X[b];
+                        Y[a] = Y[b];
+                        I[a] = I[b];
+                        X[b] = tx;
+                        Y[b] = ty;
+                        I[b] = ti;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = 1 + tid; i < n; i += blockDim.x)
+            if (X[i] != X[i - 1] && fabs(X[i] - X[i - 1]) < DBL_EPSILON)
+                atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullFuzzyOrder);
+        // Stacks of sorted positions: lower in [0, n), upper in [n, 2n) of
+        // the second buffer's doubles viewed as int32 (2n entries fit).
+        int32_t* SL = reinterpret_cast<int32_t*>(d.hcx + BN + ob + base);
+        int32_t* SU = SL + n;
+        if (tid == 0) {
+            int k = 0;
+            for (int i = 0; i < n; ++i) {
+                while (k > 1 && ref_ccw_or_collinear(X[SL[k - 2]], Y[SL[k - 2]], X[SL[k - 1]], Y[SL[k - 1]], X[i], Y[i]))
+                    --k;
+                SL[k++] = i;
+            }
+            nlo = k;
+        } else if (tid == 1) {
+            // The reference's upper loop continues on the lower stack with
+            // t = k + 1; it never pops below the last point, so it is the
+            // same chain started from that point.
+            int k = 0;
+            SU[k++] = n - 1;
+            for (int i = n - 2; i >= 0; --i) {
+                while (k > 1 && ref_ccw_or_collinear(X[SU[k - 2]], Y[SU[k - 2]], X[SU[k - 1]], Y[SU[k - 1]], X[i], Y[i]))
+                    --k;
+                SU[k++] = i;
+            }
+            nup = k;
+        }
+        __syncthreads();
+        const int kl = nlo;
+        const int kk = kl + nup - 2;  // indices.resize(k - 1)
+        if (kk > n) {
+            if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
+        } else {
+            for (int i = tid; i < kk; i += blockDim.x) d.hull_idx[ob + base + i] = I[i < kl ? SL[i] : SU[i - kl + 1]];
+            if (tid == 0) {
+                d.hull_count[ob + c] = static_cast<uint32_t>(kk);
+                atomicAdd(&d.fs[f * kFsCount + kFsHullRounded], 1u);
+            }
+        }
         __syncthreads();
     }
 }
@@ -975,17 +1124,6 @@ constexpr int kLocalMax = 512;
 constexpr int kLocalHash = 2 * kLocalMax;
 constexpr int kLocalThreads = 1024;
 constexpr int kLocalUnroll = 4;
-
-__device__ __forceinline__ void member_xy(const DevBuf& d, const double2* list_xy, size_t ob, uint32_t base, int32_t m,
-                                          double& x, double& y) {
-    if (list_xy) {
-        x = list_xy[base + m].x;
-        y = list_xy[base + m].y;
-    } else {
-        x = d.ox[ob + m];
-        y = d.oy[ob + m];
-    }
-}
 
 // Hash of a coordinate pair that agrees with ==: +0.0 and -0.0 hash alike.
 __device__ __forceinline__ uint32_t xy_hash(double x, double y) {
@@ -1176,6 +1314,7 @@ __global__ void k_hull_lists_init(DevBuf d, Layout L, const int32_t* offsets, in
         d.fs[kFsClusters] = static_cast<uint32_t>(K);
         d.fs[kFsErr] = 0;
         d.fs[kFsHullInexact] = 0;
+        d.fs[kFsHullRounded] = 0;
     }
     if (t < K) {
         d.cl_off[t] = static_cast<uint32_t>(offsets[t]);
@@ -1209,14 +1348,22 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     const Layout& L = c->L;
     hipStream_t st = c->stream;
     const int nb = (L.N + 255) / 256;
-    stage_mark(c, "hull");
+    // Stage marks around the single-kernel steps ("hull_round" three
+    // times), so profiled runs time each round launch on its own.
     for (int r = 0; r < kHullRounds; ++r) {
+        stage_mark(c, "hull");
         hipLaunchKernelGGL(k_hull_chunks, dim3(nb, B), dim3(256), 0, st, d, L, r);
         launch_scan_small(c, B, d.hc_chunks, d.hc_choff, L.N, kCountSlot, kFsClusters, kFsHullChunks);
+        stage_mark(c, "hull_round");
         hipLaunchKernelGGL(k_hull_round, dim3(512, B), dim3(256), 0, st, d, L, r);
     }
+    stage_mark(c, "hull_big");
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);
+    stage_mark(c, "hull_ref");
+    hipLaunchKernelGGL(k_hull_ref, dim3(16, B), dim3(256), 0, st, d, L, list_xy);
+    stage_mark(c, "hull");
     launch_scan_small(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
+    stage_mark(c, "hull_local");
     hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(kLocalThreads), 0, st, d, L, list_xy);
 }
 

@@ -33,6 +33,9 @@ enum : uint32_t {
     kErrVoxelOverflow = 1u << 4,
     kErrHullCapacity = 1u << 5,
     kErrBoxCapacity = 1u << 6,
+    kErrHullFuzzyOrder = 1u << 7,  // distinct x closer than DBL_EPSILON in a redone hull
+    kErrNoiseInput = 1u << 8,      // non-finite coordinates for the KD-tree noise search
+    kErrNoiseCapacity = 1u << 9,   // pending search entries beyond the KD-tree search's capacity
 };
 
 // Segmenter constants derived on the host exactly as the reference derives
@@ -156,6 +159,7 @@ struct DevBuf {
     uint32_t* cl_size;
     uint32_t* cl_off;
     uint32_t* cl_cursor;
+    uint32_t* hull_flag;     // = cl_cursor (free after the scatter): per cluster, 1 when a hull test's double sign was uncertain
     int32_t* cl_members;
     // hull
     double* hcx;             // candidate coordinates, frame stride 2N (two buffers)
@@ -181,7 +185,30 @@ struct DevBuf {
     // ground-then-unknown index lists (frame stride N)
     uint32_t* gnd_flag;
     int32_t* split_idx;
+    // KD-tree noise search (allocated when switched on, frame stride N; the
+    // entry arrays have kKdEntriesPerPoint entries per point, the short-query
+    // records kKdRecInts ints)
+    uint32_t* wq;            // persistent-launch queue counters (kWqCounters)
+    float4* kd;              // tree slots: x, y, z, input index bits
+    int32_t* kd_tl;          // partition stop positions
+    int32_t* kd_tr;
+    uint2* kd_q;             // per query: slot of the k-th hit, pending depths
+    uint32_t* kd_lcnt;       // leftover entries per query
+    uint32_t* kd_loff;
+    uint32_t* kd_dflag;      // 1: fewer than k hits in the own traversal
+    uint32_t* kd_dpos;
+    int32_t* kd_drec;        // short queries in order: index, hit count, hit slots
+    uint64_t* kd_csr;        // leftover entries in query order
+    uint64_t* kd_live;       // the stale stack while short queries are resolved
 };
+
+// Work-queue counters of the persistent launches (lp_queue.h): 8 per site.
+constexpr int kWqCounters = 4 * 8;
+
+constexpr bool kKdSearchDisabled = true;  // LP_NOISE_KDTREE rejected by lp_set_noise_search
+constexpr int kKdEntriesPerPoint = 8;
+constexpr uint32_t kKdMaxNeighbours = 7;
+constexpr int kKdRecInts = 8;
 
 // Per-frame scalar slots in DevBuf::fs.
 enum FsSlot : int {
@@ -198,6 +225,7 @@ enum FsSlot : int {
     kFsHullTotal,
     kFsHullChunks,
     kFsHullInexact,    // predicates whose double evaluation had the wrong sign
+    kFsHullRounded,    // clusters whose hull was redone with the reference's rounded predicate
     kFsCcGlobal,       // 1: too many voxels for the LDS union-find
     kFsErr,
     kFsBoxes,          // clusters with a box
@@ -209,6 +237,7 @@ enum FsSlot : int {
     kFsNoise,          // points labelled NOISE
     kFsNoiseHard,      // points left to the neighbour-cell search
     kFsNoiseCells,     // occupied noise-filter cells
+    kFsNoiseErr,       // error bits of the noise filter (kErrNoise*)
     kFsCount
 };
 
@@ -240,6 +269,11 @@ struct lp_context {
     int32_t staged_points = 0;
     // optional stages of the fused path
     bool noise_on = false;
+    bool noise_kdtree = false;      // the reference object's KD-tree search instead of the radius rule
+    bool persistent = false;        // per-point kernels as persistent (frame, tile) work-queue launches
+    int32_t point_wg = 256;         // workgroup size of the per-point kernels (64, 128 or 256)
+    int32_t mark_tile = 256;        // columns per workgroup of the JCP queue marking (64, 128 or 256)
+    void* kd_mem = nullptr;         // its device buffers
     lp::NoiseConst nc{};
     bool boxes_on = false;
     bool clouds_on = false;

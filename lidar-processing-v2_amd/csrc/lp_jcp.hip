@@ -22,6 +22,7 @@
 // for all queue pixels in parallel (k_jcp_slots); the per-row solve only
 // re-runs the ordered float sums that the reference performs.
 #include "lp_kernels.h"
+#include "lp_queue.h"
 
 namespace lp {
 
@@ -45,16 +46,18 @@ constexpr uint32_t kSG = 0, kSO = 1, kSU = 2;
 // segmenter.cpp:491-514).
 // Queue flags: a ground pixel with an obstacle pixel in its 5x5 window
 // clipped to the image (the dilated red channel of segmenter.cpp:484-513).
-// Each workgroup takes 256 columns of one row and stages rows r-2..r+2 of
-// columns c0-2..c0+257 in LDS (out-of-image pixels read as empty).
-__global__ __launch_bounds__(256) void k_jcp_mark(DevBuf d, Layout L) {
-    constexpr int kSpan = 256 + 4;
+// Each workgroup takes T columns of one row (T = 64, 128 or 256, the
+// tile width the launch chooses) and stages rows r-2..r+2 of columns
+// c0-2..c0+T+1 in LDS (out-of-image pixels read as empty).
+template <int T>
+__global__ __launch_bounds__(T) void k_jcp_mark(DevBuf d, Layout L) {
+    constexpr int kSpan = T + 4;
     __shared__ uint8_t win[5][kSpan];
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
     const int H = fd.sc.H, W = fd.sc.W;
-    const int tpr = (W + 255) / 256;
-    const int r = static_cast<int>(blockIdx.x) / tpr, c0 = (static_cast<int>(blockIdx.x) % tpr) * 256;
+    const int tpr = (W + T - 1) / T;
+    const int r = static_cast<int>(blockIdx.x) / tpr, c0 = (static_cast<int>(blockIdx.x) % tpr) * T;
     if (r >= H) return;
     const uint8_t* st = d.state + fd.pix_off;
     for (int q = threadIdx.x; q < 5 * kSpan; q += blockDim.x) {
@@ -485,10 +488,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
 // Labels per input point (populateLabels, segmenter.cpp:640-669): only the
 // winner of a pixel gets its pixel's final label; PENDING stays UNKNOWN.
 // Also flags obstacle points for the input-order split of processor.cpp:562-579.
-__global__ void k_seg_labels(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+__device__ __forceinline__ void seg_labels_tile(const DevBuf& d, const Layout& L, int f, int tile) {
     const FrameDesc& fd = d.frames[f];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tile * blockDim.x + threadIdx.x;
     if (i >= fd.n) return;
     const int gi = fd.pt_off + i;
     uint32_t lab = LP_LABEL_UNKNOWN;
@@ -504,6 +506,13 @@ __global__ void k_seg_labels(DevBuf d, Layout L) {
     d.labels[gi] = lab;
     d.obs_flag[(size_t)f * L.N + i] = lab == LP_LABEL_OBSTACLE ? 1u : 0u;
     if (d.gnd_flag) d.gnd_flag[(size_t)f * L.N + i] = lab == LP_LABEL_GROUND ? 1u : 0u;
+}
+
+__global__ void k_seg_labels(DevBuf d, Layout L) { seg_labels_tile(d, L, blockIdx.y, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_seg_labels_q(DevBuf d, Layout L, int B) {
+    wq_run(d.wq + kWqSegLabels * 8, B, blockDim.x, [&](int f) { return d.frames[f].n; },
+           [&](int f, int t) { seg_labels_tile(d, L, f, t); });
 }
 
 // Segmenter::image after segment() (segmenter.hpp:163-166), as BGR bytes.
@@ -554,7 +563,13 @@ void launch_jcp(lp_context* c, int B) {
     hipStream_t st = c->stream;
     const int pb = (L.P + 255) / 256;
     stage_mark(c, "jcp_queue");
-    hipLaunchKernelGGL(k_jcp_mark, dim3(L.H * ((L.W + 255) / 256), B), dim3(256), 0, st, d, L);
+    const int mt = c->mark_tile;
+    if (mt == 64)
+        hipLaunchKernelGGL(k_jcp_mark<64>, dim3(L.H * ((L.W + 63) / 64), B), dim3(64), 0, st, d, L);
+    else if (mt == 128)
+        hipLaunchKernelGGL(k_jcp_mark<128>, dim3(L.H * ((L.W + 127) / 128), B), dim3(128), 0, st, d, L);
+    else
+        hipLaunchKernelGGL(k_jcp_mark<256>, dim3(L.H * ((L.W + 255) / 256), B), dim3(256), 0, st, d, L);
     launch_scan(c, B, d.qflag, reinterpret_cast<uint32_t*>(d.qmap), L.P, kCountPixels, 0, kFsQueue);
     hipLaunchKernelGGL(k_jcp_fill, dim3(pb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "jcp_slots");
@@ -562,7 +577,11 @@ void launch_jcp(lp_context* c, int B) {
     stage_mark(c, "jcp_solve");
     hipLaunchKernelGGL(k_jcp_solve, dim3(B), dim3(kSolveThreads), 0, st, d, L);
     stage_mark(c, "labels");
-    hipLaunchKernelGGL(k_seg_labels, dim3((L.N + 255) / 256, B), dim3(256), 0, st, d, L);
+    const int wg = c->point_wg;
+    if (c->persistent)
+        hipLaunchKernelGGL(k_seg_labels_q, dim3(kPersistentWorkgroups), dim3(wg), 0, st, d, L, B);
+    else
+        hipLaunchKernelGGL(k_seg_labels, dim3((L.N + wg - 1) / wg, B), dim3(wg), 0, st, d, L);
 }
 
 void launch_label_image(lp_context* c, int f, int pixels, uint8_t* img) {

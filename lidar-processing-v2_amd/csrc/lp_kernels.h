@@ -9,6 +9,10 @@ namespace lp {
 // Stores msg as the calling thread's lp_last_error() text; returns code.
 int set_last_error(int code, const char* msg);
 
+// The per-frame checks of lp_batch_stage (sizes, pointers, configuration,
+// record layout) without staging: LP_OK or the error lp_batch_stage would give.
+int validate_frame(const lp_context* c, const lp_segmenter_config& cfg, const lp_cloud& cl);
+
 // Records a named stage boundary on the context stream when profiling.
 void stage_mark(lp_context* c, const char* name);
 
@@ -39,8 +43,14 @@ void launch_clear(lp_context* c, int B);
 // coordinates as (x, y) pairs for the canonical choice among identical points.
 void launch_hull(lp_context* c, int B, const double2* list_xy = nullptr);
 // Noise filter over the staged raw inputs; with `compact`, the kept points
-// become the segmenter's inputs.
+// become the segmenter's inputs.  The first applies the radius rule, the
+// second the reference object's KD-tree search.
 void launch_noise(lp_context* c, int B, bool compact);
+void launch_noise_kdtree(lp_context* c, int B, bool compact);
+// Kept points (keep flags of the noise stage) into the segmenter's inputs.
+void launch_noise_compact(lp_context* c, int B);
+// Tree levels built range by range before the per-subtree LDS kernel.
+int kd_small_level_host(int n);
 // Oriented boxes of the polygons in the second hull candidate buffer: the
 // hulls of the last hull stage (written there by the local-rank kernel) or,
 // for lists, the polygons staged by the caller.

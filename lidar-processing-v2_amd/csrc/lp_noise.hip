@@ -96,6 +96,7 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
         d.fs[f * kFsCount + kFsNoise] = 0;
         d.fs[f * kFsCount + kFsNoiseHard] = 0;
         d.fs[f * kFsCount + kFsNoiseCells] = 0;
+        d.fs[f * kFsCount + kFsNoiseErr] = 0;
     }
 }
 
@@ -438,10 +439,15 @@ void launch_noise(lp_context* c, int B, bool compact) {
     const int sb = std::min(256, std::max(32, 2048 / B));
     hipLaunchKernelGGL(k_noise_search, dim3(sb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_compact");
-    if (compact) {
-        launch_scan(c, B, d.obs_flag, reinterpret_cast<uint32_t*>(d.obs_src), L.N, kCountRawPoints, 0, kFsKept);
-        hipLaunchKernelGGL(k_noise_compact, dim3(nb, B), dim3(256), 0, st, d, L);
-    }
+    if (compact) launch_noise_compact(c, B);
+}
+
+void launch_noise_compact(lp_context* c, int B) {
+    DevBuf& d = c->d;
+    const Layout& L = c->L;
+    const int nb = (L.N + 255) / 256;
+    launch_scan(c, B, d.obs_flag, reinterpret_cast<uint32_t*>(d.obs_src), L.N, kCountRawPoints, 0, kFsKept);
+    hipLaunchKernelGGL(k_noise_compact, dim3(nb, B), dim3(256), 0, c->stream, d, L);
 }
 
 }  // namespace lp

@@ -12,6 +12,7 @@
 // are (tiles, frames) and sized for the batch capacity so one launch
 // sequence (and one captured graph) serves every batch of a shape bucket.
 #include "lp_kernels.h"
+#include "lp_queue.h"
 
 namespace lp {
 
@@ -30,6 +31,7 @@ __global__ void k_clear(DevBuf d, Layout L) {
     const int P = fd.sc.H * fd.sc.W;
     const int stride = gridDim.x * blockDim.x;
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f == 0 && t0 < kWqCounters) d.wq[t0] = 0;
     for (int c = t0; c < cells; c += stride) {
         d.cell_count[(size_t)f * L.cell_stride + c] = 0;
         d.cell_cursor[(size_t)f * L.cell_stride + c] = 0;
@@ -52,10 +54,9 @@ __global__ void k_clear(DevBuf d, Layout L) {
 // Equal depth implies equal radial cell, so the order reduces to
 // (depth, slice, index): packed into one 64-bit key and resolved with a
 // single atomicMin.
-__global__ void k_project(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
+__device__ __forceinline__ void project_tile(const DevBuf& d, const Layout& L, int f, int tile) {
     const FrameDesc& fd = d.frames[f];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tile * blockDim.x + threadIdx.x;
     if (i >= fd.n) return;
     const SegConst& s = fd.sc;
     const int gi = fd.pt_off + i;
@@ -93,6 +94,13 @@ __global__ void k_project(DevBuf d, Layout L) {
     }
     d.pt_cell[gi] = cell;
     d.pt_pix[gi] = pix;
+}
+
+__global__ void k_project(DevBuf d, Layout L) { project_tile(d, L, blockIdx.y, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_project_q(DevBuf d, Layout L, int B) {
+    wq_run(d.wq + kWqProject * 8, B, blockDim.x, [&](int f) { return d.frames[f].n; },
+           [&](int f, int t) { project_tile(d, L, f, t); });
 }
 
 // Exclusive scan of the cell histogram, one 1024-thread workgroup per frame.
@@ -720,7 +728,11 @@ void launch_segment_front(lp_context* c, int B) {
     const int nb = (L.N + 255) / 256;
     launch_clear(c, B);
     stage_mark(c, "project");
-    hipLaunchKernelGGL(k_project, dim3(nb, B), dim3(256), 0, st, d, L);
+    const int wg = c->point_wg;
+    if (c->persistent)
+        hipLaunchKernelGGL(k_project_q, dim3(kPersistentWorkgroups), dim3(wg), 0, st, d, L, B);
+    else
+        hipLaunchKernelGGL(k_project, dim3((L.N + wg - 1) / wg, B), dim3(wg), 0, st, d, L);
     stage_mark(c, "cells");
     hipLaunchKernelGGL(k_cell_scan, dim3(B), dim3(1024), 0, st, d, L);
     hipLaunchKernelGGL(k_cell_fill, dim3(nb, B), dim3(256), 0, st, d, L);

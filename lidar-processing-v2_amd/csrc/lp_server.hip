@@ -21,7 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "lp_internal.h"
+#include "lp_kernels.h"
 
 namespace {
 
@@ -120,6 +120,7 @@ void lp_server_config_default(lp_server_config* cfg) {
     cfg->oriented_boxes = 0;
     cfg->label_clouds = 0;
     cfg->vehicle_boxes = 0;
+    cfg->noise_search = LP_NOISE_RULE;
 }
 
 int lp_server_create(const lp_server_config* cfg, lp_server** out) {
@@ -144,7 +145,13 @@ int lp_server_create(const lp_server_config* cfg, lp_server** out) {
             break;
         }
         s->slot_bytes = c->arena_bytes;
-        if (cfg->noise_filter) lp_set_noise_filter(c, &cfg->noise);
+        int ns = lp_set_noise_search(c, cfg->noise_search);
+        if (ns == LP_OK && cfg->noise_filter) ns = lp_set_noise_filter(c, &cfg->noise);
+        if (ns != LP_OK) {
+            lp_context_destroy(c);
+            lp_server_destroy(s);
+            return ns;
+        }
         lp_set_oriented_boxes(c, cfg->oriented_boxes);
         lp_set_vehicle_boxes(c, cfg->vehicle_boxes);
         const int cl = lp_set_label_clouds(c, cfg->label_clouds);
@@ -176,10 +183,11 @@ int lp_server_destroy(lp_server* s) {
 int lp_server_submit(lp_server* s, const lp_segmenter_config* cfg, const lp_cloud* cloud,
                      const lp_frame_outputs* outputs, lp_frame_result* result, uint64_t tag) {
     if (!s || !cfg || !cloud || !outputs) return LP_ERR_INVALID_ARGUMENT;
-    const lp_capacity& cap = s->cfg.capacity;
-    if (cloud->num_points < 0 || cloud->num_points > cap.max_points_per_frame ||
-        cfg->image_height > cap.max_image_height || cfg->image_width > cap.max_image_width)
-        return LP_ERR_CAPACITY;
+    // A frame that staging would refuse is refused here, so it never fails
+    // the other frames of the batch it would have joined.  Every slot has
+    // the same capacity.
+    const int st = lp::validate_frame(s->slots.front().ctx, *cfg, *cloud);
+    if (st != LP_OK) return st;
     s->queue.push_back(Request{*cfg, *cloud, *outputs, result, tag});
     return LP_OK;
 }

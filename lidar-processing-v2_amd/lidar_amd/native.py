@@ -30,8 +30,10 @@ EXPORTED = [
     "lp_remove_noise", "lp_oriented_boxes", "lp_set_noise_filter", "lp_set_oriented_boxes",
     "lp_server_config_default", "lp_server_create", "lp_server_destroy", "lp_server_submit", "lp_server_poll",
     "lp_server_stats_get", "lp_cloud_from_pointcloud2", "lp_pca_boxes", "lp_set_label_clouds",
-    "lp_set_vehicle_boxes", "lp_frame_image", "lp_add_ring_info_noalloc",
+    "lp_set_vehicle_boxes", "lp_frame_image", "lp_add_ring_info_noalloc", "lp_set_noise_search", "lp_set_persistent", "lp_set_launch_shape",
 ]
+
+NOISE_RULE, NOISE_KDTREE = 0, 1
 
 MATH_EXPF, MATH_ATANF, MATH_ATAN2F, MATH_ATAN2_APPROX = range(4)
 
@@ -176,6 +178,8 @@ class FrameResult(ctypes.Structure):
         ("num_ground", ctypes.c_int32),
         ("num_unknown", ctypes.c_int32),
         ("num_vehicles", ctypes.c_int32),
+        ("num_hull_inexact", ctypes.c_int32),
+        ("num_hull_rounded", ctypes.c_int32),
     ]
 
 
@@ -191,6 +195,7 @@ class ServerConfig(ctypes.Structure):
         ("oriented_boxes", ctypes.c_int32),
         ("label_clouds", ctypes.c_int32),
         ("vehicle_boxes", ctypes.c_int32),
+        ("noise_search", ctypes.c_int32),
     ]
 
 
@@ -252,6 +257,9 @@ def lib():
         L.lp_pca_boxes.argtypes = [P, P, P, P, I, ctypes.POINTER(BoundingBox)]
         L.lp_set_label_clouds.argtypes = [P, I]
         L.lp_set_vehicle_boxes.argtypes = [P, I]
+        L.lp_set_noise_search.argtypes = [P, I]
+        L.lp_set_persistent.argtypes = [P, I]
+        L.lp_set_launch_shape.argtypes = [P, I, I]
         L.lp_frame_image.argtypes = [P, I, P, I]
         L.lp_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
         L.lp_server_config_default.restype = None
@@ -489,6 +497,18 @@ class Context:
         check(lib().lp_set_noise_filter(self.h, None if cfg is None else ctypes.byref(cfg)))
         self._noise_cfg = cfg
 
+    def set_noise_search(self, mode):
+        """NOISE_RULE (the radius rule) or NOISE_KDTREE (the reference object's search)."""
+        check(lib().lp_set_noise_search(self.h, int(mode)))
+
+    def set_persistent(self, enable):
+        """Per-point kernels as persistent (frame, tile) work-queue launches."""
+        check(lib().lp_set_persistent(self.h, 1 if enable else 0))
+
+    def set_launch_shape(self, point_workgroup=256, mark_tile=256):
+        """Per-point workgroup size and JCP marking tile width (64, 128 or 256)."""
+        check(lib().lp_set_launch_shape(self.h, int(point_workgroup), int(mark_tile)))
+
     def set_oriented_boxes(self, enable):
         check(lib().lp_set_oriented_boxes(self.h, 1 if enable else 0))
         self._boxes_on = bool(enable)
@@ -583,6 +603,7 @@ class Context:
                             if self._boxes_on or self._vehicles_on else [],
                             ground_index=b["gidx"][:r.num_ground], unknown_index=b["uidx"][:r.num_unknown],
                             num_ground=r.num_ground, num_unknown=r.num_unknown, num_vehicles=r.num_vehicles,
+                            num_hull_inexact=r.num_hull_inexact, num_hull_rounded=r.num_hull_rounded,
                             vehicle_class=b["vcls"][:K] if self._vehicles_on else None,
                             cluster_labels=b["clab"][:no], num_clusters=K, hull_offsets=b["hoff"][:K + 1],
                             hull_indices=b["hidx"][:V], num_voxels=r.num_voxels, jcp_queue=r.jcp_queue,
@@ -667,7 +688,8 @@ class Server:
     """lp_server: continuous batching over several contexts."""
 
     def __init__(self, slots=3, max_batch_frames=8, max_points=262144, max_height=128, max_width=2048,
-                 hbm_budget_bytes=0, noise_cfg=None, boxes=False, ccfg=None, label_clouds=False, vehicles=False):
+                 hbm_budget_bytes=0, noise_cfg=None, boxes=False, ccfg=None, label_clouds=False, vehicles=False,
+                 noise_search=0):
         cfg = ServerConfig()
         lib().lp_server_config_default(ctypes.byref(cfg))
         cfg.slots = slots
@@ -684,6 +706,7 @@ class Server:
         cfg.oriented_boxes = 1 if boxes else 0
         cfg.label_clouds = 1 if label_clouds else 0
         cfg.vehicle_boxes = 1 if vehicles else 0
+        cfg.noise_search = int(noise_search)
         h = ctypes.c_void_p()
         check(lib().lp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

@@ -365,3 +365,32 @@ def test_cpp_dropin_processor(tmp_path, oracle):
     assert (rows, cols) == (f.height, f.width)
     _, _, dbg = oracle.segment(f.x, f.y, f.z, f.ring, debug=True)
     assert np.array_equal(np.frombuffer(tail[len(body) + 8:], np.uint8), dbg["image"])
+
+
+def test_cpp_dropin_grows_past_the_reserve(tmp_path, oracle):
+    """A 128x2048 scan of more than 200 000 points (the reference's reserve
+    hint, segmenter.cpp:62) through the drop-in classes: the context grows
+    for the call instead of failing, and the results equal the oracle's."""
+    import os
+    import subprocess
+    f = frame("c4")
+    assert f.n > 200000
+    rec = np.zeros(f.n, dtype=np.dtype({"names": ["x", "y", "z", "intensity", "ring"],
+                                        "formats": ["<f4", "<f4", "<f4", "<f4", "<u2"],
+                                        "offsets": [0, 4, 8, 16, 20], "itemsize": 32}))
+    rec["x"], rec["y"], rec["z"], rec["intensity"], rec["ring"] = f.x, f.y, f.z, f.intensity, f.ring
+    src = tmp_path / "in.bin"
+    dst = tmp_path / "out.bin"
+    with open(src, "wb") as fh:
+        fh.write(np.int32(f.n).tobytes())
+        fh.write(rec.tobytes())
+    exe = os.path.join(os.path.dirname(N.library_path()), "processor_example")
+    subprocess.run([exe, str(src), str(dst)], check=True, timeout=120)
+    buf = np.fromfile(dst, np.int32)
+    seg = buf[:f.n].view(np.uint32)
+    no = buf[f.n]
+    clab = buf[f.n + 1:f.n + 1 + no]
+    ref = oracle.process_frame(f.x, f.y, f.z, f.ring)
+    assert ref["status"] == 0
+    assert np.array_equal(seg, ref["seg_labels"])
+    assert np.array_equal(clab, ref["cluster_labels"])

@@ -78,3 +78,41 @@ def test_oversized_frame_rejected_at_submit():
         srv.submit(slot, x.x, x.y, x.z, x.ring, f["cfg"], 0)
     assert srv.poll() == []
     srv.close()
+
+
+def test_invalid_frame_rejected_at_submit_keeps_the_batch():
+    """Frames that staging would refuse (empty image, fewer than four radial
+    rings) are refused by submit, so the valid frames around them still run
+    in one batch and give their single-frame results."""
+    pools = serving.sensor_pools(pool_frames=2, sensors=[32, 64])
+    ref = alone(pools, False, False)
+    srv = N.Server(slots=1, max_batch_frames=8)
+    slots, tags = [], []
+    bad_cfgs = [N.segmenter_config(image_height=0), N.segmenter_config(max_distance_m=6.0)]
+    tag = 0
+    for s, p in enumerate(pools):
+        for k, f in enumerate(p["frames"]):
+            slot = N.FrameSlot(f.n)
+            srv.submit(slot, f.x, f.y, f.z, f.ring, p["cfg"], tag)
+            slots.append((slot, s, k))
+            tags.append(tag)
+            tag += 1
+            bad = N.FrameSlot(f.n)
+            with pytest.raises(N.LidarError):
+                srv.submit(bad, f.x, f.y, f.z, f.ring, bad_cfgs[k % 2], 1000 + tag)
+    import time
+    done = []
+    t_end = time.monotonic() + 60.0
+    while len(done) < len(tags) and time.monotonic() < t_end:
+        done += srv.poll()
+        time.sleep(0.0005)
+    st = srv.stats()
+    srv.close()
+    assert sorted(done) == tags
+    assert st["batches"] == 1 and st["frames"] == len(tags)
+    for (slot, s, k) in slots:
+        got = slot.result()
+        want = ref[s, k]
+        assert got["status"] == 0
+        for key in ("seg_labels", "obstacle_index", "cluster_labels", "hull_offsets", "hull_indices"):
+            assert np.array_equal(got[key], want[key]), key
