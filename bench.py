@@ -488,8 +488,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--streams", type=int, default=2, help="streams the batch is split over")
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--streams", type=int, default=4, help="streams the batch is split over")
     ap.add_argument("--plain-frames", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -529,8 +529,8 @@ def main():
         if args.serve_seconds > 0:
             log("serving (10 Hz per sensor)")
             extra["serving"] = run_serving(args.serve_seconds)
-            log("serving, saturated (120 Hz per sensor)")
-            extra["serving_saturated"] = run_serving(min(args.serve_seconds, 5.0), rate_hz=120.0, jitter_ms=2.0)
+            log("serving, saturated (1 kHz per sensor, above the service rate)")
+            extra["serving_saturated"] = run_serving(min(args.serve_seconds, 2.0), rate_hz=1000.0, jitter_ms=0.5)
     cpu = None
     cpu_par = None
     if not args.no_cpu:

@@ -523,6 +523,7 @@ int create_lane(const lp_capacity* cap, lp_context** out) {
     c->arena_bytes = bytes;
     layout_bytes(L, &c->d, static_cast<char*>(c->arena));
     launch_mt19937(c->d.mt_raw, kMtRawCount, c->stream);
+    launch_hash_init(c);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         lp_context_destroy(c);

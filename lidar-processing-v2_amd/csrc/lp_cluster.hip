@@ -475,6 +475,7 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     hipLaunchKernelGGL(k_component_label, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan_small(c, B, d.cl_size, d.cl_off, L.N, kCountSlot, kFsClusters, -1);
     hipLaunchKernelGGL(k_cluster_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
+    launch_hash_reset(c, B, d.vlist, kFsVoxels);
 }
 
 }  // namespace lp

@@ -1350,12 +1350,16 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     const int nb = (L.N + 255) / 256;
     // Stage marks around the single-kernel steps ("hull_round" three
     // times), so profiled runs time each round launch on its own.
+    // Chunk wavefronts loop over the frame's chunks: about 2048 workgroups
+    // in all, at most 512 per frame; workgroups beyond that would mostly
+    // find no chunk and only cost dispatch.
+    const int round_grid = std::min(512, std::max(8, 2048 / B));
     for (int r = 0; r < kHullRounds; ++r) {
         stage_mark(c, "hull");
         hipLaunchKernelGGL(k_hull_chunks, dim3(nb, B), dim3(256), 0, st, d, L, r);
         launch_scan_small(c, B, d.hc_chunks, d.hc_choff, L.N, kCountSlot, kFsClusters, kFsHullChunks);
         stage_mark(c, "hull_round");
-        hipLaunchKernelGGL(k_hull_round, dim3(512, B), dim3(256), 0, st, d, L, r);
+        hipLaunchKernelGGL(k_hull_round, dim3(round_grid, B), dim3(256), 0, st, d, L, r);
     }
     stage_mark(c, "hull_big");
     hipLaunchKernelGGL(k_hull_big, dim3(16, B), dim3(1024), 0, st, d, L, kHullRounds);

@@ -23,6 +23,7 @@ constexpr int kBigCell = 8192;        // larger cells: one workgroup; beyond: ca
 constexpr int kHullChunk = 256;       // points per chunk-hull wave
 constexpr int kHullFinal = 8192;      // candidates a final hull workgroup can hold
 constexpr int kNoiseOccWords = 1024 * 1024 / 32;  // noise filter: 1024 x 1024 column bits per frame
+constexpr int32_t kHashMinClean = 0x7fffffff;     // hmin of a free hash slot (the voxel minimum's identity)
 
 // Per-frame error bits (device), mapped to status codes on fetch.
 enum : uint32_t {

@@ -32,6 +32,11 @@ void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stri
                         uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot);
 
 void launch_mt19937(uint32_t* out, int count, hipStream_t st);
+// Resets the hash slots slots[f * N .. + fs[count_slot]) of every frame to
+// their clean state (all slots when the count exceeds N).
+void launch_hash_reset(lp_context* c, int B, const int32_t* slots, int count_slot);
+// Clears every hash slot of the context (at creation).
+void launch_hash_init(lp_context* c);
 void launch_segment_front(lp_context* c, int B);
 void launch_jcp(lp_context* c, int B);
 void launch_cluster(lp_context* c, int B, bool from_split);

@@ -80,16 +80,13 @@ __device__ __forceinline__ float noise_dist(const float4& t, const float4& p) {
     return d0 * d0 + (d1 * d1 + (d2 * d2 + 0.0f));
 }
 
+// The hash arrays (hkey, hmin, hcount) are clean between uses: every user
+// resets the slots it occupied when it is done (k_hash_reset), so only the
+// column bits and the counters are cleared here.
 __global__ void k_noise_clear(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const int stride = gridDim.x * blockDim.x;
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int h = t0; h < L.hash_cap; h += stride) {
-        const size_t o = (size_t)f * L.hash_cap + h;
-        d.hkey[o] = -1;
-        d.hmin[o] = 0;
-        d.hcount[o] = 0;
-    }
     for (int q = t0; q < kNoiseOccWords; q += stride) d.nocc[(size_t)f * kNoiseOccWords + q] = 0;
     if (t0 == 0) {
         d.fs[f * kFsCount + kFsKept] = 0;
@@ -190,7 +187,9 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const size_t ho = (size_t)f * L.hash_cap;
     const int32_t h = d.vslot[(size_t)f * L.N + i];
     const int start = static_cast<int>(d.ncell_start[(size_t)f * L.N + d.hdense[ho + h]]);
-    const int pos = start + static_cast<int>(wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h])));
+    // hmin is the cell's cursor here, counting up from its clean value.
+    const uint32_t cur = wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h]));
+    const int pos = start + static_cast<int>(cur - static_cast<uint32_t>(kHashMinClean));
     d.hstart[ho + h] = start;  // every point of the cell writes the same value
     const size_t po = (size_t)f * L.N;
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
@@ -438,6 +437,7 @@ void launch_noise(lp_context* c, int B, bool compact) {
     // at least 128 and at most 1024 per frame.
     const int sb = std::min(256, std::max(32, 2048 / B));
     hipLaunchKernelGGL(k_noise_search, dim3(sb, B), dim3(256), 0, st, d, L);
+    launch_hash_reset(c, B, d.ncell_slot, kFsNoiseCells);
     stage_mark(c, "noise_compact");
     if (compact) launch_noise_compact(c, B);
 }

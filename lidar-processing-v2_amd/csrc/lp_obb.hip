@@ -216,6 +216,9 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
                     double a = __builtin_nan("");
                     if (cand_dir(X, Y, n, e, 1, ux, uy)) {
                         Extent ex = extent_init();
+                        // Unrolled so the vertex loads of four steps are in
+                        // flight together; the order of the updates stays.
+#pragma unroll 4
                         for (int q = 0; q < n; ++q) {
                             const double tx = X[q] - cx;
                             const double ty = Y[q] - cy;

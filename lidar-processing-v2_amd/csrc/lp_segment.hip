@@ -23,7 +23,8 @@ constexpr uint32_t kEmptyZ = 0x7fc00000u;           // NaN marks an empty cell
 
 // ---------------------------------------------------------------------------
 // Per-batch clear (fill kernels instead of memsets so the whole sequence can
-// be captured in a graph).
+// be captured in a graph).  The voxel hash is clean on entry (its previous
+// users reset the slots they occupied).
 __global__ void k_clear(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const FrameDesc fd = d.frames[f];
@@ -37,12 +38,6 @@ __global__ void k_clear(DevBuf d, Layout L) {
         d.cell_cursor[(size_t)f * L.cell_stride + c] = 0;
     }
     for (int p = t0; p < P; p += stride) d.pix_key[fd.pix_off + p] = ~0ull;
-    for (int h = t0; h < L.hash_cap; h += stride) {
-        const size_t o = (size_t)f * L.hash_cap + h;
-        d.hkey[o] = -1;
-        d.hmin[o] = 0x7fffffff;
-        d.hcount[o] = 0;
-    }
     for (int k = t0; k < kRansacIters; k += stride) d.hyp_count[f * kRansacIters + k] = 0;
     if (t0 < kFsKept) d.fs[f * kFsCount + t0] = 0;  // the noise filter's counts run ahead of this clear
 }
@@ -756,6 +751,49 @@ void launch_segment_front(lp_context* c, int B) {
 void launch_clear(lp_context* c, int B) {
     stage_mark(c, "clear");
     hipLaunchKernelGGL(k_clear, dim3(256, B), dim3(256), 0, c->stream, c->d, c->L);
+}
+
+// Hash slots back to their clean state: hkey -1 (free), hmin kHashMinClean,
+// hcount 0.  k_hash_reset takes the slots a stage occupied (slots[0 ..
+// fs[count_slot]) per frame); k_hash_init clears every slot (context
+// creation, or a frame whose list is incomplete).
+__global__ void k_hash_reset(DevBuf d, Layout L, const int32_t* slots, int count_slot) {
+    const int f = blockIdx.y;
+    const uint32_t cnt = d.fs[f * kFsCount + count_slot];
+    const size_t ho = (size_t)f * L.hash_cap;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cnt > static_cast<uint32_t>(L.N)) {
+        for (int h = t0; h < L.hash_cap; h += stride) {
+            d.hkey[ho + h] = -1;
+            d.hmin[ho + h] = kHashMinClean;
+            d.hcount[ho + h] = 0;
+        }
+        return;
+    }
+    for (int v = t0; v < static_cast<int>(cnt); v += stride) {
+        const int32_t h = slots[(size_t)f * L.N + v];
+        d.hkey[ho + h] = -1;
+        d.hmin[ho + h] = kHashMinClean;
+        d.hcount[ho + h] = 0;
+    }
+}
+
+__global__ void k_hash_init(DevBuf d, Layout L) {
+    const size_t total = (size_t)L.B * L.hash_cap;
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total; o += (size_t)gridDim.x * blockDim.x) {
+        d.hkey[o] = -1;
+        d.hmin[o] = kHashMinClean;
+        d.hcount[o] = 0;
+    }
+}
+
+void launch_hash_reset(lp_context* c, int B, const int32_t* slots, int count_slot) {
+    hipLaunchKernelGGL(k_hash_reset, dim3(64, B), dim3(256), 0, c->stream, c->d, c->L, slots, count_slot);
+}
+
+void launch_hash_init(lp_context* c) {
+    hipLaunchKernelGGL(k_hash_init, dim3(1024), dim3(256), 0, c->stream, c->d, c->L);
 }
 
 void launch_mt19937(uint32_t* out, int count, hipStream_t st) {
