@@ -205,7 +205,9 @@ void lp_noise_config_default(lp_noise_config* cfg);
  *                    point again.  min_neighbours up to 7; non-finite
  *                    coordinates give LP_ERR_INVALID_ARGUMENT.  Its device
  *                    buffers (about 208 bytes per point and frame) are
- *                    allocated when it is first selected. */
+ *                    allocated when it is first selected.
+ * In this build LP_NOISE_KDTREE is rejected with LP_ERR_INVALID_ARGUMENT
+ * ("switched off"): its kernels are compiled but not reachable. */
 enum { LP_NOISE_RULE = 0, LP_NOISE_KDTREE = 1 };
 int lp_set_noise_search(lp_context* ctx, int32_t mode);
 
