@@ -203,9 +203,10 @@ def run_c1_plain(args):
     return out
 
 
-def run_batched(frames, steps, noise=False, boxes=False, streams=2, load=0.5):
+def run_batched(frames, steps, noise=False, boxes=False, streams=2, load=0.5, point_workgroup=256, mark_tile=256,
+                persistent=False):
     """Frames/s of one batch of `frames` (any sizes) replayed as graphs on
-    `streams` streams."""
+    `streams` streams with the given launch knobs."""
     from lidar_amd import native as N
     maxn = max(f.n for f in frames)
     H = max(f.height for f in frames)
@@ -214,6 +215,8 @@ def run_batched(frames, steps, noise=False, boxes=False, streams=2, load=0.5):
     if noise:
         ctx.set_noise_filter(N.noise_config())
     ctx.set_oriented_boxes(boxes)
+    ctx.set_launch_shape(point_workgroup, mark_tile)
+    ctx.set_persistent(persistent)
     cfgs = [N.segmenter_config(image_height=f.height, image_width=f.width, elevation_up_deg=f.fov_up_deg,
                                elevation_down_deg=f.fov_down_deg) for f in frames]
     ctx.stage([(f.x, f.y, f.z, f.ring) for f in frames], cfgs)
@@ -231,7 +234,8 @@ def run_batched(frames, steps, noise=False, boxes=False, streams=2, load=0.5):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ctx.close()
-    return dict(frames=len(frames), streams=streams, hash_load_factor=load,
+    return dict(frames=len(frames), streams=streams, hash_load_factor=load, point_workgroup=point_workgroup,
+                mark_tile=mark_tile, persistent=persistent,
                 frames_per_s=round(len(frames) * steps / dt, 2),
                 points_per_s=round(sum(f.n for f in frames) * steps / dt, 1),
                 ms_per_batch=round(dt / steps * 1e3, 4))
@@ -515,17 +519,27 @@ def main():
         from lidar_amd import native as N
         log("c1 plain method")
         extra["c1_plain"] = run_c1_plain(args)
-        c3 = [synth.workload_c3(i) for i in range(32)]
-        log("autotune c3")
-        c3_cfgs = [N.segmenter_config(image_height=f.height, image_width=f.width, elevation_up_deg=f.fov_up_deg,
-                                      elevation_down_deg=f.fov_down_deg) for f in c3]
-        tuned = autotune.tune(c3, c3_cfgs, streams=(1, 2, 3), loads=(0.5, 0.7), steps=5, noise=False, boxes=False)
-        extra["c3_autotune"] = tuned
-        log("batched c3 (mixed widths, tuned)")
-        extra["batched_c3"] = run_batched(c3, 5, streams=tuned["best"]["streams"],
-                                          load=tuned["best"]["hash_load_factor"])
-        log("batched c4 (128x2048)")
-        extra["batched_c4"] = run_batched([synth.workload_c4(i) for i in range(16)], 5)
+        # Per shape bucket: c3 (32 frames of mixed widths) and c4 (16 frames
+        # of 128x2048), tuned over streams x load factor, then the point
+        # workgroup size, the marking tile and the launch form (the tuner's
+        # result list holds the persistent / grid A/B at the best setting).
+        def cfgs_of(frames):
+            return [N.segmenter_config(image_height=f.height, image_width=f.width, elevation_up_deg=f.fov_up_deg,
+                                       elevation_down_deg=f.fov_down_deg) for f in frames]
+
+        knobs = ("streams", "hash_load_factor", "point_workgroup", "mark_tile", "persistent")
+        for name, frames in (("c3", [synth.workload_c3(i) for i in range(32)]),
+                             ("c4", [synth.workload_c4(i) for i in range(16)])):
+            log(f"autotune {name}")
+            tuned = autotune.tune(frames, cfgs_of(frames), streams=(1, 2, 4), loads=(0.5, 0.7),
+                                  workgroups=(64, 128, 256), tiles=(64, 128, 256), persistent=(False, True), steps=5,
+                                  noise=False, boxes=False)
+            extra[f"{name}_autotune"] = tuned
+            log(f"batched {name} (tuned)")
+            best = {k: tuned["best"][k] for k in knobs}
+            extra[f"batched_{name}"] = run_batched(frames, 5, streams=best["streams"], load=best["hash_load_factor"],
+                                                   point_workgroup=best["point_workgroup"],
+                                                   mark_tile=best["mark_tile"], persistent=best["persistent"])
         if args.serve_seconds > 0:
             log("serving (10 Hz per sensor)")
             extra["serving"] = run_serving(args.serve_seconds)

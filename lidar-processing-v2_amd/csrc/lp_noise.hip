@@ -190,7 +190,7 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     // hmin is the cell's cursor here, counting up from its clean value.
     const uint32_t cur = wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h]));
     const int pos = start + static_cast<int>(cur - static_cast<uint32_t>(kHashMinClean));
-    d.hstart[ho + h] = start;  // every point of the cell writes the same value
+    if (pos == start) d.hstart[ho + h] = start;  // once per cell: the point that took its first position
     const size_t po = (size_t)f * L.N;
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
     d.didx[po + pos] = i;
