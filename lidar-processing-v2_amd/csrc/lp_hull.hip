@@ -1120,9 +1120,14 @@ __global__ __launch_bounds__(256) void k_hull_ref(DevBuf d, Layout L, const doub
 // it, and the prefix sums of the bins are the ranks.  Each thread takes four
 // members at a time so their loads and searches overlap.  Larger hulls take
 // one wavefront per vertex.
+// Two launches split the clusters: those with at most kLocalSmallV hull
+// vertices and kLocalSmallSize members (most of them) take one wavefront
+// each, with 2.6 KiB of LDS, so a CU holds as many clusters as it holds
+// wavefronts; the others take a 1024-thread workgroup with room for
+// kLocalMax vertices.  Each launch skips the other's clusters.
 constexpr int kLocalMax = 512;
-constexpr int kLocalHash = 2 * kLocalMax;
-constexpr int kLocalThreads = 1024;
+constexpr int kLocalSmallV = 64;
+constexpr int kLocalSmallSize = 1024;
 constexpr int kLocalUnroll = 4;
 
 // Hash of a coordinate pair that agrees with ==: +0.0 and -0.0 hash alike.
@@ -1134,14 +1139,16 @@ __device__ __forceinline__ uint32_t xy_hash(double x, double y) {
     return static_cast<uint32_t>(h);
 }
 
+template <int kLocalThreads, int kVMax, bool kSmall>
 __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L, const double2* list_xy) {
-    __shared__ int32_t canon[kLocalMax];
-    __shared__ int32_t sorted[kLocalMax];
-    __shared__ int16_t spos[kLocalMax];
+    constexpr int kLocalHash = 2 * kVMax;
+    __shared__ int32_t canon[kVMax];
+    __shared__ int32_t sorted[kVMax];
+    __shared__ int16_t spos[kVMax];
     __shared__ int32_t table[kLocalHash];
-    __shared__ double vx[kLocalMax];
-    __shared__ double vy[kLocalMax];
-    __shared__ uint32_t bins[kLocalMax + 1];
+    __shared__ double vx[kVMax];
+    __shared__ double vy[kVMax];
+    __shared__ uint32_t bins[kVMax + 1];
     int tile, f;
     xcd_tile_frame(tile, f);
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
@@ -1156,10 +1163,11 @@ __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L
         const uint32_t base = d.cl_off[ob + c];
         const int size = static_cast<int>(d.cl_size[ob + c]);
         const int32_t* members = d.cl_members + ob + base;
+        if ((V <= kLocalSmallV && size <= kLocalSmallSize) != kSmall) continue;  // the other launch's
         // A cluster left without a hull ran out of the chunk rounds' and the
         // final workgroup's capacity.
         if (V == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
-        if (V > kLocalMax) {
+        if (!kSmall && V > kVMax) {
             for (int i = w; i < V; i += kLocalThreads / kWave) {
                 const int32_t idx = d.hull_idx[ob + base + i];
                 double px, py;
@@ -1368,7 +1376,12 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     stage_mark(c, "hull");
     launch_scan_small(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
     stage_mark(c, "hull_local");
-    hipLaunchKernelGGL(k_hull_local, dim3(256, B), dim3(kLocalThreads), 0, st, d, L, list_xy);
+    // About 8192 wavefronts for the small clusters (every CU full), at
+    // least 8 per frame; the large ones as before.
+    const int small_grid = std::min(1024, std::max(8, 8192 / B));
+    hipLaunchKernelGGL((k_hull_local<kWave, kLocalSmallV, true>), dim3(small_grid, B), dim3(kWave), 0, st, d, L,
+                       list_xy);
+    hipLaunchKernelGGL((k_hull_local<1024, kLocalMax, false>), dim3(256, B), dim3(1024), 0, st, d, L, list_xy);
 }
 
 }  // namespace lp
