@@ -371,8 +371,19 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[
             } else {
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const uint64_t ok = __shfl_xor(key[s], j, kWave);
-                    const uint32_t ov = __shfl_xor(val[s], j, kWave);
+                    // Partners inside a 16-lane row move by DPP (lane_xor),
+                    // the others through a shuffle.
+                    uint64_t ok;
+                    uint32_t ov;
+                    if (j < 16) {
+                        const uint32_t klo = lane_xor(static_cast<uint32_t>(key[s]), j);
+                        const uint32_t khi = lane_xor(static_cast<uint32_t>(key[s] >> 32), j);
+                        ok = (static_cast<uint64_t>(khi) << 32) | klo;
+                        ov = lane_xor(val[s], j);
+                    } else {
+                        ok = __shfl_xor(key[s], j, kWave);
+                        ov = __shfl_xor(val[s], j, kWave);
+                    }
                     const bool asc = ((s * kWave + lane) & k) == 0;
                     const bool lower = (lane & j) == 0;
                     const bool other_less = ok < key[s] || (ok == key[s] && ov < val[s]);
@@ -464,14 +475,23 @@ __device__ int cull_sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t*
     // Counter-clockwise: leftmost, lowest, rightmost, highest; repeated
     // points dropped so no edge has zero length.
     const uint64_t ext[4] = {mnx, (mny << 32) | (mny >> 32), mxx, (mxy << 32) | (mxy >> 32)};
-    double px[4], py[4];
+    // Appended by static-index selects, so the four corners stay in
+    // registers (a dynamically indexed array would live in scratch memory).
+    double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
     int npoly = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const bool fresh = npoly == 0 || ext[e] != ext[e > 0 ? e - 1 : 0];
         if (fresh) {
-            px[npoly] = static_cast<double>(key_float(static_cast<uint32_t>(ext[e] >> 32)));
-            py[npoly] = static_cast<double>(key_float(static_cast<uint32_t>(ext[e])));
+            const double ex_x = static_cast<double>(key_float(static_cast<uint32_t>(ext[e] >> 32)));
+            const double ex_y = static_cast<double>(key_float(static_cast<uint32_t>(ext[e])));
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (t == npoly) {
+                    px[t] = ex_x;
+                    py[t] = ex_y;
+                }
+            }
             ++npoly;
         }
     }
@@ -489,8 +509,10 @@ __device__ int cull_sort_float_chunk(double* X, double* Y, int32_t* I, uint16_t*
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (k < npoly && inside) {
-                    const int b = k + 1 < npoly ? k + 1 : 0;
-                    inside = orient_exact(px[k], py[k], px[b], py[b], qx, qy, &dummy) > 0;
+                    const bool wrap_to_0 = k + 1 >= npoly;
+                    const double bx = wrap_to_0 ? px[0] : px[k < 3 ? k + 1 : 0];
+                    const double by = wrap_to_0 ? py[0] : py[k < 3 ? k + 1 : 0];
+                    inside = orient_exact(px[k], py[k], bx, by, qx, qy, &dummy) > 0;
                 }
             }
             keep = !inside;
