@@ -258,11 +258,55 @@ __device__ __forceinline__ void sort_cell_regs(const DevBuf& d, const FrameDesc&
     }
 }
 
+// Ascending bitonic sort of 16 keys, one per lane of each 16-lane row (all
+// exchanges are DPP moves inside the row).
+__device__ __forceinline__ uint32_t row_sort16_u32(uint32_t key) {
+    const int r = lane_id() & 15;
+#pragma unroll
+    for (int k = 2; k <= 16; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t other = lane_xor(key, j);
+            const bool asc = (r & k) == 0;
+            const bool lower = (r & j) == 0;
+            key = (lower == asc) ? min(key, other) : max(key, other);
+        }
+    }
+    return key;
+}
+
+// sort_cell_regs for a cell of n <= 16 points held by one 16-lane row
+// (element e = lane & 15); every lane of the row calls it.
+__device__ __forceinline__ void sort_cell_row(const DevBuf& d, const FrameDesc& fd, int32_t* items, int n, int radial,
+                                              float* zmin_cell) {
+    const int e = lane_id() & 15;
+    if (radial < kRansacRadialBins) {
+        uint32_t key = e < n ? static_cast<uint32_t>(items[e]) : 0xffffffffu;
+        key = row_sort16_u32(key);
+        if (e < n) items[e] = static_cast<int32_t>(key);
+    }
+    if (radial >= 1) {
+        uint32_t key = e < n ? ord_f32(d.z[fd.pt_off + items[e]]) : 0xffffffffu;
+        key = row_sort16_u32(key);
+        // Robust minimum (segmenter.cpp:249-260): the largest i in [1, n/2]
+        // with s[i] - s[i-1] > 0.5f, else 0.
+        const uint32_t prev = __shfl_up(key, 1, 16);
+        int best = (e >= 1 && e <= n / 2 && unord_f32(key) - unord_f32(prev) > 0.5F) ? e : 0;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) best = max(best, static_cast<int>(lane_xor(static_cast<uint32_t>(best), o)));
+        const uint32_t kb = __shfl(key, (lane_id() & ~15) + best, kWave);
+        if (e == 0) *zmin_cell = unord_f32(kb);
+    }
+}
+
 // One wavefront per non-empty cell (a grid-stride loop over the cell list):
 // sorts the cell's point indices ascending (so the cell lists follow the
 // reference's input order) for the RANSAC cells, and sorts z for the robust
-// minimum.  Cells up to kSmallCell (512) points sort in registers; larger
-// ones are queued for k_cell_sort_big.
+// minimum.  A wavefront takes four consecutive cells of the list: those of
+// up to 16 points are sorted by one 16-lane row each (DPP bitonic), the
+// others one after the other by the whole wavefront.  Cells up to
+// kSmallCell (512) points sort in registers; larger ones are queued for
+// k_cell_sort_big.
 __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     static_assert(kSmallCell == 8 * kWave, "the register sorts cover cells up to kSmallCell points");
     int tile, f;
@@ -275,32 +319,51 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     const int used = static_cast<int>(d.fs[f * kFsCount + kFsCellsUsed]);
     const int waves = gridDim.x * 4;
     float* zmin = d.zmin + co;
-    for (int q = tile * 4 + w; q < used; q += waves) {
-        const int cell = static_cast<int>(d.cell_list[co + q]);
-        const uint32_t n = d.cell_count[co + cell];
-        if (n > static_cast<uint32_t>(kSmallCell)) {
-            if (lane == 0) {
-                const uint32_t k = atomicAdd(&d.fs[f * kFsCount + kFsNumBig], 1u);
-                if (k < static_cast<uint32_t>(L.cell_stride)) d.big_cells[co + k] = cell;
+    for (int q4 = (tile * 4 + w) * 4; q4 < used; q4 += waves * 4) {
+        // Row g of the wavefront takes list entry q4 + g.
+        const int g = lane >> 4;
+        const int qg = q4 + g;
+        int cell_g = -1;
+        uint32_t n_g = 0;
+        if (qg < used) {
+            cell_g = static_cast<int>(d.cell_list[co + qg]);
+            n_g = d.cell_count[co + cell_g];
+        }
+        if (cell_g >= 0 && n_g <= 16u) {
+            const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell_g];
+            sort_cell_row(d, fd, d.cell_items + (size_t)f * L.N + start, static_cast<int>(n_g), cell_g % s.rings,
+                          zmin + cell_g);
+        }
+        // Rows whose cell is larger: the whole wavefront, one cell at a time.
+        const uint64_t large = __ballot(cell_g >= 0 && n_g > 16u && (lane & 15) == 0);
+        for (int gg = 0; gg < 4; ++gg) {
+            if (!((large >> (gg * 16)) & 1ull)) continue;
+            const int cell = __shfl(cell_g, gg * 16, kWave);
+            const uint32_t n = static_cast<uint32_t>(__shfl(static_cast<int>(n_g), gg * 16, kWave));
+            if (n > static_cast<uint32_t>(kSmallCell)) {
+                if (lane == 0) {
+                    const uint32_t k = atomicAdd(&d.fs[f * kFsCount + kFsNumBig], 1u);
+                    if (k < static_cast<uint32_t>(L.cell_stride)) d.big_cells[co + k] = cell;
+                }
+                continue;
             }
-            continue;
+            const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+            int32_t* items = d.cell_items + (size_t)f * L.N + start;
+            const int radial = cell % s.rings;
+            if (n <= static_cast<uint32_t>(kWave)) {
+                sort_cell_regs<1>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+                continue;
+            }
+            if (n <= static_cast<uint32_t>(2 * kWave)) {
+                sort_cell_regs<2>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+                continue;
+            }
+            if (n <= static_cast<uint32_t>(4 * kWave)) {
+                sort_cell_regs<4>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+                continue;
+            }
+            sort_cell_regs<8>(d, fd, items, static_cast<int>(n), radial, zmin + cell);  // n <= kSmallCell
         }
-        const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
-        int32_t* items = d.cell_items + (size_t)f * L.N + start;
-        const int radial = cell % s.rings;
-        if (n <= static_cast<uint32_t>(kWave)) {
-            sort_cell_regs<1>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
-            continue;
-        }
-        if (n <= static_cast<uint32_t>(2 * kWave)) {
-            sort_cell_regs<2>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
-            continue;
-        }
-        if (n <= static_cast<uint32_t>(4 * kWave)) {
-            sort_cell_regs<4>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
-            continue;
-        }
-        sort_cell_regs<8>(d, fd, items, static_cast<int>(n), radial, zmin + cell);  // n <= kSmallCell
     }
 }
 
