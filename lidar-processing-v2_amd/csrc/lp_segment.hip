@@ -421,36 +421,64 @@ __global__ __launch_bounds__(1024) void k_cell_sort_big(DevBuf d, Layout L) {
 // RECM recurrence along each azimuth slice (segmenter.cpp:215-267).  The
 // 49 float additions must happen in order (a log-step min-plus scan would
 // re-associate them), so each slice is one sequential lane.
-__global__ void k_recm(DevBuf d, Layout L) {
+// One wavefront per 64 slices.  A slice's cells are contiguous (cell =
+// slice * rings + r), so the wavefront's 64 x rings minima are one
+// contiguous block: it is loaded into LDS with coalesced reads, each lane runs
+// its slice's recurrence there, and the elevations go back the same way.
+// Grids with more than kRecmRings rings read and write global memory
+// directly (eight rings' minima loaded ahead of their eight steps).
+constexpr int kRecmRings = 64;
+
+__global__ __launch_bounds__(kWave) void k_recm(DevBuf d, Layout L) {
+    __shared__ float blk[kWave * kRecmRings];
     const int f = blockIdx.y;
     const SegConst& s = d.frames[f].sc;
-    const int sl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sl >= s.slices) return;
+    const int s0 = blockIdx.x * kWave;
+    if (s0 >= s.slices) return;
+    const int lane = lane_id();
+    const int sl = s0 + lane;
+    const int R = s.rings;
     const size_t co = (size_t)f * L.cell_stride;
-    const int off = sl * s.rings;
     float* E = d.elev + co;
     const float* zm = d.zmin + co;
+    auto step = [&](float prev, float z) {
+        if (f2u(z) == kEmptyZ) return prev + s.dz;
+        const float b = prev + s.dz;
+        return (b < z) ? b : z;  // std::min(curr_z_min, prev + dz)
+    };
+    if (R <= kRecmRings) {
+        const int ns = min(kWave, s.slices - s0);
+        const int total = ns * R;
+        const float* src = zm + (size_t)s0 * R;
+        for (int i = lane; i < total; i += kWave) blk[i] = src[i];
+        __syncthreads();
+        if (lane < ns) {
+            float* row = blk + lane * R;
+            float prev = s.e0;
+            row[0] = prev;
+            for (int r = 1; r < R; ++r) {
+                prev = step(prev, row[r]);
+                row[r] = prev;
+            }
+        }
+        __syncthreads();
+        float* dst = E + (size_t)s0 * R;
+        for (int i = lane; i < total; i += kWave) dst[i] = blk[i];
+        return;
+    }
+    if (sl >= s.slices) return;
+    const int off = sl * R;
     E[off] = s.e0;
     float prev = s.e0;
-    // Eight rings' minima are loaded together ahead of their eight
-    // recurrence steps (the loads do not depend on the recurrence).
-    for (int r0 = 1; r0 < s.rings; r0 += 8) {
+    for (int r0 = 1; r0 < R; r0 += 8) {
         float zz[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) zz[k] = (r0 + k < s.rings) ? zm[off + r0 + k] : 0.0f;
+        for (int k = 0; k < 8; ++k) zz[k] = (r0 + k < R) ? zm[off + r0 + k] : 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            if (r0 + k >= s.rings) break;
-            const float z = zz[k];
-            float e;
-            if (f2u(z) == kEmptyZ) {
-                e = prev + s.dz;
-            } else {
-                const float b = prev + s.dz;
-                e = (b < z) ? b : z;  // std::min(curr_z_min, prev + dz)
-            }
-            E[off + r0 + k] = e;
-            prev = e;
+            if (r0 + k >= R) break;
+            prev = step(prev, zz[k]);
+            E[off + r0 + k] = prev;
         }
     }
 }
@@ -798,7 +826,8 @@ void launch_segment_front(lp_context* c, int B) {
     hipLaunchKernelGGL(k_cell_sort, dim3(std::min(1024, std::max(16, 2048 / B)), B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_cell_sort_big, dim3(64, B), dim3(1024), 0, st, d, L);
     stage_mark(c, "recm");
-    hipLaunchKernelGGL(k_recm, dim3((L.cand_stride / kRansacRadialBins + 255) / 256, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_recm, dim3((L.cand_stride / kRansacRadialBins + kWave - 1) / kWave, B), dim3(kWave), 0, st, d,
+                       L);
     stage_mark(c, "ransac");
     // 128 workgroups (512 wavefronts) per frame walk the slices x 4 near cells.
     hipLaunchKernelGGL(k_cand_count, dim3(128, B), dim3(256), 0, st, d, L);
