@@ -574,7 +574,10 @@ def main():
     launches = {}
     for name, _ in g["stages"]:
         launches[name] = launches.get(name, 0) + 1
-    cand = [(s, ms) for s, ms in totals.items() if s in single]
+    # Among the stages on the scope's path (kernels with a share of its
+    # per-frame bytes); the DROR and box stages beside it have none.
+    cand = [(s, ms) for s, ms in totals.items()
+            if s in single and kernel_bytes_8d(single[s], 1, 1, 1, 1, 1) > 0]
     dom, dom_ms = max(cand, key=lambda t: t[1])
     kern = single[dom]
     dom_bytes = kernel_bytes_8d(kern, n, P, n_obs, vox, hv) * B
