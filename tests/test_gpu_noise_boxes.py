@@ -157,6 +157,30 @@ def test_filter_off_again_restores_plain_path(ctx, oracle):
     assert np.array_equal(out["seg_labels"], ref["seg_labels"])
 
 
+def test_shared_hash_stays_clean_across_stages(ctx, oracle):
+    """The noise cells and the voxels share one hash table per frame, which
+    each stage leaves clean for the next (only its own slots are reset).
+    Alternating the stages on one context must give what each gives alone."""
+    f2, f1 = synth.workload_c2(1), synth.workload_c1(2)
+    ref_noise, _ = oracle.noise_definition(f2.x, f2.y, f2.z)
+    sco, scg = cfgs(oracle, f1)
+    ref_plain = oracle.process_frame(f1.x, f1.y, f1.z, f1.ring, scfg=sco)
+    for round_ in range(2):
+        print(f"shared hash round {round_}", flush=True)
+        st, lab = ctx.remove_noise(f2.x, f2.y, f2.z)
+        assert st == 0 and np.array_equal(lab, ref_noise)
+        out = ctx.process([(f1.x, f1.y, f1.z, f1.ring)], [scg], use_graph=bool(round_))[0]
+        assert out["status"] == 0
+        assert np.array_equal(out["seg_labels"], ref_plain["seg_labels"])
+        assert np.array_equal(out["cluster_labels"], ref_plain["cluster_labels"])
+        ctx.set_noise_filter(N.noise_config())
+        try:
+            fused = ctx.process([(f2.x, f2.y, f2.z, f2.ring)], [cfgs(oracle, f2)[1]], use_graph=bool(round_))[0]
+        finally:
+            ctx.set_noise_filter(None)
+        assert fused["status"] == 0 and np.array_equal(fused["noise_labels"], ref_noise)
+
+
 def test_cpp_dropin_full_pipeline(tmp_path, oracle):
     """NoiseRemover -> Segmenter -> Clusterer -> Polygonizer (hulls and boxes)
     through the drop-in C++ classes (cpp/processor_example.cpp, mode full)
