@@ -528,16 +528,16 @@ def main():
                                        elevation_down_deg=f.fov_down_deg) for f in frames]
 
         knobs = ("streams", "hash_load_factor", "point_workgroup", "mark_tile", "persistent")
-        for name, frames in (("c3", [synth.workload_c3(i) for i in range(32)]),
+        for name, bucket in (("c3", [synth.workload_c3(i) for i in range(32)]),
                              ("c4", [synth.workload_c4(i) for i in range(16)])):
             log(f"autotune {name}")
-            tuned = autotune.tune(frames, cfgs_of(frames), streams=(1, 2, 4), loads=(0.5, 0.7),
+            tuned = autotune.tune(bucket, cfgs_of(bucket), streams=(1, 2, 4), loads=(0.5, 0.7),
                                   workgroups=(64, 128, 256), tiles=(64, 128, 256), persistent=(False, True), steps=5,
                                   noise=False, boxes=False)
             extra[f"{name}_autotune"] = tuned
             log(f"batched {name} (tuned)")
             best = {k: tuned["best"][k] for k in knobs}
-            extra[f"batched_{name}"] = run_batched(frames, 5, streams=best["streams"], load=best["hash_load_factor"],
+            extra[f"batched_{name}"] = run_batched(bucket, 5, streams=best["streams"], load=best["hash_load_factor"],
                                                    point_workgroup=best["point_workgroup"],
                                                    mark_tile=best["mark_tile"], persistent=best["persistent"])
         if args.serve_seconds > 0:
