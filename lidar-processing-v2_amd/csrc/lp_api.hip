@@ -154,7 +154,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.rring, B * N);
     go(x.noise, B * N);
     go(x.dsort, B * N);
-    go(x.didx, B * N);
+    go(x.spos, B * N);
     go(x.nocc, B * kNoiseOccWords);
     go(x.ncell_slot, B * N);
     go(x.ncell_start, B * N);
@@ -173,6 +173,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.vslot, B * N);
     go(x.cl_flag, B * N);
     go(x.clabel, B * N);
+    x.nloc = x.clabel;
     go(x.cell_count, B * C);
     go(x.cell_start, B * (C + 1));
     go(x.cell_cursor, B * C);
@@ -217,6 +218,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.cl_cursor, B * N);
     x.hull_flag = x.cl_cursor;
     go(x.cl_members, B * N);
+    x.nopen = x.cl_members;
     go(x.hcx, 2 * B * N);
     go(x.hcy, 2 * B * N);
     go(x.hci, 2 * B * N);

@@ -97,7 +97,9 @@ struct DevBuf {
     uint16_t* rring;
     uint8_t* noise;
     float4* dsort;
-    int32_t* didx;
+    int32_t* spos;           // per input point (frame stride N): its position in dsort
+    int32_t* nopen;          // = cl_members (free until clustering): points the input-order pass left open
+    int32_t* nloc;           // = clabel (free until clustering): position of each point inside its cell
     uint32_t* nocc;          // per frame: kNoiseOccWords words, one bit per occupied 1 m (x, y) column
     int32_t* ncell_slot;     // per frame (stride N): hash slot of each occupied noise cell, by dense id
     uint32_t* ncell_start;   // per frame (stride N): first cell-sorted position of each cell, by dense id
@@ -236,6 +238,7 @@ enum FsSlot : int {
     // kept across the per-run clear: written by the noise filter before it
     kFsKept,           // points kept by the noise filter
     kFsNoise,          // points labelled NOISE
+    kFsNoiseOpen,      // points the input-order pass left undecided
     kFsNoiseHard,      // points left to the neighbour-cell search
     kFsNoiseCells,     // occupied noise-filter cells
     kFsNoiseErr,       // error bits of the noise filter (kErrNoise*)

@@ -12,20 +12,23 @@
 // order keeps a point's azimuth neighbours of the same beam next to it, so
 // this decides almost every VALID point of a surface with 17 coalesced
 // distance tests (any min_neighbours points within the radius prove VALID).
-// The rest go through the grid.
+// The rest are listed (the open list) and go through the grid.
 //
 // Layout: the frame's points are bucketed into 1 m grid cells through an
 // open-addressing hash of cell keys (one slot per occupied cell, load at most
 // the context's hash load factor), counted, scanned and scattered into a
-// cell-sorted float4 array (w holds the cell's hash slot).  The count runs in two passes.  The first, one thread per
-// point in cell order, checks the point's own cell outward from its own
+// cell-sorted float4 array (w holds the cell's hash slot).  The same pass
+// that runs the input-order test inserts the point and takes its place in
+// the cell from the cell count's atomic add, so the scatter needs no
+// atomics.  The grid count runs in two passes.  The first, one
+// thread per open point, checks the point's own cell outward from its own
 // position (points next to each other in scan order are mostly neighbours in
-// space), so almost every point of a dense surface reaches min_neighbours
-// after a few distance tests; a point whose ball lies inside its cell is
-// decided there too.  The rest (rain, isolated returns, points near a cell
-// face) are listed and the second pass, one thread per listed point, visits
-// the cube of cells within ceil(r) + 1 nearest first: the slow searches run
-// side by side instead of holding up wavefronts of easy points.
+// space), so almost every open point of a dense surface reaches
+// min_neighbours after a few distance tests; a point whose ball lies inside
+// its cell is decided there too.  The rest (rain, isolated returns, points
+// near a cell face) are listed and the second pass, one wavefront per listed
+// point, visits the cube of cells within ceil(r) + 1 nearest first: the slow
+// searches run side by side instead of holding up wavefronts of easy points.
 #include "lp_device.h"
 #include "lp_internal.h"
 #include "lp_kernels.h"
@@ -91,6 +94,7 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
     if (t0 == 0) {
         d.fs[f * kFsCount + kFsKept] = 0;
         d.fs[f * kFsCount + kFsNoise] = 0;
+        d.fs[f * kFsCount + kFsNoiseOpen] = 0;
         d.fs[f * kFsCount + kFsNoiseHard] = 0;
         d.fs[f * kFsCount + kFsNoiseCells] = 0;
         d.fs[f * kFsCount + kFsNoiseErr] = 0;
@@ -99,10 +103,14 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
 
 constexpr int kNearWindow = 8;  // input-order neighbours tested on each side
 
-// Input-order pass: noise[i] = 0 (VALID) when at least min_neighbours of the
-// points i - 8 .. i + 8 (i itself included) lie within the radius, else 2
-// (undecided, labelled by the grid passes).
-__global__ __launch_bounds__(256) void k_noise_near(DevBuf d, Layout L) {
+// One thread per input point, two jobs on the same coordinates:
+//  * input-order pass: noise[i] = 0 (VALID) when at least min_neighbours of
+//    the points i - 8 .. i + 8 (i itself included) lie within the radius,
+//    else 2, and the point joins the open list for the grid passes;
+//  * grid insert: the point's cell gets a hash slot (vslot) and the point a
+//    position inside its cell (nloc): the cell's count before the point's run
+//    plus its rank in the run, so the scatter needs no atomics.
+__global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     __shared__ float4 tile_pts[256 + 2 * kNearWindow];
     int tile, f;
     xcd_tile_frame(tile, f);
@@ -132,18 +140,15 @@ __global__ __launch_bounds__(256) void k_noise_near(DevBuf d, Layout L) {
     const bool valid = cnt >= nc.k;
     const size_t po = (size_t)f * L.N;
     d.noise[po + i] = valid ? 0 : 2;
-    if (valid) d.obs_flag[po + i] = 1u;
-}
+    if (valid) {
+        d.obs_flag[po + i] = 1u;
+    } else {
+        const uint32_t q = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseOpen]);
+        d.nopen[po + q] = i;
+    }
 
-__global__ void k_noise_insert(DevBuf d, Layout L) {
-    int tile, f;
-    xcd_tile_frame(tile, f);
-    const FrameDesc& fd = d.frames[f];
-    const int i = tile * blockDim.x + threadIdx.x;
-    if (i >= fd.n_raw) return;
-    const int gi = fd.pt_off + i;
     int cx, cy, cz;
-    noise_cell(d.rx[gi], d.ry[gi], d.rz[gi], cx, cy, cz);
+    noise_cell(t.x, t.y, t.z, cx, cy, cz);
     const int32_t key = noise_key(cx, cy, cz);
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const size_t ho = (size_t)f * L.hash_cap;
@@ -151,7 +156,7 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
     // inserts once: its lowest lane probes and adds the run's count; all
     // runs at once.
     const WaveRun run = wave_runs(key);
-    uint32_t h = 0;
+    uint32_t h = 0, before = 0;
     bool fresh = false;
     if (lane_id() == run.leader) {
         h = noise_slot(key, mask);
@@ -161,7 +166,7 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
             if (prev == -1 || prev == key) break;
             h = (h + 1) & mask;
         }
-        atomicAdd(&d.hcount[ho + h], run.count);
+        before = atomicAdd(&d.hcount[ho + h], run.count);
         atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
     }
     // Every occupied cell gets a dense id (in any order): the cell counts are
@@ -171,12 +176,13 @@ __global__ void k_noise_insert(DevBuf d, Layout L) {
         d.ncell_slot[(size_t)f * L.N + id] = static_cast<int32_t>(h);
         d.hdense[ho + h] = static_cast<int32_t>(id);
     }
-    const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
-    d.vslot[(size_t)f * L.N + i] = slot;
+    d.vslot[po + i] = static_cast<int32_t>(__shfl(h, run.leader, kWave));
+    d.nloc[po + i] = static_cast<int32_t>(__shfl(before, run.leader, kWave) + run.rank);
 }
 
 // ncell_start holds the exclusive scan of the cell counts by dense id on
-// entry; the first position of every slot's cell goes to hstart.
+// entry.  Each point goes to its cell's start plus its place in the cell;
+// the point at place 0 writes the cell's start to hstart.
 __global__ void k_noise_scatter(DevBuf d, Layout L) {
     int tile, f;
     xcd_tile_frame(tile, f);
@@ -184,16 +190,15 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const int i = tile * blockDim.x + threadIdx.x;
     if (i >= fd.n_raw) return;
     const int gi = fd.pt_off + i;
-    const size_t ho = (size_t)f * L.hash_cap;
-    const int32_t h = d.vslot[(size_t)f * L.N + i];
-    const int start = static_cast<int>(d.ncell_start[(size_t)f * L.N + d.hdense[ho + h]]);
-    // hmin is the cell's cursor here, counting up from its clean value.
-    const uint32_t cur = wave_atomic_inc(reinterpret_cast<uint32_t*>(&d.hmin[ho + h]));
-    const int pos = start + static_cast<int>(cur - static_cast<uint32_t>(kHashMinClean));
-    if (pos == start) d.hstart[ho + h] = start;  // once per cell: the point that took its first position
     const size_t po = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    const int32_t h = d.vslot[po + i];
+    const int loc = d.nloc[po + i];
+    const int start = static_cast<int>(d.ncell_start[po + d.hdense[ho + h]]);
+    if (loc == 0) d.hstart[ho + h] = start;
+    const int pos = start + loc;
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
-    d.didx[po + pos] = i;
+    d.spos[po + i] = pos;
 }
 
 // Squared distance from the point to the box of cell offset (dx, dy, dz)
@@ -217,22 +222,9 @@ __device__ __forceinline__ void noise_label(const DevBuf& d, int f, size_t po, i
 
 constexpr int kOwnCellBudget = 64;  // distance tests of the own-cell pass
 
-// Own-cell pass.  A point is VALID once min_neighbours points of its cell
-// are within its radius (the count includes itself, and the search order
-// does not change the outcome), NOISE when its whole cell has been checked
-// and no other cell comes within its radius, and listed otherwise.
-__global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
-    int tile, f;
-    xcd_tile_frame(tile, f);
-    const FrameDesc& fd = d.frames[f];
-    const int n = fd.n_raw;
-    const int pos = tile * blockDim.x + threadIdx.x;
-    if (pos >= n) return;
-    const NoiseConst nc = *d.nc;
-    const size_t po = (size_t)f * L.N;
-    const size_t ho = (size_t)f * L.hash_cap;
-    const int i = d.didx[po + pos];
-    if (d.noise[po + i] == 0) return;  // decided by the input-order pass
+// Own-cell test of point i at cell-sorted position pos (see k_noise_count).
+__device__ __forceinline__ void noise_own_cell(const DevBuf& d, const NoiseConst& nc, int f, size_t po, size_t ho,
+                                               int i, int pos) {
     const float4 t = d.dsort[po + pos];
     const float r2 = noise_radius_sqr(nc, t.x, t.y);
     const uint32_t k = nc.k;
@@ -278,10 +270,29 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
     }
     if (state == 2) {
         const uint32_t q = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseHard]);
-        d.vslot[po + q] = pos;
+        d.vslot[po + q] = i;
         d.obs_src[po + q] = static_cast<int32_t>(carry);
     } else {
         noise_label(d, f, po, i, state == 1);
+    }
+}
+
+// Own-cell pass over the open list (a grid-stride loop).  A point is VALID
+// once min_neighbours points of its cell are within its radius (the count
+// includes itself, and the search order does not change the outcome), NOISE
+// when its whole cell has been checked and no other cell comes within its
+// radius, and listed for the neighbour-cell pass otherwise.
+__global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
+    int tile, f;
+    xcd_tile_frame(tile, f);
+    const int nopen = static_cast<int>(d.fs[f * kFsCount + kFsNoiseOpen]);
+    const NoiseConst nc = *d.nc;
+    const size_t po = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    for (int q = tile * blockDim.x + threadIdx.x; q < nopen; q += gridDim.x * blockDim.x) {
+        const int i = d.nopen[po + q];
+        const int pos = d.spos[po + i];
+        noise_own_cell(d, nc, f, po, ho, i, pos);
     }
 }
 
@@ -322,7 +333,8 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
     const uint32_t* occ = d.nocc + (size_t)f * kNoiseOccWords;
     uint32_t noise_here = 0;
     for (int q = tile * (blockDim.x / kWave) + threadIdx.x / kWave; q < nhard; q += waves) {
-        const int pos = d.vslot[po + q];
+        const int i = d.vslot[po + q];
+        const int pos = d.spos[po + i];
         const uint32_t carry = static_cast<uint32_t>(d.obs_src[po + q]);
         const float4 t = d.dsort[po + pos];
         const float r2 = noise_radius_sqr(nc, t.x, t.y);
@@ -389,7 +401,6 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
         }
         if (lane == 0) {
             const bool noise = cnt < k;
-            const int i = d.didx[po + pos];
             d.noise[po + i] = noise ? 1 : 0;
             d.obs_flag[po + i] = noise ? 0u : 1u;
             noise_here += noise ? 1u : 0u;
@@ -426,12 +437,13 @@ void launch_noise(lp_context* c, int B, bool compact) {
     const int nb = (L.N + 255) / 256;
     stage_mark(c, "noise");
     hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_noise_near, dim3(nb, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_noise_insert, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_noise_prep, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan_gather(c, B, d.hcount, L.hash_cap, d.ncell_slot, d.ncell_start, L.N, kCountSlot, kFsNoiseCells, -1);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_count");
-    hipLaunchKernelGGL(k_noise_count, dim3(nb, B), dim3(256), 0, st, d, L);
+    // The open list is a few percent of the points: a quarter of the point
+    // grid, striding over the list.
+    hipLaunchKernelGGL(k_noise_count, dim3((nb + 3) / 4, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_search");
     // About 8192 wavefronts in all (8 per SIMD: the search is latency-bound),
     // at least 128 and at most 1024 per frame.
