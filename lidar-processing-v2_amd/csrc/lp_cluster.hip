@@ -28,20 +28,73 @@ constexpr float kHalfPiF = 1.57079637050628662109f;  // M_PI_2f
 constexpr int kCcLds = 32768;                        // voxels per frame for the LDS union-find
 
 // Obstacle split: input-order compaction of OBSTACLE points
-// (processor.cpp:562-579); cl_flag holds the scan of obs_flag on entry.
-__global__ void k_obs_gather(DevBuf d, Layout L) {
+// (processor.cpp:562-579), in two passes over tiles of 2048 points (eight
+// rows of 256, coalesced) straight from the labels: the obstacle count of
+// every tile, then each tile's compaction after the counts of the tiles
+// before it, which moves the obstacles' index and x, y, z.  cl_flag receives
+// every point's exclusive obstacle count (the split into ground and unknown
+// clouds reads it); the last tile writes the obstacle count.
+constexpr int kObsThreads = 256;
+constexpr int kObsRows = 8;
+constexpr int kObsTile = kObsThreads * kObsRows;
+
+__device__ __forceinline__ uint32_t obstacle_bits(const uint32_t* lab, int base, int n) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kObsRows; ++k) {
+        const int i = base + k * kObsThreads + static_cast<int>(threadIdx.x);
+        m |= (i < n && lab[i] == LP_LABEL_OBSTACLE) ? (1u << k) : 0u;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kObsThreads) void k_obs_count(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kObsThreads / kWave + 1];
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= fd.n) return;
-    const int gi = fd.pt_off + i;
-    if (d.labels[gi] != LP_LABEL_OBSTACLE) return;
+    const int n = fd.n;
+    const int base = blockIdx.x * kObsTile;
+    if (base >= n) return;
+    const uint32_t m = obstacle_bits(d.labels + fd.pt_off, base, n);
+    uint32_t total;
+    block_exclusive_sum<kObsThreads>(static_cast<uint32_t>(__popc(m)), lds, &total);
+    if (threadIdx.x == 0) d.blk[(size_t)f * L.blk_stride + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kObsThreads) void k_obs_gather(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kObsRows * kObsThreads / kWave + 1];
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int n = fd.n;
+    const int tile = blockIdx.x;
+    const int base = tile * kObsTile;
+    if (base >= n) {
+        if (tile == 0 && threadIdx.x == 0) d.fs[f * kFsCount + kFsObstacles] = 0;
+        return;
+    }
+    const uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
+    uint32_t pre = 0;
+    for (int i = threadIdx.x; i < tile; i += kObsThreads) pre += blk[i];
+    uint32_t pre_total;
+    block_exclusive_sum<kObsThreads>(pre, lds, &pre_total);
+    const uint32_t m = obstacle_bits(d.labels + fd.pt_off, base, n);
+    uint32_t pos[kObsRows];
+    const uint32_t total = tile_compact<kObsThreads, kObsRows>(m, pos, lds);
     const size_t ob = (size_t)f * L.N;
-    const uint32_t k = d.cl_flag[ob + i];
-    d.obs_src[ob + k] = i;
-    d.ox[ob + k] = d.x[gi];
-    d.oy[ob + k] = d.y[gi];
-    d.oz[ob + k] = d.z[gi];
+#pragma unroll
+    for (int k = 0; k < kObsRows; ++k) {
+        const int i = base + k * kObsThreads + static_cast<int>(threadIdx.x);
+        const uint32_t o = pre_total + pos[k];
+        if (i < n) d.cl_flag[ob + i] = o;
+        if ((m >> k) & 1u) {
+            const int gi = fd.pt_off + i;
+            d.obs_src[ob + o] = i;
+            d.ox[ob + o] = d.x[gi];
+            d.oy[ob + o] = d.y[gi];
+            d.oz[ob + o] = d.z[gi];
+        }
+    }
+    if (threadIdx.x == 0 && base + kObsTile >= n) d.fs[f * kFsCount + kFsObstacles] = pre_total + total;
 }
 
 // Ground and unsegmented clouds of the same split (processor.cpp:565-577):
@@ -447,8 +500,9 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     const int nb = (L.N + 255) / 256;
     stage_mark(c, "split");
     if (from_split) {
-        launch_scan(c, B, d.obs_flag, d.cl_flag, L.N, kCountPoints, 0, kFsObstacles);
-        hipLaunchKernelGGL(k_obs_gather, dim3(nb, B), dim3(256), 0, st, d, L);
+        const int nt = (L.N + kObsTile - 1) / kObsTile;  // at most blk_stride tiles
+        hipLaunchKernelGGL(k_obs_count, dim3(nt, B), dim3(kObsThreads), 0, st, d, L);
+        hipLaunchKernelGGL(k_obs_gather, dim3(nt, B), dim3(kObsThreads), 0, st, d, L);
         if (c->clouds_on) {
             launch_scan(c, B, d.gnd_flag, d.obs_flag, L.N, kCountPoints, 0, kFsGround);
             hipLaunchKernelGGL(k_label_split, dim3(nb, B), dim3(256), 0, st, d, L);

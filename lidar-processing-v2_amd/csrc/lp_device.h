@@ -378,4 +378,36 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T* lds, T* total) {
     return res;
 }
 
+// Stream compaction inside one tile of NT * R elements held row by row
+// (element k * NT + threadIdx.x of the tile is the thread's element of row
+// k, so every row is read and written with coalesced accesses).  bits has
+// bit k set when the thread's element of row k is kept; pos[k] receives the
+// number of kept elements before it in tile order.  Returns the tile's kept
+// count.  lds holds R * NT / 64 + 1 words.
+template <int NT, int R>
+__device__ __forceinline__ uint32_t tile_compact(uint32_t bits, uint32_t (&pos)[R], uint32_t* lds) {
+    constexpr int W = NT / kWave;
+    static_assert(R * W <= kWave, "one wavefront scans the row totals");
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    uint64_t bal[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        bal[k] = __ballot((bits >> k) & 1u);
+        if (lane_id() == 0) lds[k * W + w] = static_cast<uint32_t>(__popcll(bal[k]));
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        const uint32_t v = threadIdx.x < R * W ? lds[threadIdx.x] : 0u;
+        const uint32_t inc = wave_inclusive_sum(v);
+        if (threadIdx.x < R * W) lds[threadIdx.x] = inc - v;
+        if (threadIdx.x == kWave - 1) lds[R * W] = inc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) pos[k] = lds[k * W + w] + lanes_below(bal[k]);
+    const uint32_t total = lds[R * W];
+    __syncthreads();
+    return total;
+}
+
 }  // namespace lpd

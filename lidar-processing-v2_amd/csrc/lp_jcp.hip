@@ -487,7 +487,6 @@ __global__ __launch_bounds__(kSolveThreads) void k_jcp_solve(DevBuf d, Layout L)
 
 // Labels per input point (populateLabels, segmenter.cpp:640-669): only the
 // winner of a pixel gets its pixel's final label; PENDING stays UNKNOWN.
-// Also flags obstacle points for the input-order split of processor.cpp:562-579.
 __device__ __forceinline__ void seg_labels_tile(const DevBuf& d, const Layout& L, int f, int tile) {
     const FrameDesc& fd = d.frames[f];
     const int i = tile * blockDim.x + threadIdx.x;
@@ -504,7 +503,6 @@ __device__ __forceinline__ void seg_labels_tile(const DevBuf& d, const Layout& L
         }
     }
     d.labels[gi] = lab;
-    d.obs_flag[(size_t)f * L.N + i] = lab == LP_LABEL_OBSTACLE ? 1u : 0u;
     if (d.gnd_flag) d.gnd_flag[(size_t)f * L.N + i] = lab == LP_LABEL_GROUND ? 1u : 0u;
 }
 

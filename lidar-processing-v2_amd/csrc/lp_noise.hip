@@ -140,9 +140,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     const bool valid = cnt >= nc.k;
     const size_t po = (size_t)f * L.N;
     d.noise[po + i] = valid ? 0 : 2;
-    if (valid) {
-        d.obs_flag[po + i] = 1u;
-    } else {
+    if (!valid) {
         const uint32_t q = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseOpen]);
         d.nopen[po + q] = i;
     }
@@ -214,7 +212,6 @@ __device__ __forceinline__ float cell_gap_sqr(float fx, float fy, float fz, int 
 // Writes the label of point i and counts NOISE labels per wavefront.
 __device__ __forceinline__ void noise_label(const DevBuf& d, int f, size_t po, int i, bool noise) {
     d.noise[po + i] = noise ? 1 : 0;
-    d.obs_flag[po + i] = noise ? 0u : 1u;  // keep flag for the compaction scan
     const uint64_t m = __ballot(noise);
     if (m && lane_id() == __ffsll(static_cast<unsigned long long>(m)) - 1)
         atomicAdd(&d.fs[f * kFsCount + kFsNoise], static_cast<uint32_t>(__popcll(m)));
@@ -402,30 +399,84 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
         if (lane == 0) {
             const bool noise = cnt < k;
             d.noise[po + i] = noise ? 1 : 0;
-            d.obs_flag[po + i] = noise ? 0u : 1u;
             noise_here += noise ? 1u : 0u;
         }
     }
     if (lane == 0 && noise_here) atomicAdd(&d.fs[f * kFsCount + kFsNoise], noise_here);
 }
 
-// Stable compaction of the kept points into the segmenter's inputs; the
-// frame's point count becomes the kept count.
-__global__ void k_noise_compact(DevBuf d, Layout L) {
-    int tile, f;
-    xcd_tile_frame(tile, f);
+// Stable compaction of the kept points (label 0) into the segmenter's
+// inputs, in two passes over tiles of 2048 points (eight rows of 256, each
+// read and written with coalesced accesses): the kept count of every tile,
+// then each tile's compaction after the counts of the tiles before it, which
+// moves x, y, z, ring to their places.  The keep flags are the label bytes;
+// the last tile writes the kept count, which becomes the frame's point count.
+constexpr int kKeepThreads = 256;
+constexpr int kKeepRows = 8;
+constexpr int kKeepTile = kKeepThreads * kKeepRows;
+
+__device__ __forceinline__ uint32_t keep_bits(const uint8_t* lab, int base, int n) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kKeepRows; ++k) {
+        const int i = base + k * kKeepThreads + static_cast<int>(threadIdx.x);
+        m |= (i < n && lab[i] == 0) ? (1u << k) : 0u;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kKeepThreads) void k_keep_count(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kKeepThreads / kWave + 1];
+    const int f = blockIdx.y;
+    const int n = d.frames[f].n_raw;
+    const int base = blockIdx.x * kKeepTile;
+    if (base >= n) return;
+    const uint32_t m = keep_bits(d.noise + (size_t)f * L.N, base, n);
+    uint32_t total;
+    block_exclusive_sum<kKeepThreads>(static_cast<uint32_t>(__popc(m)), lds, &total);
+    if (threadIdx.x == 0) d.blk[(size_t)f * L.blk_stride + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kKeepThreads) void k_keep_move(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kKeepRows * kKeepThreads / kWave + 1];
+    const int f = blockIdx.y;
     FrameDesc& fd = d.frames[f];
     const int n = fd.n_raw;
-    const int i = tile * blockDim.x + threadIdx.x;
-    const size_t po = (size_t)f * L.N;
-    if (tile == 0 && threadIdx.x == 0) fd.n = static_cast<int32_t>(d.fs[f * kFsCount + kFsKept]);
-    if (i >= n || !d.obs_flag[po + i]) return;
-    const int gi = fd.pt_off + i;
-    const int go = fd.pt_off + static_cast<int>(reinterpret_cast<const uint32_t*>(d.obs_src)[po + i]);
-    d.x[go] = d.rx[gi];
-    d.y[go] = d.ry[gi];
-    d.z[go] = d.rz[gi];
-    if (fd.has_ring) d.ring[go] = d.rring[gi];
+    const int tile = blockIdx.x;
+    const int base = tile * kKeepTile;
+    if (base >= n) {
+        if (tile == 0 && threadIdx.x == 0) {
+            d.fs[f * kFsCount + kFsKept] = 0;
+            fd.n = 0;
+        }
+        return;
+    }
+    const uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
+    uint32_t pre = 0;
+    for (int i = threadIdx.x; i < tile; i += kKeepThreads) pre += blk[i];
+    uint32_t pre_total;
+    block_exclusive_sum<kKeepThreads>(pre, lds, &pre_total);
+    const uint32_t m = keep_bits(d.noise + (size_t)f * L.N, base, n);
+    uint32_t pos[kKeepRows];
+    const uint32_t total = tile_compact<kKeepThreads, kKeepRows>(m, pos, lds);
+    const int off = fd.pt_off;
+    const bool ring = fd.has_ring != 0;
+#pragma unroll
+    for (int k = 0; k < kKeepRows; ++k) {
+        if ((m >> k) & 1u) {
+            const int gi = off + base + k * kKeepThreads + static_cast<int>(threadIdx.x);
+            const int go = off + static_cast<int>(pre_total + pos[k]);
+            d.x[go] = d.rx[gi];
+            d.y[go] = d.ry[gi];
+            d.z[go] = d.rz[gi];
+            if (ring) d.ring[go] = d.rring[gi];
+        }
+    }
+    if (threadIdx.x == 0 && base + kKeepTile >= n) {
+        const uint32_t kept = pre_total + total;
+        d.fs[f * kFsCount + kFsKept] = kept;
+        fd.n = static_cast<int32_t>(kept);
+    }
 }
 
 }  // namespace
@@ -455,11 +506,9 @@ void launch_noise(lp_context* c, int B, bool compact) {
 }
 
 void launch_noise_compact(lp_context* c, int B) {
-    DevBuf& d = c->d;
-    const Layout& L = c->L;
-    const int nb = (L.N + 255) / 256;
-    launch_scan(c, B, d.obs_flag, reinterpret_cast<uint32_t*>(d.obs_src), L.N, kCountRawPoints, 0, kFsKept);
-    hipLaunchKernelGGL(k_noise_compact, dim3(nb, B), dim3(256), 0, c->stream, d, L);
+    const int nt = (c->L.N + kKeepTile - 1) / kKeepTile;  // at most blk_stride tiles
+    hipLaunchKernelGGL(k_keep_count, dim3(nt, B), dim3(kKeepThreads), 0, c->stream, c->d, c->L);
+    hipLaunchKernelGGL(k_keep_move, dim3(nt, B), dim3(kKeepThreads), 0, c->stream, c->d, c->L);
 }
 
 }  // namespace lp
