@@ -137,7 +137,7 @@ struct DevBuf {
     uint8_t* state;
     int32_t* mapping;
     float4* pxyz;
-    uint32_t* qflag;
+    uint8_t* qflag;          // 1: the pixel joins the JCP queue
     int32_t* qmap;
     // JCP queue (frame stride pix_stride)
     int32_t* qlist;

@@ -74,27 +74,77 @@ __global__ __launch_bounds__(T) void k_jcp_mark(DevBuf d, Layout L) {
 #pragma unroll
             for (int dc = 0; dc < 5; ++dc) q |= win[dr][threadIdx.x + dc] == kObstacle ? 1u : 0u;
     }
-    d.qflag[(size_t)f * L.P + r * W + c] = q;
+    d.qflag[(size_t)f * L.P + r * W + c] = static_cast<uint8_t>(q);
 }
 
-// qmap holds the exclusive scan of qflag on entry.  The scan value at a
-// row's first pixel is the row's first queue index (queue order is raster
-// order), kept for the row solve.
-__global__ void k_jcp_fill(DevBuf d, Layout L) {
+// The queue in raster order, from the flags in two passes over tiles of 2048
+// pixels (eight rows of 256, coalesced): the queue count of every tile, then
+// each tile's compaction after the counts of the tiles before it.  qlist[j]
+// is the pixel of queue entry j, qmap[p] the queue index of pixel p (-1 when
+// not queued), qrow[r] the first queue index at or after row r's first pixel
+// (kept for the row solve); the last tile writes the queue length.
+constexpr int kQueueThreads = 256;
+constexpr int kQueueRows = 8;
+constexpr int kQueueTile = kQueueThreads * kQueueRows;
+
+__device__ __forceinline__ uint32_t queue_bits(const uint8_t* qf, int base, int P) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kQueueRows; ++k) {
+        const int p = base + k * kQueueThreads + static_cast<int>(threadIdx.x);
+        m |= (p < P && qf[p] != 0) ? (1u << k) : 0u;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kQueueThreads) void k_jcp_count(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kQueueThreads / kWave + 1];
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
     const int P = fd.sc.H * fd.sc.W;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const size_t o = (size_t)f * L.P + p;
-    const int pos = d.qmap[o];
-    if (p % fd.sc.W == 0) d.qrow[(size_t)f * (L.H + 1) + p / fd.sc.W] = pos;
-    if (d.qflag[o]) {
-        d.qlist[(size_t)f * L.P + pos] = p;
-        d.qmap[o] = pos;
-    } else {
-        d.qmap[o] = -1;
+    const int base = blockIdx.x * kQueueTile;
+    if (base >= P) return;
+    const uint32_t m = queue_bits(d.qflag + (size_t)f * L.P, base, P);
+    uint32_t total;
+    block_exclusive_sum<kQueueThreads>(static_cast<uint32_t>(__popc(m)), lds, &total);
+    if (threadIdx.x == 0) d.blk[(size_t)f * L.blk_stride + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kQueueThreads) void k_jcp_fill(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kQueueRows * kQueueThreads / kWave + 1];
+    const int f = blockIdx.y;
+    const FrameDesc& fd = d.frames[f];
+    const int W = fd.sc.W;
+    const int P = fd.sc.H * W;
+    const int tile = blockIdx.x;
+    const int base = tile * kQueueTile;
+    if (base >= P) {
+        if (tile == 0 && threadIdx.x == 0) d.fs[f * kFsCount + kFsQueue] = 0;
+        return;
     }
+    const uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
+    uint32_t pre = 0;
+    for (int i = threadIdx.x; i < tile; i += kQueueThreads) pre += blk[i];
+    uint32_t pre_total;
+    block_exclusive_sum<kQueueThreads>(pre, lds, &pre_total);
+    const size_t po = (size_t)f * L.P;
+    const uint32_t m = queue_bits(d.qflag + po, base, P);
+    uint32_t pos[kQueueRows];
+    const uint32_t total = tile_compact<kQueueThreads, kQueueRows>(m, pos, lds);
+#pragma unroll
+    for (int k = 0; k < kQueueRows; ++k) {
+        const int p = base + k * kQueueThreads + static_cast<int>(threadIdx.x);
+        if (p >= P) continue;
+        const int j = static_cast<int>(pre_total + pos[k]);
+        if (p % W == 0) d.qrow[(size_t)f * (L.H + 1) + p / W] = j;
+        if ((m >> k) & 1u) {
+            d.qlist[po + j] = p;
+            d.qmap[po + p] = j;
+        } else {
+            d.qmap[po + p] = -1;
+        }
+    }
+    if (threadIdx.x == 0 && base + kQueueTile >= P) d.fs[f * kFsCount + kFsQueue] = pre_total + total;
 }
 
 __device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int v) {
@@ -568,8 +618,9 @@ void launch_jcp(lp_context* c, int B) {
         hipLaunchKernelGGL(k_jcp_mark<128>, dim3(L.H * ((L.W + 127) / 128), B), dim3(128), 0, st, d, L);
     else
         hipLaunchKernelGGL(k_jcp_mark<256>, dim3(L.H * ((L.W + 255) / 256), B), dim3(256), 0, st, d, L);
-    launch_scan(c, B, d.qflag, reinterpret_cast<uint32_t*>(d.qmap), L.P, kCountPixels, 0, kFsQueue);
-    hipLaunchKernelGGL(k_jcp_fill, dim3(pb, B), dim3(256), 0, st, d, L);
+    const int qt = (L.P + kQueueTile - 1) / kQueueTile;  // at most blk_stride tiles
+    hipLaunchKernelGGL(k_jcp_count, dim3(qt, B), dim3(kQueueThreads), 0, st, d, L);
+    hipLaunchKernelGGL(k_jcp_fill, dim3(qt, B), dim3(kQueueThreads), 0, st, d, L);
     stage_mark(c, "jcp_slots");
     hipLaunchKernelGGL(k_jcp_slots, dim3(pb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "jcp_solve");
