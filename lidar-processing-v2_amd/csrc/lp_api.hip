@@ -176,8 +176,9 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     x.nloc = x.clabel;
     go(x.cell_count, B * C);
     go(x.cell_start, B * (C + 1));
-    go(x.cell_cursor, B * C);
+    go(x.pt_loc, B * N);
     go(x.cell_items, B * N);
+    go(x.cell_zkey, B * N);
     go(x.zmin, B * C);
     go(x.elev, B * C);
     go(x.big_cells, B * C);

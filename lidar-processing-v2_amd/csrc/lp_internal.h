@@ -118,8 +118,9 @@ struct DevBuf {
     // per cell (frame stride cell_stride)
     uint32_t* cell_count;
     uint32_t* cell_start;    // stride cell_stride + 1 used
-    uint32_t* cell_cursor;
+    int32_t* pt_loc;         // per point (CSR): its place inside its polar cell
     int32_t* cell_items;     // frame stride N
+    uint32_t* cell_zkey;     // frame stride N: ord_f32(z) of the cell_items entries, in fill order
     float* zmin;
     float* elev;
     uint32_t* big_cells;     // per frame list of cells needing a workgroup sort

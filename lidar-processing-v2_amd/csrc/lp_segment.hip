@@ -33,10 +33,7 @@ __global__ void k_clear(DevBuf d, Layout L) {
     const int stride = gridDim.x * blockDim.x;
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (f == 0 && t0 < kWqCounters) d.wq[t0] = 0;
-    for (int c = t0; c < cells; c += stride) {
-        d.cell_count[(size_t)f * L.cell_stride + c] = 0;
-        d.cell_cursor[(size_t)f * L.cell_stride + c] = 0;
-    }
+    for (int c = t0; c < cells; c += stride) d.cell_count[(size_t)f * L.cell_stride + c] = 0;
     for (int p = t0; p < P; p += stride) d.pix_key[fd.pix_off + p] = ~0ull;
     for (int k = t0; k < kRansacIters; k += stride) d.hyp_count[f * kRansacIters + k] = 0;
     if (t0 < kFsKept) d.fs[f * kFsCount + t0] = 0;  // the noise filter's counts run ahead of this clear
@@ -79,7 +76,7 @@ __device__ __forceinline__ void project_tile(const DevBuf& d, const Layout& L, i
                 const int col = static_cast<uint16_t>((s.w_minus_1 * az) / kTwoPi);
                 cell = slice * s.rings + radial;
                 pix = row * s.W + col;
-                wave_atomic_inc(&d.cell_count[(size_t)f * L.cell_stride + cell]);
+                d.pt_loc[gi] = static_cast<int32_t>(wave_atomic_inc(&d.cell_count[(size_t)f * L.cell_stride + cell]));
                 const unsigned long long key = (static_cast<unsigned long long>(f2u(dd)) << 32) |
                                                (static_cast<unsigned long long>(slice) << L.idx_bits) |
                                                static_cast<unsigned long long>(i);
@@ -148,18 +145,20 @@ __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
     }
 }
 
-// Scatter point indices into their cells (unordered; sorted per cell next).
+// Scatter point indices into their cells (unordered; sorted per cell next),
+// each at the place the histogram's atomic add gave it, with its z key beside
+// it for the robust minimum.
 __global__ void k_cell_fill(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const FrameDesc& fd = d.frames[f];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= fd.n) return;
-    const int cell = d.pt_cell[fd.pt_off + i];
+    const int gi = fd.pt_off + i;
+    const int cell = d.pt_cell[gi];
     if (cell < 0) return;
-    const size_t co = (size_t)f * L.cell_stride;
-    const uint32_t slot = wave_atomic_inc(&d.cell_cursor[co + cell]);
-    const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
-    d.cell_items[(size_t)f * L.N + start + slot] = i;
+    const uint32_t pos = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell] + static_cast<uint32_t>(d.pt_loc[gi]);
+    d.cell_items[(size_t)f * L.N + pos] = i;
+    d.cell_zkey[(size_t)f * L.N + pos] = ord_f32(d.z[gi]);
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -211,7 +210,7 @@ __device__ __forceinline__ float robust_min(const uint32_t* keys, int n, int tid
 // Sorted cell of up to 64*S points held in registers: the point indices
 // ascending for the RANSAC cells, and z for the robust minimum.
 template <int S>
-__device__ __forceinline__ void sort_cell_regs(const DevBuf& d, const FrameDesc& fd, int32_t* items, int n, int radial,
+__device__ __forceinline__ void sort_cell_regs(int32_t* items, const uint32_t* zkeys, int n, int radial,
                                                float* zmin_cell) {
     const int lane = lane_id();
     uint32_t key[S];
@@ -232,7 +231,7 @@ __device__ __forceinline__ void sort_cell_regs(const DevBuf& d, const FrameDesc&
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int e = s * kWave + lane;
-            key[s] = e < n ? ord_f32(d.z[fd.pt_off + items[e]]) : 0xffffffffu;
+            key[s] = e < n ? zkeys[e] : 0xffffffffu;
         }
         reg_sort_u32<S>(key);
         // Robust minimum (segmenter.cpp:249-260): the largest i in [1, n/2]
@@ -277,7 +276,7 @@ __device__ __forceinline__ uint32_t row_sort16_u32(uint32_t key) {
 
 // sort_cell_regs for a cell of n <= 16 points held by one 16-lane row
 // (element e = lane & 15); every lane of the row calls it.
-__device__ __forceinline__ void sort_cell_row(const DevBuf& d, const FrameDesc& fd, int32_t* items, int n, int radial,
+__device__ __forceinline__ void sort_cell_row(int32_t* items, const uint32_t* zkeys, int n, int radial,
                                               float* zmin_cell) {
     const int e = lane_id() & 15;
     if (radial < kRansacRadialBins) {
@@ -286,7 +285,7 @@ __device__ __forceinline__ void sort_cell_row(const DevBuf& d, const FrameDesc& 
         if (e < n) items[e] = static_cast<int32_t>(key);
     }
     if (radial >= 1) {
-        uint32_t key = e < n ? ord_f32(d.z[fd.pt_off + items[e]]) : 0xffffffffu;
+        uint32_t key = e < n ? zkeys[e] : 0xffffffffu;
         key = row_sort16_u32(key);
         // Robust minimum (segmenter.cpp:249-260): the largest i in [1, n/2]
         // with s[i] - s[i-1] > 0.5f, else 0.
@@ -331,8 +330,8 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
         }
         if (cell_g >= 0 && n_g <= 16u) {
             const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell_g];
-            sort_cell_row(d, fd, d.cell_items + (size_t)f * L.N + start, static_cast<int>(n_g), cell_g % s.rings,
-                          zmin + cell_g);
+            sort_cell_row(d.cell_items + (size_t)f * L.N + start, d.cell_zkey + (size_t)f * L.N + start,
+                          static_cast<int>(n_g), cell_g % s.rings, zmin + cell_g);
         }
         // Rows whose cell is larger: the whole wavefront, one cell at a time.
         const uint64_t large = __ballot(cell_g >= 0 && n_g > 16u && (lane & 15) == 0);
@@ -349,20 +348,21 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
             }
             const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
             int32_t* items = d.cell_items + (size_t)f * L.N + start;
+            const uint32_t* zk = d.cell_zkey + (size_t)f * L.N + start;
             const int radial = cell % s.rings;
             if (n <= static_cast<uint32_t>(kWave)) {
-                sort_cell_regs<1>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+                sort_cell_regs<1>(items, zk, static_cast<int>(n), radial, zmin + cell);
                 continue;
             }
             if (n <= static_cast<uint32_t>(2 * kWave)) {
-                sort_cell_regs<2>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+                sort_cell_regs<2>(items, zk, static_cast<int>(n), radial, zmin + cell);
                 continue;
             }
             if (n <= static_cast<uint32_t>(4 * kWave)) {
-                sort_cell_regs<4>(d, fd, items, static_cast<int>(n), radial, zmin + cell);
+                sort_cell_regs<4>(items, zk, static_cast<int>(n), radial, zmin + cell);
                 continue;
             }
-            sort_cell_regs<8>(d, fd, items, static_cast<int>(n), radial, zmin + cell);  // n <= kSmallCell
+            sort_cell_regs<8>(items, zk, static_cast<int>(n), radial, zmin + cell);  // n <= kSmallCell
         }
     }
 }
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(1024) void k_cell_sort_big(DevBuf d, Layout L) {
         }
         if (radial >= 1) {
             for (int t = tid; t < m; t += 1024)
-                a[t] = t < static_cast<int>(n) ? ord_f32(d.z[fd.pt_off + items[t]]) : 0xffffffffu;
+                a[t] = t < static_cast<int>(n) ? d.cell_zkey[(size_t)f * L.N + start + t] : 0xffffffffu;
             __syncthreads();
             bitonic_u32<1024>(a, m, tid, sync);
             const float r = robust_min<1024>(a, static_cast<int>(n), tid, sync, [&](int v) {
