@@ -213,6 +213,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.vnb, B * N * 13);
     go(x.vmin, B * N);
     go(x.vcnt, B * N);
+    go(x.vlab, B * N);
     go(x.cl_root, B * N);
     go(x.cl_size, B * N);
     go(x.cl_off, B * N);

@@ -439,40 +439,93 @@ __device__ __forceinline__ int32_t root_of_point(const DevBuf& d, size_t ob, siz
     return d.vpar[ob + d.hdense[ho + d.vslot[ob + k]]];
 }
 
-__global__ void k_component_flag(DevBuf d, Layout L) {
-    const int f = blockIdx.y;
-    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nobs) return;
-    const CluConst cc = *d.cc;
-    const size_t ob = (size_t)f * L.N;
-    const size_t ho = (size_t)f * L.hash_cap;
-    const int32_t r = root_of_point(d, ob, ho, k);
-    d.cl_flag[ob + k] = (d.vmin[ob + r] == k && d.vcnt[ob + r] >= cc.min_size) ? 1u : 0u;
+// Clusters in the order of their first point (the reference numbers them in
+// input order): a component is a cluster when it has at least min_size
+// points, and its first point is its minimum obstacle index.  Two passes over
+// tiles of 2048 obstacles (eight rows of 256): the first points of every
+// tile, then each tile's compaction after the counts of the tiles before it,
+// which numbers the clusters, writes their root, size and an empty cursor,
+// and gives each root its cluster label (vlab).  The last tile writes the
+// cluster count.
+constexpr int kCompThreads = 256;
+constexpr int kCompRows = 8;
+constexpr int kCompTile = kCompThreads * kCompRows;
+
+__device__ __forceinline__ uint32_t first_point_bits(const DevBuf& d, size_t ob, size_t ho, int base, int nobs,
+                                                     uint32_t min_size, int32_t (&root)[kCompRows]) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kCompRows; ++k) {
+        const int q = base + k * kCompThreads + static_cast<int>(threadIdx.x);
+        root[k] = -1;
+        if (q < nobs) {
+            const int32_t r = root_of_point(d, ob, ho, q);
+            root[k] = r;
+            if (d.vmin[ob + r] == q && d.vcnt[ob + r] >= min_size) m |= 1u << k;
+        }
+    }
+    return m;
 }
 
-// cl_flag holds the exclusive scan of the first-point flags on entry.
+__global__ __launch_bounds__(kCompThreads) void k_component_count(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kCompThreads / kWave + 1];
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int base = blockIdx.x * kCompTile;
+    if (base >= nobs) return;
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    int32_t root[kCompRows];
+    const uint32_t m = first_point_bits(d, ob, ho, base, nobs, d.cc->min_size, root);
+    uint32_t total;
+    block_exclusive_sum<kCompThreads>(static_cast<uint32_t>(__popc(m)), lds, &total);
+    if (threadIdx.x == 0) d.blk[(size_t)f * L.blk_stride + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kCompThreads) void k_component_number(DevBuf d, Layout L) {
+    __shared__ uint32_t lds[kCompRows * kCompThreads / kWave + 1];
+    const int f = blockIdx.y;
+    const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
+    const int tile = blockIdx.x;
+    const int base = tile * kCompTile;
+    if (base >= nobs) {
+        if (tile == 0 && threadIdx.x == 0) d.fs[f * kFsCount + kFsClusters] = 0;
+        return;
+    }
+    const uint32_t* blk = d.blk + (size_t)f * L.blk_stride;
+    uint32_t pre = 0;
+    for (int i = threadIdx.x; i < tile; i += kCompThreads) pre += blk[i];
+    uint32_t pre_total;
+    block_exclusive_sum<kCompThreads>(pre, lds, &pre_total);
+    const size_t ob = (size_t)f * L.N;
+    const size_t ho = (size_t)f * L.hash_cap;
+    int32_t root[kCompRows];
+    const uint32_t m = first_point_bits(d, ob, ho, base, nobs, d.cc->min_size, root);
+    uint32_t pos[kCompRows];
+    const uint32_t total = tile_compact<kCompThreads, kCompRows>(m, pos, lds);
+#pragma unroll
+    for (int k = 0; k < kCompRows; ++k) {
+        if ((m >> k) & 1u) {
+            const uint32_t lab = pre_total + pos[k];
+            const int32_t r = root[k];
+            d.cl_root[ob + lab] = r;
+            d.cl_size[ob + lab] = d.vcnt[ob + r];
+            d.cl_cursor[ob + lab] = 0;
+            d.vlab[ob + r] = static_cast<int32_t>(lab);
+        }
+    }
+    if (threadIdx.x == 0 && base + kCompTile >= nobs) d.fs[f * kFsCount + kFsClusters] = pre_total + total;
+}
+
 __global__ void k_component_label(DevBuf d, Layout L) {
     const int f = blockIdx.y;
     const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nobs) return;
-    const CluConst cc = *d.cc;
     const size_t ob = (size_t)f * L.N;
     const size_t ho = (size_t)f * L.hash_cap;
     const int32_t r = root_of_point(d, ob, ho, k);
-    const uint32_t cnt = d.vcnt[ob + r];
-    int32_t lab = -1;
-    if (cnt >= cc.min_size) {
-        const int32_t first = d.vmin[ob + r];
-        lab = static_cast<int32_t>(d.cl_flag[ob + first]);
-        if (first == k) {
-            d.cl_root[ob + lab] = r;
-            d.cl_size[ob + lab] = cnt;
-            d.cl_cursor[ob + lab] = 0;
-        }
-    }
-    d.clabel[ob + k] = lab;
+    d.clabel[ob + k] = d.vcnt[ob + r] >= d.cc->min_size ? d.vlab[ob + r] : -1;
 }
 
 // Members of each cluster (unordered within the cluster) and the round-0
@@ -524,8 +577,9 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     hipLaunchKernelGGL(k_voxel_cc, dim3(B), dim3(1024), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_cc_global, dim3(nb, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_voxel_stats, dim3(nb, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_component_flag, dim3(nb, B), dim3(256), 0, st, d, L);
-    launch_scan(c, B, d.cl_flag, d.cl_flag, L.N, kCountSlot, kFsObstacles, kFsClusters);
+    const int ct = (L.N + kCompTile - 1) / kCompTile;  // at most blk_stride tiles
+    hipLaunchKernelGGL(k_component_count, dim3(ct, B), dim3(kCompThreads), 0, st, d, L);
+    hipLaunchKernelGGL(k_component_number, dim3(ct, B), dim3(kCompThreads), 0, st, d, L);
     hipLaunchKernelGGL(k_component_label, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan_small(c, B, d.cl_size, d.cl_off, L.N, kCountSlot, kFsClusters, -1);
     hipLaunchKernelGGL(k_cluster_scatter, dim3(nb, B), dim3(256), 0, st, d, L);

@@ -158,6 +158,7 @@ struct DevBuf {
     int32_t* vnb;            // [13][N] per frame: dense id of each forward neighbour voxel, -1 none
     int32_t* vmin;           // per root: minimum obstacle index of the component
     uint32_t* vcnt;          // per root: points in the component
+    int32_t* vlab;           // per root: cluster label (components of at least min_size points)
     // clusters (frame stride N)
     int32_t* cl_root;
     uint32_t* cl_size;

@@ -497,11 +497,11 @@ __global__ __launch_bounds__(256) void k_cand_count(DevBuf d, Layout L) {
     const size_t co = (size_t)f * L.cell_stride;
     const uint32_t n = d.cell_count[co + cell];
     const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
-    const int32_t* items = d.cell_items + (size_t)f * L.N + start;
+    const uint32_t* zk = d.cell_zkey + (size_t)f * L.N + start;  // the cell's z in any order
     const float e = d.elev[co + cell];
     uint32_t c = 0;
     for (uint32_t t = lane; t < n; t += kWave) {
-        const float z = d.z[fd.pt_off + items[t]];
+        const float z = unord_f32(zk[t]);
         c += (fabsf(e - z) < s.two_thr) ? 1u : 0u;
     }
     c = wave_sum(c);
