@@ -341,9 +341,12 @@ __device__ __forceinline__ float key_float(uint32_t k) {
 }
 
 // Bitonic sort of 64*S (key, value) pairs held in registers, element
-// e = s * 64 + lane in slot s; pairs are distinct (values are unique), so the
-// network needs no tie rule.  Partners within a lane swap registers, others
-// are exchanged with lane shuffles: no LDS round trips.
+// e = s * 64 + lane in slot s, by key alone.  Equal keys are the same (x, y)
+// (or sort-last padding), and any value of such a group may survive an
+// exchange: the index a vertex reports among exact duplicates is settled
+// later by coordinates, so the network needs no tie rule.  Partners within a
+// lane swap registers, others are exchanged with lane shuffles: no LDS round
+// trips.
 template <int S>
 __device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[S]) {
     const int lane = lane_id();
@@ -358,7 +361,7 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[
                     if (s & js) continue;
                     const int t = s | js;
                     const bool asc = ((s * kWave) & k) == 0;
-                    const bool gt = key[s] > key[t] || (key[s] == key[t] && val[s] > val[t]);
+                    const bool gt = key[s] > key[t];
                     if (gt == asc) {
                         const uint64_t tk = key[s];
                         key[s] = key[t];
@@ -386,7 +389,7 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&key)[S], uint32_t (&val)[
                     }
                     const bool asc = ((s * kWave + lane) & k) == 0;
                     const bool lower = (lane & j) == 0;
-                    const bool other_less = ok < key[s] || (ok == key[s] && ov < val[s]);
+                    const bool other_less = ok < key[s];
                     if (other_less == (lower == asc)) {
                         key[s] = ok;
                         val[s] = ov;
@@ -615,7 +618,10 @@ __global__ void k_hull_chunks(DevBuf d, Layout L, int r) {
     d.hc_count[((r + 1) & 1) * BN + ob + c] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_hull_round(DevBuf d, Layout L, int r) {
+// Register budget for three wavefronts per SIMD (168 VGPRs, a few spills in
+// the cold paths): the chunks' sort and prune phases are VALU- and
+// LDS-latency-bound, and a third wavefront hides more of it than two.
+__global__ __launch_bounds__(256, 3) void k_hull_round(DevBuf d, Layout L, int r) {
     __shared__ double sx[kWavesPerBlock][kHullChunk];
     __shared__ double sy[kWavesPerBlock][kHullChunk];
     __shared__ int32_t si[kWavesPerBlock][kHullChunk];
