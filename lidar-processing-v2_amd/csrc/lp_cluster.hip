@@ -511,6 +511,7 @@ __global__ __launch_bounds__(kCompThreads) void k_component_number(DevBuf d, Lay
             d.cl_root[ob + lab] = r;
             d.cl_size[ob + lab] = d.vcnt[ob + r];
             d.cl_cursor[ob + lab] = 0;
+            d.hc_count[ob + lab] = 0;  // the round-0 hull input count (first buffer)
             d.vlab[ob + r] = static_cast<int32_t>(lab);
         }
     }
@@ -528,9 +529,36 @@ __global__ void k_component_label(DevBuf d, Layout L) {
     d.clabel[ob + k] = d.vcnt[ob + r] >= d.cc->min_size ? d.vlab[ob + r] : -1;
 }
 
+// True when the obstacle at input index i (pixel pix of a frame W pixels
+// wide) has, in the two range-image rows above it in its column, obstacles
+// of its own cluster with exactly its x and y.  Vertical surfaces repeat one
+// (x, y) down a column; such a point is never a hull vertex while a copy of
+// it stays in the hull input, and down every column run of copies the top
+// two never pass this test, so each run keeps two copies (a cluster of one
+// repeated point keeps the two copies the reference returns for it).
+__device__ __forceinline__ bool column_repeat(const DevBuf& d, const Layout& L, const FrameDesc& fd, size_t ob,
+                                              int i, int32_t lab) {
+    const int pix = d.pt_pix[fd.pt_off + i];
+    const int W = fd.sc.W;
+    if (pix < 2 * W) return false;
+    const float x = d.x[fd.pt_off + i], y = d.y[fd.pt_off + i];
+    const unsigned long long imask = (1ull << L.idx_bits) - 1;
+#pragma unroll
+    for (int dr = 1; dr <= 2; ++dr) {
+        const unsigned long long key = d.pix_key[fd.pix_off + pix - dr * W];
+        if (key == ~0ull) return false;
+        const int j = static_cast<int>(key & imask);
+        const int gj = fd.pt_off + j;
+        if (d.labels[gj] != LP_LABEL_OBSTACLE || d.x[gj] != x || d.y[gj] != y) return false;
+        if (d.clabel[ob + d.cl_flag[ob + j]] != lab) return false;
+    }
+    return true;
+}
+
 // Members of each cluster (unordered within the cluster) and the round-0
-// hull input: the cluster's points as PointXY doubles (processor.cpp:647).
-__global__ void k_cluster_scatter(DevBuf d, Layout L) {
+// hull input: the cluster's points as PointXY doubles (processor.cpp:647),
+// less the column repeats (their own count, in the first hull buffer).
+__global__ void k_cluster_scatter(DevBuf d, Layout L, int from_split) {
     const int f = blockIdx.y;
     const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -538,12 +566,14 @@ __global__ void k_cluster_scatter(DevBuf d, Layout L) {
     const size_t ob = (size_t)f * L.N;
     const int32_t lab = d.clabel[ob + k];
     if (lab < 0) return;
+    const uint32_t off = d.cl_off[ob + lab];
     const uint32_t pos = wave_atomic_inc(&d.cl_cursor[ob + lab]);
-    const uint32_t dst = d.cl_off[ob + lab] + pos;
-    d.cl_members[ob + dst] = k;
-    d.hcx[ob + dst] = static_cast<double>(d.ox[ob + k]);
-    d.hcy[ob + dst] = static_cast<double>(d.oy[ob + k]);
-    d.hci[ob + dst] = k;
+    d.cl_members[ob + off + pos] = k;
+    if (from_split && column_repeat(d, L, d.frames[f], ob, d.obs_src[ob + k], lab)) return;
+    const uint32_t hp = off + wave_atomic_inc(&d.hc_count[ob + lab]);
+    d.hcx[ob + hp] = static_cast<double>(d.ox[ob + k]);
+    d.hcy[ob + hp] = static_cast<double>(d.oy[ob + k]);
+    d.hci[ob + hp] = k;
 }
 
 void launch_cluster(lp_context* c, int B, bool from_split) {
@@ -582,7 +612,7 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
     hipLaunchKernelGGL(k_component_number, dim3(ct, B), dim3(kCompThreads), 0, st, d, L);
     hipLaunchKernelGGL(k_component_label, dim3(nb, B), dim3(256), 0, st, d, L);
     launch_scan_small(c, B, d.cl_size, d.cl_off, L.N, kCountSlot, kFsClusters, -1);
-    hipLaunchKernelGGL(k_cluster_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_cluster_scatter, dim3(nb, B), dim3(256), 0, st, d, L, from_split ? 1 : 0);
     launch_hash_reset(c, B, d.vlist, kFsVoxels);
 }
 

@@ -611,7 +611,7 @@ __global__ void k_hull_chunks(DevBuf d, Layout L, int r) {
         d.hull_flag[ob + c] = 0;
     }
     if (r == 0 || d.hull_count[ob + c] == 0) {
-        const uint32_t cnt = r == 0 ? d.cl_size[ob + c] : d.hc_count[(r & 1) * BN + ob + c];
+        const uint32_t cnt = d.hc_count[(r & 1) * BN + ob + c];
         nch = cnt <= static_cast<uint32_t>(kHullChunk) ? 1u : (cnt + kHullChunk - 1) / kHullChunk;
     }
     d.hc_chunks[ob + c] = nch;
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(256, 3) void k_hull_round(DevBuf d, Layout L, int r
     for (uint32_t t = tile * kWavesPerBlock + w; t < total; t += gridDim.x * kWavesPerBlock) {
         const int c = upper_bound_u32(choff, K, t) - 1;
         const uint32_t ci = t - choff[c];
-        const uint32_t cnt = r == 0 ? d.cl_size[ob + c] : d.hc_count[in_b + ob + c];
+        const uint32_t cnt = d.hc_count[in_b + ob + c];
         const uint32_t base = d.cl_off[ob + c];
         const bool final_chunk = cnt <= static_cast<uint32_t>(kHullChunk);
         const uint32_t lo = ci * kHullChunk;
@@ -1355,6 +1355,7 @@ __global__ void k_hull_lists_init(DevBuf d, Layout L, const int32_t* offsets, in
     if (t < K) {
         d.cl_off[t] = static_cast<uint32_t>(offsets[t]);
         d.cl_size[t] = static_cast<uint32_t>(offsets[t + 1] - offsets[t]);
+        d.hc_count[t] = d.cl_size[t];  // round-0 input: the whole list
     }
     if (t < T) {
         int lo = 0, hi = K;
