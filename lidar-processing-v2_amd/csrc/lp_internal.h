@@ -99,7 +99,7 @@ struct DevBuf {
     float4* dsort;
     int32_t* spos;           // per input point (frame stride N): its position in dsort
     int32_t* nopen;          // = cl_members (free until clustering): points the input-order pass left open
-    int32_t* nloc;           // = clabel (free until clustering): position of each point inside its cell
+    int32_t* nloc;           // = clabel (free until clustering): each point's place in its cell, then the listed points' input indices
     uint32_t* nocc;          // per frame: kNoiseOccWords words, one bit per occupied 1 m (x, y) column
     int32_t* ncell_slot;     // per frame (stride N): hash slot of each occupied noise cell, by dense id
     uint32_t* ncell_start;   // per frame (stride N): first cell-sorted position of each cell, by dense id

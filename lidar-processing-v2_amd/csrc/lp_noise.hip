@@ -267,7 +267,8 @@ __device__ __forceinline__ void noise_own_cell(const DevBuf& d, const NoiseConst
     }
     if (state == 2) {
         const uint32_t q = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseHard]);
-        d.vslot[po + q] = i;
+        d.vslot[po + q] = pos;  // the listed point: position, input index and carry, loaded side by side
+        d.nloc[po + q] = i;
         d.obs_src[po + q] = static_cast<int32_t>(carry);
     } else {
         noise_label(d, f, po, i, state == 1);
@@ -330,8 +331,8 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
     const uint32_t* occ = d.nocc + (size_t)f * kNoiseOccWords;
     uint32_t noise_here = 0;
     for (int q = tile * (blockDim.x / kWave) + threadIdx.x / kWave; q < nhard; q += waves) {
-        const int i = d.vslot[po + q];
-        const int pos = d.spos[po + i];
+        const int pos = d.vslot[po + q];
+        const int i = d.nloc[po + q];
         const uint32_t carry = static_cast<uint32_t>(d.obs_src[po + q]);
         const float4 t = d.dsort[po + pos];
         const float r2 = noise_radius_sqr(nc, t.x, t.y);
