@@ -158,7 +158,6 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.nocc, B * kNoiseOccWords);
     go(x.ncell_slot, B * N);
     go(x.ncell_start, B * N);
-    go(x.hstart, B * HC);
     go(x.pt_cell, B * N);
     go(x.pt_pix, B * N);
     go(x.labels, B * N);
@@ -204,10 +203,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.slot_src, B * P * 24);
     go(x.qsum_ok, B * P);
     go(x.fin, B * P);
-    go(x.hkey, B * HC);
-    go(x.hmin, B * HC);
-    go(x.hdense, B * HC);
-    go(x.hcount, B * HC);
+    go(x.hslot, B * HC);
     go(x.vlist, B * N);
     go(x.vpar, B * N);
     go(x.vnb, B * N * 13);

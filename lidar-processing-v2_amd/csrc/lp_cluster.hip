@@ -226,7 +226,7 @@ __device__ __forceinline__ void voxel_insert_tile(const DevBuf& d, const Layout&
     if (lane_id() == run.leader) {
         h = hash_slot(key, mask);
         for (uint32_t probe = 0; probe <= mask; ++probe) {
-            const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+            const int32_t prev = atomicCAS(&d.hslot[ho + h].key, -1, key);
             if (prev == -1) {
                 const uint32_t pos = atomicAdd(&d.fs[f * kFsCount + kFsVoxels], 1u);
                 if (pos < static_cast<uint32_t>(L.N)) {
@@ -235,14 +235,14 @@ __device__ __forceinline__ void voxel_insert_tile(const DevBuf& d, const Layout&
                     d.vmin[ob + pos] = 0x7fffffff;
                     d.vcnt[ob + pos] = 0;
                 }
-                d.hdense[ho + h] = static_cast<int32_t>(pos);
+                d.hslot[ho + h].dense = static_cast<int32_t>(pos);
                 break;
             }
             if (prev == key) break;
             h = (h + 1) & mask;
         }
-        atomicMin(&d.hmin[ho + h], k);
-        atomicAdd(&d.hcount[ho + h], run.count);
+        atomicMin(&d.hslot[ho + h].aux, k);
+        atomicAdd(&d.hslot[ho + h].count, run.count);
     }
     const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
     d.vslot[ob + k] = slot;
@@ -259,10 +259,10 @@ __global__ __launch_bounds__(256) void k_voxel_insert_q(DevBuf d, Layout L, int 
            [&](int f, int t) { voxel_insert_tile(d, L, f, t); });
 }
 
-__device__ __forceinline__ int32_t hash_find(const int32_t* hkey, uint32_t mask, int32_t key) {
+__device__ __forceinline__ int32_t hash_find(const HashSlot* hs, uint32_t mask, int32_t key) {
     uint32_t h = hash_slot(key, mask);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const int32_t k = hkey[h];
+        const int32_t k = hs[h].key;
         if (k == key) return static_cast<int32_t>(h);
         if (k == -1) return -1;
         h = (h + 1) & mask;
@@ -293,8 +293,8 @@ __device__ __forceinline__ int32_t voxel_neighbour(const DevBuf& d, size_t ho, u
         aa += dm.na;
     else if (aa >= dm.na)
         aa -= dm.na;
-    const int32_t g = hash_find(d.hkey + ho, mask, dm.nr * (dm.na * ee + aa) + rr);
-    return g < 0 ? -1 : d.hdense[ho + g];
+    const int32_t g = hash_find(d.hslot + ho, mask, dm.nr * (dm.na * ee + aa) + rr);
+    return g < 0 ? -1 : d.hslot[ho + g].dense;
 }
 
 // ---- union-find in LDS ----------------------------------------------------
@@ -339,7 +339,7 @@ __global__ void k_voxel_neighbours(DevBuf d, Layout L) {
     const size_t ob = (size_t)f * L.N;
     const size_t ho = (size_t)f * L.hash_cap;
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
-    const int32_t key = d.hkey[ho + d.vlist[ob + v]];
+    const int32_t key = d.hslot[ho + d.vlist[ob + v]].key;
     int32_t* nb = d.vnb + ob * 13;
 #pragma unroll
     for (int q = 0; q < 13; ++q) nb[(size_t)q * L.N + v] = voxel_neighbour(d, ho, mask, dm, key, q);
@@ -431,12 +431,13 @@ __global__ void k_voxel_stats(DevBuf d, Layout L) {
         d.vpar[ob + v] = r;
     }
     const int32_t h = d.vlist[ob + v];
-    atomicMin(&d.vmin[ob + r], d.hmin[ho + h]);
-    atomicAdd(&d.vcnt[ob + r], d.hcount[ho + h]);
+    const HashSlot hv = d.hslot[ho + h];
+    atomicMin(&d.vmin[ob + r], hv.aux);
+    atomicAdd(&d.vcnt[ob + r], hv.count);
 }
 
 __device__ __forceinline__ int32_t root_of_point(const DevBuf& d, size_t ob, size_t ho, int k) {
-    return d.vpar[ob + d.hdense[ho + d.vslot[ob + k]]];
+    return d.vpar[ob + d.hslot[ho + d.vslot[ob + k]].dense];
 }
 
 // Clusters in the order of their first point (the reference numbers them in

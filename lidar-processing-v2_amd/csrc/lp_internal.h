@@ -23,7 +23,7 @@ constexpr int kBigCell = 8192;        // larger cells: one workgroup; beyond: ca
 constexpr int kHullChunk = 256;       // points per chunk-hull wave
 constexpr int kHullFinal = 8192;      // candidates a final hull workgroup can hold
 constexpr int kNoiseOccWords = 1024 * 1024 / 32;  // noise filter: 1024 x 1024 column bits per frame
-constexpr int32_t kHashMinClean = 0x7fffffff;     // hmin of a free hash slot (the voxel minimum's identity)
+constexpr int32_t kHashMinClean = 0x7fffffff;     // aux of a free hash slot (the voxel minimum's identity)
 
 // Per-frame error bits (device), mapped to status codes on fetch.
 enum : uint32_t {
@@ -86,6 +86,19 @@ struct FrameDesc {
     SegConst sc;
 };
 
+// One slot of the shared open-addressing hash (frame stride hash_cap), all
+// fields in one 16-byte record so a touched slot costs one cache line:
+// key (-1 free), count (points in the cell / voxel), dense (the occupied
+// slot's dense id) and aux (voxels: minimum obstacle index, kHashMinClean
+// when free; noise cells: first cell-sorted position, reset to
+// kHashMinClean afterwards).
+struct alignas(16) HashSlot {
+    int32_t key;
+    uint32_t count;
+    int32_t dense;
+    int32_t aux;
+};
+
 // Device pointers of one context (all sized for the capacity).
 struct DevBuf {
     // packed inputs (CSR by FrameDesc::pt_off)
@@ -103,7 +116,6 @@ struct DevBuf {
     uint32_t* nocc;          // per frame: kNoiseOccWords words, one bit per occupied 1 m (x, y) column
     int32_t* ncell_slot;     // per frame (stride N): hash slot of each occupied noise cell, by dense id
     uint32_t* ncell_start;   // per frame (stride N): first cell-sorted position of each cell, by dense id
-    int32_t* hstart;         // per frame (stride hash_cap): first cell-sorted position of the slot's cell
     // per point
     int32_t* pt_cell;
     int32_t* pt_pix;
@@ -148,10 +160,7 @@ struct DevBuf {
     uint8_t* qsum_ok;
     uint8_t* fin;
     // voxel hash (frame stride hash_cap)
-    int32_t* hkey;
-    int32_t* hmin;           // minimum obstacle index of the voxel's points
-    int32_t* hdense;         // slot -> dense voxel id
-    uint32_t* hcount;        // points in the voxel
+    HashSlot* hslot;
     // dense voxels (frame stride N)
     int32_t* vlist;          // dense id -> hash slot (insertion order)
     int32_t* vpar;           // union-find parent / component root (dense ids)

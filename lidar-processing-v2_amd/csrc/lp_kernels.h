@@ -27,8 +27,9 @@ void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t
 // The same in one launch, one workgroup per frame: for cluster-sized counts.
 void launch_scan_small(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
                        int count_slot, int total_slot);
-// The same with element i of frame f read as in[f * in_stride + gather[f * stride + i]].
-void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, const int32_t* gather,
+// The same with element i of frame f read as
+// in[(f * in_stride + gather[f * stride + i]) * elem] (a field of a record array).
+void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, int elem, const int32_t* gather,
                         uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot);
 
 void launch_mt19937(uint32_t* out, int count, hipStream_t st);

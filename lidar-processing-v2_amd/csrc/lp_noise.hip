@@ -60,10 +60,10 @@ __device__ __forceinline__ uint32_t noise_slot(int32_t key, uint32_t mask) {
     return h & mask;
 }
 
-__device__ __forceinline__ int32_t noise_find(const int32_t* hkey, uint32_t mask, int32_t key) {
+__device__ __forceinline__ int32_t noise_find(const HashSlot* hs, uint32_t mask, int32_t key) {
     uint32_t h = noise_slot(key, mask);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const int32_t k = hkey[h];
+        const int32_t k = hs[h].key;
         if (k == key) return static_cast<int32_t>(h);
         if (k == -1) return -1;
         h = (h + 1) & mask;
@@ -83,7 +83,7 @@ __device__ __forceinline__ float noise_dist(const float4& t, const float4& p) {
     return d0 * d0 + (d1 * d1 + (d2 * d2 + 0.0f));
 }
 
-// The hash arrays (hkey, hmin, hcount) are clean between uses: every user
+// The hash slots are clean between uses: every user
 // resets the slots it occupied when it is done (k_hash_reset), so only the
 // column bits and the counters are cleared here.
 __global__ void k_noise_clear(DevBuf d, Layout L) {
@@ -159,12 +159,12 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     if (lane_id() == run.leader) {
         h = noise_slot(key, mask);
         for (uint32_t probe = 0; probe <= mask; ++probe) {
-            const int32_t prev = atomicCAS(&d.hkey[ho + h], -1, key);
+            const int32_t prev = atomicCAS(&d.hslot[ho + h].key, -1, key);
             fresh = prev == -1;
             if (prev == -1 || prev == key) break;
             h = (h + 1) & mask;
         }
-        before = atomicAdd(&d.hcount[ho + h], run.count);
+        before = atomicAdd(&d.hslot[ho + h].count, run.count);
         atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
     }
     // Every occupied cell gets a dense id (in any order): the cell counts are
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     if (fresh) {
         const uint32_t id = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseCells]);
         d.ncell_slot[(size_t)f * L.N + id] = static_cast<int32_t>(h);
-        d.hdense[ho + h] = static_cast<int32_t>(id);
+        d.hslot[ho + h].dense = static_cast<int32_t>(id);
     }
     d.vslot[po + i] = static_cast<int32_t>(__shfl(h, run.leader, kWave));
     d.nloc[po + i] = static_cast<int32_t>(__shfl(before, run.leader, kWave) + run.rank);
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
 
 // ncell_start holds the exclusive scan of the cell counts by dense id on
 // entry.  Each point goes to its cell's start plus its place in the cell;
-// the point at place 0 writes the cell's start to hstart.
+// the point at place 0 writes the cell's start to the slot's aux field.
 __global__ void k_noise_scatter(DevBuf d, Layout L) {
     int tile, f;
     xcd_tile_frame(tile, f);
@@ -192,8 +192,8 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const size_t ho = (size_t)f * L.hash_cap;
     const int32_t h = d.vslot[po + i];
     const int loc = d.nloc[po + i];
-    const int start = static_cast<int>(d.ncell_start[po + d.hdense[ho + h]]);
-    if (loc == 0) d.hstart[ho + h] = start;
+    const int start = static_cast<int>(d.ncell_start[po + d.hslot[ho + h].dense]);
+    if (loc == 0) d.hslot[ho + h].aux = start;
     const int pos = start + loc;
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
     d.spos[po + i] = pos;
@@ -233,8 +233,9 @@ __device__ __forceinline__ void noise_own_cell(const DevBuf& d, const NoiseConst
         state = 1;  // a NaN radius admits no point
     } else {
         const int h = __float_as_int(t.w);
-        const int s0 = d.hstart[ho + h];
-        const int s1 = s0 + static_cast<int>(d.hcount[ho + h]);
+        const HashSlot hv = d.hslot[ho + h];
+        const int s0 = hv.aux;
+        const int s1 = s0 + static_cast<int>(hv.count);
         uint32_t cnt = 0;
         int budget = kOwnCellBudget;
         int up = pos, dn = pos - 1;
@@ -341,8 +342,9 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
         if (r < static_cast<float>(kNoiseMaxExtent - 1)) {
             cnt = carry & 0xffu;
             const int h0 = __float_as_int(t.w);
-            const int s0 = d.hstart[ho + h0];
-            const int s1 = s0 + static_cast<int>(d.hcount[ho + h0]);
+            const HashSlot hv = d.hslot[ho + h0];
+            const int s0 = hv.aux;
+            const int s1 = s0 + static_cast<int>(hv.count);
             const int up = pos + static_cast<int>((carry >> 8) & 0xffu);
             const int dn = pos - 1 - static_cast<int>((carry >> 16) & 0xffu);
             cnt = wave_count_range(d, po, t, r2, up, s1, cnt, k);
@@ -367,12 +369,14 @@ __global__ __launch_bounds__(256) void k_noise_search(DevBuf d, Layout L) {
                     if (ci < cube && max(abs(dx), max(abs(dy), abs(dz))) == sh &&
                         cell_gap_sqr(fx, fy, fz, dx, dy, dz) <= reach && ax >= 0 && ax <= 1023 && ay >= 0 &&
                         ay <= 1023 && az >= 0 && az <= 2047 && ((occ[ay * 32 + (ax >> 5)] >> (ax & 31)) & 1u))
-                        h = noise_find(d.hkey + ho, mask, noise_key(ax, ay, az));
+                        h = noise_find(d.hslot + ho, mask, noise_key(ax, ay, az));
                     // The points of all cells found in this batch, 64 at a
                     // time: point g belongs to the lane whose prefix range
                     // holds it (binary search over the lanes' inclusive sums).
-                    const int c_n = h >= 0 ? static_cast<int>(d.hcount[ho + h]) : 0;
-                    const int c_a = h >= 0 ? d.hstart[ho + h] : 0;
+                    HashSlot hv{};
+                    if (h >= 0) hv = d.hslot[ho + h];
+                    const int c_n = h >= 0 ? static_cast<int>(hv.count) : 0;
+                    const int c_a = h >= 0 ? hv.aux : 0;
                     const int incl = wave_inclusive_sum(c_n);
                     const int found = __shfl(incl, kWave - 1, kWave);
                     const int start = c_a - (incl - c_n);  // point g of this lane's cell: start + g
@@ -490,7 +494,8 @@ void launch_noise(lp_context* c, int B, bool compact) {
     stage_mark(c, "noise");
     hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_noise_prep, dim3(nb, B), dim3(256), 0, st, d, L);
-    launch_scan_gather(c, B, d.hcount, L.hash_cap, d.ncell_slot, d.ncell_start, L.N, kCountSlot, kFsNoiseCells, -1);
+    launch_scan_gather(c, B, &d.hslot[0].count, L.hash_cap, 4, d.ncell_slot, d.ncell_start, L.N, kCountSlot,
+                       kFsNoiseCells, -1);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_count");
     // The open list is a few percent of the points: a quarter of the point

@@ -21,16 +21,18 @@ __device__ __forceinline__ int scan_count(const DevBuf& d, const Layout& L, int 
 }
 
 // Element i of frame f: in[f * in_stride + i], or with a gather index
-// in[f * in_stride + gather[f * stride + i]].
+// in[(f * in_stride + gather[f * stride + i]) * elem], elem words apart
+// (a field of a record array).
 struct ScanIn {
     const uint32_t* in;
     size_t in_stride;
     const int32_t* gather;
+    int elem;
 };
 
 __device__ __forceinline__ uint32_t scan_value(const ScanIn& si, size_t stride, int f, int i) {
-    const uint32_t* a = si.in + (size_t)f * si.in_stride;
-    return si.gather ? a[si.gather[(size_t)f * stride + i]] : a[i];
+    if (!si.gather) return si.in[(size_t)f * si.in_stride + i];
+    return si.in[((size_t)f * si.in_stride + si.gather[(size_t)f * stride + i]) * si.elem];
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L, ScanIn si, size_t stride,
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_scan_small(DevBuf d, Layout L
 void launch_scan_small(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
                        int count_slot, int total_slot) {
     hipLaunchKernelGGL(k_scan_small, dim3(B), dim3(kSmallThreads), 0, c->stream, c->d, c->L,
-                       ScanIn{in, stride, nullptr}, out, stride, kind, count_slot, total_slot);
+                       ScanIn{in, stride, nullptr, 1}, out, stride, kind, count_slot, total_slot);
 }
 
 static void launch_scan_in(lp_context* c, int B, ScanIn si, uint32_t* out, size_t stride, ScanCount kind,
@@ -144,12 +146,12 @@ static void launch_scan_in(lp_context* c, int B, ScanIn si, uint32_t* out, size_
 
 void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
                  int count_slot, int total_slot) {
-    launch_scan_in(c, B, ScanIn{in, stride, nullptr}, out, stride, kind, count_slot, total_slot);
+    launch_scan_in(c, B, ScanIn{in, stride, nullptr, 1}, out, stride, kind, count_slot, total_slot);
 }
 
-void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, const int32_t* gather,
+void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, int elem, const int32_t* gather,
                         uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot) {
-    launch_scan_in(c, B, ScanIn{in, in_stride, gather}, out, stride, kind, count_slot, total_slot);
+    launch_scan_in(c, B, ScanIn{in, in_stride, gather, elem}, out, stride, kind, count_slot, total_slot);
 }
 
 }  // namespace lp

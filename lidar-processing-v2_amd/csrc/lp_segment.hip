@@ -845,8 +845,8 @@ void launch_clear(lp_context* c, int B) {
     hipLaunchKernelGGL(k_clear, dim3(256, B), dim3(256), 0, c->stream, c->d, c->L);
 }
 
-// Hash slots back to their clean state: hkey -1 (free), hmin kHashMinClean,
-// hcount 0.  k_hash_reset takes the slots a stage occupied (slots[0 ..
+// Hash slots back to their clean state: key -1 (free), count 0, aux
+// kHashMinClean.  k_hash_reset takes the slots a stage occupied (slots[0 ..
 // fs[count_slot]) per frame); k_hash_init clears every slot (context
 // creation, or a frame whose list is incomplete).
 __global__ void k_hash_reset(DevBuf d, Layout L, const int32_t* slots, int count_slot) {
@@ -857,26 +857,20 @@ __global__ void k_hash_reset(DevBuf d, Layout L, const int32_t* slots, int count
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (cnt > static_cast<uint32_t>(L.N)) {
         for (int h = t0; h < L.hash_cap; h += stride) {
-            d.hkey[ho + h] = -1;
-            d.hmin[ho + h] = kHashMinClean;
-            d.hcount[ho + h] = 0;
+            d.hslot[ho + h] = HashSlot{-1, 0u, 0, kHashMinClean};
         }
         return;
     }
     for (int v = t0; v < static_cast<int>(cnt); v += stride) {
         const int32_t h = slots[(size_t)f * L.N + v];
-        d.hkey[ho + h] = -1;
-        d.hmin[ho + h] = kHashMinClean;
-        d.hcount[ho + h] = 0;
+        d.hslot[ho + h] = HashSlot{-1, 0u, 0, kHashMinClean};
     }
 }
 
 __global__ void k_hash_init(DevBuf d, Layout L) {
     const size_t total = (size_t)L.B * L.hash_cap;
     for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total; o += (size_t)gridDim.x * blockDim.x) {
-        d.hkey[o] = -1;
-        d.hmin[o] = kHashMinClean;
-        d.hcount[o] = 0;
+        d.hslot[o] = HashSlot{-1, 0u, 0, kHashMinClean};
     }
 }
 
