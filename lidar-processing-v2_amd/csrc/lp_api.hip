@@ -159,6 +159,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.ncell_slot, B * N);
     go(x.ncell_start, B * N);
     go(x.pt_cell, B * N);
+    go(x.pt_xyzc, B * N);
     go(x.pt_pix, B * N);
     go(x.labels, B * N);
     go(x.obs_flag, B * N);

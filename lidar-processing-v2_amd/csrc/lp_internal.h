@@ -118,6 +118,7 @@ struct DevBuf {
     uint32_t* ncell_start;   // per frame (stride N): first cell-sorted position of each cell, by dense id
     // per point
     int32_t* pt_cell;
+    float4* pt_xyzc;         // per point (CSR): x, y, z and the polar cell (int bits), for gathers
     int32_t* pt_pix;
     uint32_t* labels;
     uint32_t* obs_flag;      // scan scratch over points

@@ -86,6 +86,7 @@ __device__ __forceinline__ void project_tile(const DevBuf& d, const Layout& L, i
     }
     d.pt_cell[gi] = cell;
     d.pt_pix[gi] = pix;
+    d.pt_xyzc[gi] = make_float4(x, y, z, __int_as_float(cell));
 }
 
 __global__ void k_project(DevBuf d, Layout L) { project_tile(d, L, blockIdx.y, blockIdx.x); }
@@ -560,10 +561,10 @@ __global__ __launch_bounds__(256) void k_cand_fill(DevBuf d, Layout L) {
         bool pred = false;
         float x = 0, y = 0, z = 0;
         if (t < n) {
-            const int gi = fd.pt_off + items[t];
-            x = d.x[gi];
-            y = d.y[gi];
-            z = d.z[gi];
+            const float4 pc = d.pt_xyzc[fd.pt_off + items[t]];  // one gather per point
+            x = pc.x;
+            y = pc.y;
+            z = pc.z;
             pred = fabsf(e - z) < s.two_thr;
         }
         const uint64_t mask = __ballot(pred);
@@ -790,9 +791,9 @@ __global__ void k_pixel_label(DevBuf d, Layout L) {
         return;
     }
     const int i = static_cast<int>(key & ((1ull << L.idx_bits) - 1));
-    const int gi = fd.pt_off + i;
-    const float x = d.x[gi], y = d.y[gi], z = d.z[gi];
-    const int cell = d.pt_cell[gi];
+    const float4 pc = d.pt_xyzc[fd.pt_off + i];  // one gather: x, y, z and the cell
+    const float x = pc.x, y = pc.y, z = pc.z;
+    const int cell = __float_as_int(pc.w);
     const float e = d.elev[(size_t)f * L.cell_stride + cell];
     uint8_t lab = (z >= e + s.thr) ? kObstacle : kGround;
     const uint32_t* fs = d.fs + f * kFsCount;
