@@ -1410,7 +1410,12 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     const int small_grid = std::min(1024, std::max(8, 8192 / B));
     hipLaunchKernelGGL((k_hull_local<kWave, kLocalSmallV, true>), dim3(small_grid, B), dim3(kWave), 0, st, d, L,
                        list_xy);
-    hipLaunchKernelGGL((k_hull_local<1024, kLocalMax, false>), dim3(256, B), dim3(1024), 0, st, d, L, list_xy);
+    // The large clusters are a few dozen per frame at most: about 4096
+    // workgroups in all, 32 to 256 per frame (empty workgroups still cost
+    // dispatch).
+    const int large_grid = std::min(256, std::max(32, 4096 / B));
+    hipLaunchKernelGGL((k_hull_local<1024, kLocalMax, false>), dim3(large_grid, B), dim3(1024), 0, st, d, L,
+                       list_xy);
 }
 
 }  // namespace lp

@@ -825,7 +825,8 @@ void launch_segment_front(lp_context* c, int B) {
     hipLaunchKernelGGL(k_cell_fill, dim3(nb, B), dim3(256), 0, st, d, L);
     // About 8192 wavefronts in all, at least 64 and at most 4096 per frame.
     hipLaunchKernelGGL(k_cell_sort, dim3(std::min(1024, std::max(16, 2048 / B)), B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_cell_sort_big, dim3(64, B), dim3(1024), 0, st, d, L);
+    // Cells beyond kSmallCell points are rare: about 1024 workgroups in all.
+    hipLaunchKernelGGL(k_cell_sort_big, dim3(std::min(64, std::max(4, 1024 / B)), B), dim3(1024), 0, st, d, L);
     stage_mark(c, "recm");
     hipLaunchKernelGGL(k_recm, dim3((L.cand_stride / kRansacRadialBins + kWave - 1) / kWave, B), dim3(kWave), 0, st, d,
                        L);
