@@ -103,79 +103,147 @@ __global__ void k_noise_clear(DevBuf d, Layout L) {
 
 constexpr int kNearWindow = 8;  // input-order neighbours tested on each side
 
-// One thread per input point, two jobs on the same coordinates:
+constexpr int kPrepRows = 2;  // points per thread: consecutive runs of 256, their atomics in flight together
+constexpr int kPrepTile = 256 * kPrepRows;
+
+// Lanes of the wavefront that count: one atomic add by the lowest of them
+// for all, each lane's value the counter before it plus the flagged lanes
+// below it, over kPrepRows flag sets in row order.
+__device__ __forceinline__ void wave_reserve(uint32_t* ctr, const bool (&flag)[kPrepRows],
+                                             uint32_t (&out)[kPrepRows]) {
+    uint64_t m[kPrepRows];
+    uint32_t total = 0;
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        m[s] = __ballot(flag[s]);
+        total += static_cast<uint32_t>(__popcll(m[s]));
+    }
+    uint32_t base = 0;
+    if (total) {
+        if (lane_id() == static_cast<int>(__ffsll(static_cast<unsigned long long>(m[0] | m[1])) - 1))
+            base = atomicAdd(ctr, total);
+        base = __shfl(base, static_cast<int>(__ffsll(static_cast<unsigned long long>(m[0] | m[1])) - 1), kWave);
+    }
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        out[s] = base + lanes_below(m[s]);
+        base += static_cast<uint32_t>(__popcll(m[s]));
+    }
+}
+
+// Two jobs on the same coordinates, kPrepRows points per thread (rows of 256
+// consecutive points, so the lanes of a wavefront hold consecutive points):
 //  * input-order pass: noise[i] = 0 (VALID) when at least min_neighbours of
 //    the points i - 8 .. i + 8 (i itself included) lie within the radius,
 //    else 2, and the point joins the open list for the grid passes;
 //  * grid insert: the point's cell gets a hash slot (vslot) and the point a
 //    position inside its cell (nloc): the cell's count before the point's run
 //    plus its rank in the run, so the scatter needs no atomics.
+// The rows' atomics are issued side by side: the kernel waits on atomic
+// round trips, and two points per thread halve the waits per point.
 __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
-    __shared__ float4 tile_pts[256 + 2 * kNearWindow];
+    __shared__ float4 tile_pts[kPrepTile + 2 * kNearWindow];
+    static_assert(kPrepRows == 2, "wave_reserve combines two rows");
     int tile, f;
     xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
     const int n = fd.n_raw;
-    const int base = tile * blockDim.x;
+    const int base = tile * kPrepTile;
     if (base >= n) return;
-    for (int q = threadIdx.x; q < 256 + 2 * kNearWindow; q += blockDim.x) {
+    for (int q = threadIdx.x; q < kPrepTile + 2 * kNearWindow; q += blockDim.x) {
         const int j = min(max(base - kNearWindow + q, 0), n - 1);
         const int gj = fd.pt_off + j;
         tile_pts[q] = make_float4(d.rx[gj], d.ry[gj], d.rz[gj], 0.0f);
     }
     __syncthreads();
-    const int i = base + threadIdx.x;
-    if (i >= n) return;
     const NoiseConst nc = *d.nc;
-    const float4 t = tile_pts[threadIdx.x + kNearWindow];
-    const float r2 = noise_radius_sqr(nc, t.x, t.y);
-    const int lo = max(i - kNearWindow, 0), hi = min(i + kNearWindow, n - 1);
-    uint32_t cnt = 0;
-#pragma unroll
-    for (int o = -kNearWindow; o <= kNearWindow; ++o) {
-        const int j = i + o;
-        const bool in = j >= lo && j <= hi;
-        cnt += (in && noise_dist(t, tile_pts[threadIdx.x + kNearWindow + o]) <= r2) ? 1u : 0u;
-    }
-    const bool valid = cnt >= nc.k;
     const size_t po = (size_t)f * L.N;
-    d.noise[po + i] = valid ? 0 : 2;
-    if (!valid) {
-        const uint32_t q = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseOpen]);
-        d.nopen[po + q] = i;
-    }
-
-    int cx, cy, cz;
-    noise_cell(t.x, t.y, t.z, cx, cy, cz);
-    const int32_t key = noise_key(cx, cy, cz);
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const size_t ho = (size_t)f * L.hash_cap;
+    const int lane = lane_id();
+    int i[kPrepRows], cxs[kPrepRows], cys[kPrepRows];
+    bool open[kPrepRows];
+    int32_t key[kPrepRows];
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        const int e = s * 256 + static_cast<int>(threadIdx.x);
+        i[s] = base + e;
+        const bool in_frame = i[s] < n;
+        const float4 t = tile_pts[e + kNearWindow];
+        const float r2 = noise_radius_sqr(nc, t.x, t.y);
+        const int lo = max(i[s] - kNearWindow, 0), hi = min(i[s] + kNearWindow, n - 1);
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int o = -kNearWindow; o <= kNearWindow; ++o) {
+            const int j = i[s] + o;
+            const bool in = j >= lo && j <= hi;
+            cnt += (in && noise_dist(t, tile_pts[e + kNearWindow + o]) <= r2) ? 1u : 0u;
+        }
+        open[s] = in_frame && cnt < nc.k;
+        if (in_frame) d.noise[po + i[s]] = open[s] ? 2 : 0;
+        int cz;
+        noise_cell(t.x, t.y, t.z, cxs[s], cys[s], cz);
+        // Lanes past the frame's end take key -2: never a cell key (keys
+        // are non-negative), they form their own trailing run and insert
+        // nothing.
+        key[s] = in_frame ? noise_key(cxs[s], cys[s], cz) : -2;
+    }
+    uint32_t q[kPrepRows];
+    wave_reserve(&d.fs[f * kFsCount + kFsNoiseOpen], open, q);
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s)
+        if (open[s]) d.nopen[po + q[s]] = i[s];
+
     // A run of lanes with the same cell (consecutive points mostly share one)
     // inserts once: its lowest lane probes and adds the run's count; all
-    // runs at once.
-    const WaveRun run = wave_runs(key);
-    uint32_t h = 0, before = 0;
-    bool fresh = false;
-    if (lane_id() == run.leader) {
-        h = noise_slot(key, mask);
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-            const int32_t prev = atomicCAS(&d.hslot[ho + h].key, -1, key);
-            fresh = prev == -1;
-            if (prev == -1 || prev == key) break;
-            h = (h + 1) & mask;
+    // runs of both rows at once.
+    WaveRun run[kPrepRows];
+    bool lead[kPrepRows];
+    uint32_t h[kPrepRows];
+    int32_t prev[kPrepRows];
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        run[s] = wave_runs(key[s]);
+        lead[s] = lane == run[s].leader && key[s] >= 0;
+        h[s] = noise_slot(key[s], mask);
+        prev[s] = lead[s] ? atomicCAS(&d.hslot[ho + h[s]].key, -1, key[s]) : key[s];
+    }
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        // Collisions (another cell's key in the probed slot) probe on.
+        for (uint32_t probe = 1; probe <= mask && prev[s] != -1 && prev[s] != key[s]; ++probe) {
+            h[s] = (h[s] + 1) & mask;
+            prev[s] = atomicCAS(&d.hslot[ho + h[s]].key, -1, key[s]);
         }
-        before = atomicAdd(&d.hslot[ho + h].count, run.count);
-        atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cy * 32 + (cx >> 5)], 1u << (cx & 31));
+    }
+    bool fresh[kPrepRows];
+    uint32_t before[kPrepRows];
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        fresh[s] = lead[s] && prev[s] == -1;
+        before[s] = 0;
+        if (lead[s]) {
+            before[s] = atomicAdd(&d.hslot[ho + h[s]].count, run[s].count);
+            atomicOr(&d.nocc[(size_t)f * kNoiseOccWords + cys[s] * 32 + (cxs[s] >> 5)], 1u << (cxs[s] & 31));
+        }
     }
     // Every occupied cell gets a dense id (in any order): the cell counts are
     // scanned over the ids, not over the whole table.
-    if (fresh) {
-        const uint32_t id = wave_atomic_inc(&d.fs[f * kFsCount + kFsNoiseCells]);
-        d.ncell_slot[(size_t)f * L.N + id] = static_cast<int32_t>(h);
-        d.hslot[ho + h].dense = static_cast<int32_t>(id);
+    uint32_t id[kPrepRows];
+    wave_reserve(&d.fs[f * kFsCount + kFsNoiseCells], fresh, id);
+#pragma unroll
+    for (int s = 0; s < kPrepRows; ++s) {
+        if (fresh[s]) {
+            d.ncell_slot[(size_t)f * L.N + id[s]] = static_cast<int32_t>(h[s]);
+            d.hslot[ho + h[s]].dense = static_cast<int32_t>(id[s]);
+        }
+        const uint32_t slot = __shfl(h[s], run[s].leader, kWave);
+        const uint32_t place = __shfl(before[s], run[s].leader, kWave) + run[s].rank;
+        if (i[s] < n) {
+            d.vslot[po + i[s]] = static_cast<int32_t>(slot);
+            d.nloc[po + i[s]] = static_cast<int32_t>(place);
+        }
     }
-    d.vslot[po + i] = static_cast<int32_t>(__shfl(h, run.leader, kWave));
-    d.nloc[po + i] = static_cast<int32_t>(__shfl(before, run.leader, kWave) + run.rank);
 }
 
 // ncell_start holds the exclusive scan of the cell counts by dense id on
@@ -493,7 +561,7 @@ void launch_noise(lp_context* c, int B, bool compact) {
     const int nb = (L.N + 255) / 256;
     stage_mark(c, "noise");
     hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
-    hipLaunchKernelGGL(k_noise_prep, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_noise_prep, dim3((L.N + kPrepTile - 1) / kPrepTile, B), dim3(256), 0, st, d, L);
     launch_scan_gather(c, B, &d.hslot[0].count, L.hash_cap, 4, d.ncell_slot, d.ncell_start, L.N, kCountSlot,
                        kFsNoiseCells, -1);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
