@@ -205,47 +205,81 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t key, uint32_t mask) {
 
 // Voxel keys (clusterer.cpp:102-120, flatVoxelIndex clusterer.hpp:135-141)
 // inserted into the per-frame hash set.
+constexpr int kVoxRows = 2;  // obstacles per thread: rows of blockDim.x, their atomics in flight together
+
+// Obstacles tile * kVoxRows * blockDim.x + s * blockDim.x + threadIdx.x for
+// the rows s: lanes of a wavefront hold consecutive obstacles.
 __device__ __forceinline__ void voxel_insert_tile(const DevBuf& d, const Layout& L, int f, int tile) {
     const int nobs = static_cast<int>(d.fs[f * kFsCount + kFsObstacles]);
-    const int k = tile * blockDim.x + threadIdx.x;
-    if (k >= nobs) return;
+    const int base = tile * kVoxRows * static_cast<int>(blockDim.x);
+    if (base >= nobs) return;
     const CluConst cc = *d.cc;
     const Dims dm = dims_of(d, f, cc);
     const size_t ob = (size_t)f * L.N;
-    const int32_t ri = static_cast<int32_t>(d.sr[ob + k] / cc.res_r);
-    const int32_t ai = static_cast<int32_t>(d.sa[ob + k] / cc.res_az);
-    const int32_t ei = static_cast<int32_t>(d.se[ob + k] / cc.res_el);
-    const int32_t key = dm.nr * (dm.na * ei + ai) + ri;
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const size_t ho = (size_t)f * L.hash_cap;
+    const int lane = lane_id();
+    int k[kVoxRows];
+    int32_t key[kVoxRows];
+#pragma unroll
+    for (int s = 0; s < kVoxRows; ++s) {
+        k[s] = base + s * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+        key[s] = -2;  // past the obstacles: never a voxel key (keys are non-negative), inserts nothing
+        if (k[s] < nobs) {
+            const int32_t ri = static_cast<int32_t>(d.sr[ob + k[s]] / cc.res_r);
+            const int32_t ai = static_cast<int32_t>(d.sa[ob + k[s]] / cc.res_az);
+            const int32_t ei = static_cast<int32_t>(d.se[ob + k[s]] / cc.res_el);
+            key[s] = dm.nr * (dm.na * ei + ai) + ri;
+        }
+    }
     // A run of lanes with the same voxel inserts once: its lowest lane (which
     // holds the run's smallest point index, lanes follow point order) probes
-    // and records the run's count and minimum index; all runs at once.
-    const WaveRun run = wave_runs(key);
-    uint32_t h = 0;
-    if (lane_id() == run.leader) {
-        h = hash_slot(key, mask);
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-            const int32_t prev = atomicCAS(&d.hslot[ho + h].key, -1, key);
-            if (prev == -1) {
-                const uint32_t pos = atomicAdd(&d.fs[f * kFsCount + kFsVoxels], 1u);
-                if (pos < static_cast<uint32_t>(L.N)) {
-                    d.vlist[ob + pos] = static_cast<int32_t>(h);
-                    d.vpar[ob + pos] = static_cast<int32_t>(pos);
-                    d.vmin[ob + pos] = 0x7fffffff;
-                    d.vcnt[ob + pos] = 0;
-                }
-                d.hslot[ho + h].dense = static_cast<int32_t>(pos);
-                break;
-            }
-            if (prev == key) break;
-            h = (h + 1) & mask;
-        }
-        atomicMin(&d.hslot[ho + h].aux, k);
-        atomicAdd(&d.hslot[ho + h].count, run.count);
+    // and records the run's count and minimum index; all runs of both rows
+    // at once.
+    WaveRun run[kVoxRows];
+    bool lead[kVoxRows];
+    uint32_t h[kVoxRows];
+    int32_t prev[kVoxRows];
+#pragma unroll
+    for (int s = 0; s < kVoxRows; ++s) {
+        run[s] = wave_runs(key[s]);
+        lead[s] = lane == run[s].leader && key[s] >= 0;
+        h[s] = hash_slot(key[s], mask);
+        prev[s] = lead[s] ? atomicCAS(&d.hslot[ho + h[s]].key, -1, key[s]) : key[s];
     }
-    const int32_t slot = static_cast<int32_t>(__shfl(h, run.leader, kWave));
-    d.vslot[ob + k] = slot;
+#pragma unroll
+    for (int s = 0; s < kVoxRows; ++s) {
+        for (uint32_t probe = 1; probe <= mask && prev[s] != -1 && prev[s] != key[s]; ++probe) {
+            h[s] = (h[s] + 1) & mask;
+            prev[s] = atomicCAS(&d.hslot[ho + h[s]].key, -1, key[s]);
+        }
+    }
+    bool fresh[kVoxRows];
+#pragma unroll
+    for (int s = 0; s < kVoxRows; ++s) {
+        fresh[s] = lead[s] && prev[s] == -1;
+        if (lead[s]) {
+            atomicMin(&d.hslot[ho + h[s]].aux, k[s]);
+            atomicAdd(&d.hslot[ho + h[s]].count, run[s].count);
+        }
+    }
+    // Dense ids of the new voxels, in any order.
+    uint32_t pos[kVoxRows];
+    wave_reserve<kVoxRows>(&d.fs[f * kFsCount + kFsVoxels], fresh, pos);
+#pragma unroll
+    for (int s = 0; s < kVoxRows; ++s) {
+        if (fresh[s]) {
+            if (pos[s] < static_cast<uint32_t>(L.N)) {
+                d.vlist[ob + pos[s]] = static_cast<int32_t>(h[s]);
+                d.vpar[ob + pos[s]] = static_cast<int32_t>(pos[s]);
+                d.vmin[ob + pos[s]] = 0x7fffffff;
+                d.vcnt[ob + pos[s]] = 0;
+            }
+            d.hslot[ho + h[s]].dense = static_cast<int32_t>(pos[s]);
+        }
+        const int32_t slot = static_cast<int32_t>(__shfl(h[s], run[s].leader, kWave));
+        if (k[s] < nobs) d.vslot[ob + k[s]] = slot;
+    }
 }
 
 __global__ void k_voxel_insert(DevBuf d, Layout L) {
@@ -255,7 +289,8 @@ __global__ void k_voxel_insert(DevBuf d, Layout L) {
 }
 
 __global__ __launch_bounds__(256) void k_voxel_insert_q(DevBuf d, Layout L, int B) {
-    wq_run(d.wq + kWqVoxelInsert * 8, B, blockDim.x, [&](int f) { return d.fs[f * kFsCount + kFsObstacles]; },
+    wq_run(d.wq + kWqVoxelInsert * 8, B, kVoxRows * blockDim.x,
+           [&](int f) { return d.fs[f * kFsCount + kFsObstacles]; },
            [&](int f, int t) { voxel_insert_tile(d, L, f, t); });
 }
 
@@ -601,7 +636,8 @@ void launch_cluster(lp_context* c, int B, bool from_split) {
         hipLaunchKernelGGL(k_voxel_insert_q, dim3(kPersistentWorkgroups), dim3(wg), 0, st, d, L, B);
     } else {
         hipLaunchKernelGGL(k_spherical, dim3((L.N + wg - 1) / wg, B), dim3(wg), 0, st, d, L);
-        hipLaunchKernelGGL(k_voxel_insert, dim3((L.N + wg - 1) / wg, B), dim3(wg), 0, st, d, L);
+        hipLaunchKernelGGL(k_voxel_insert, dim3((L.N + kVoxRows * wg - 1) / (kVoxRows * wg), B), dim3(wg), 0, st, d,
+                           L);
     }
     stage_mark(c, "components");
     hipLaunchKernelGGL(k_voxel_neighbours, dim3(nb, B), dim3(256), 0, st, d, L);

@@ -378,6 +378,34 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T* lds, T* total) {
     return res;
 }
 
+// Places in a counter for R rows of flags: one atomic add by the lowest
+// flagged lane for the whole wavefront; each flagged lane's place is the
+// counter before the add plus the flagged (row, lane) pairs before it, rows
+// in order.
+template <int R>
+__device__ __forceinline__ void wave_reserve(uint32_t* ctr, const bool (&flag)[R], uint32_t (&out)[R]) {
+    uint64_t m[R];
+    uint64_t any = 0;
+    uint32_t total = 0;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        m[s] = __ballot(flag[s]);
+        any |= m[s];
+        total += static_cast<uint32_t>(__popcll(m[s]));
+    }
+    uint32_t base = 0;
+    if (total) {
+        const int first = static_cast<int>(__ffsll(static_cast<unsigned long long>(any))) - 1;
+        if (lane_id() == first) base = atomicAdd(ctr, total);
+        base = __shfl(base, first, kWave);
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        out[s] = base + lanes_below(m[s]);
+        base += static_cast<uint32_t>(__popcll(m[s]));
+    }
+}
+
 // Stream compaction inside one tile of NT * R elements held row by row
 // (element k * NT + threadIdx.x of the tile is the thread's element of row
 // k, so every row is read and written with coalesced accesses).  bits has

@@ -106,31 +106,6 @@ constexpr int kNearWindow = 8;  // input-order neighbours tested on each side
 constexpr int kPrepRows = 2;  // points per thread: consecutive runs of 256, their atomics in flight together
 constexpr int kPrepTile = 256 * kPrepRows;
 
-// Lanes of the wavefront that count: one atomic add by the lowest of them
-// for all, each lane's value the counter before it plus the flagged lanes
-// below it, over kPrepRows flag sets in row order.
-__device__ __forceinline__ void wave_reserve(uint32_t* ctr, const bool (&flag)[kPrepRows],
-                                             uint32_t (&out)[kPrepRows]) {
-    uint64_t m[kPrepRows];
-    uint32_t total = 0;
-#pragma unroll
-    for (int s = 0; s < kPrepRows; ++s) {
-        m[s] = __ballot(flag[s]);
-        total += static_cast<uint32_t>(__popcll(m[s]));
-    }
-    uint32_t base = 0;
-    if (total) {
-        if (lane_id() == static_cast<int>(__ffsll(static_cast<unsigned long long>(m[0] | m[1])) - 1))
-            base = atomicAdd(ctr, total);
-        base = __shfl(base, static_cast<int>(__ffsll(static_cast<unsigned long long>(m[0] | m[1])) - 1), kWave);
-    }
-#pragma unroll
-    for (int s = 0; s < kPrepRows; ++s) {
-        out[s] = base + lanes_below(m[s]);
-        base += static_cast<uint32_t>(__popcll(m[s]));
-    }
-}
-
 // Two jobs on the same coordinates, kPrepRows points per thread (rows of 256
 // consecutive points, so the lanes of a wavefront hold consecutive points):
 //  * input-order pass: noise[i] = 0 (VALID) when at least min_neighbours of
@@ -143,7 +118,6 @@ __device__ __forceinline__ void wave_reserve(uint32_t* ctr, const bool (&flag)[k
 // round trips, and two points per thread halve the waits per point.
 __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     __shared__ float4 tile_pts[kPrepTile + 2 * kNearWindow];
-    static_assert(kPrepRows == 2, "wave_reserve combines two rows");
     int tile, f;
     xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
@@ -189,7 +163,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
         key[s] = in_frame ? noise_key(cxs[s], cys[s], cz) : -2;
     }
     uint32_t q[kPrepRows];
-    wave_reserve(&d.fs[f * kFsCount + kFsNoiseOpen], open, q);
+    wave_reserve<kPrepRows>(&d.fs[f * kFsCount + kFsNoiseOpen], open, q);
 #pragma unroll
     for (int s = 0; s < kPrepRows; ++s)
         if (open[s]) d.nopen[po + q[s]] = i[s];
@@ -230,7 +204,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     // Every occupied cell gets a dense id (in any order): the cell counts are
     // scanned over the ids, not over the whole table.
     uint32_t id[kPrepRows];
-    wave_reserve(&d.fs[f * kFsCount + kFsNoiseCells], fresh, id);
+    wave_reserve<kPrepRows>(&d.fs[f * kFsCount + kFsNoiseCells], fresh, id);
 #pragma unroll
     for (int s = 0; s < kPrepRows; ++s) {
         if (fresh[s]) {
