@@ -588,7 +588,10 @@ def main():
         try:
             t = json.load(open(tpath)).get(kern)
             if t:
-                traffic = round(t["hbm_bytes_per_frame"] * B / launches[dom])
+                # hbm_bytes_per_frame is per dispatch (the average over the
+                # kernel's launch sites), so one launch over B frames moves
+                # B times it.
+                traffic = round(t["hbm_bytes_per_frame"] * B)
         except (OSError, ValueError, KeyError):
             traffic = None
     b8 = bytes_8d(n, P, n_obs, vox, hv)
