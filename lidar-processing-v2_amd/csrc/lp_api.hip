@@ -222,7 +222,6 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.hcy, 2 * B * N);
     go(x.hci, 2 * B * N);
     go(x.hc_count, 2 * B * N);
-    go(x.hc_chunks, B * N);
     go(x.hc_choff, B * N);
     go(x.hull_idx, B * N);
     go(x.hull_count, B * N);

@@ -467,8 +467,28 @@ __global__ void k_voxel_stats(DevBuf d, Layout L) {
     }
     const int32_t h = d.vlist[ob + v];
     const HashSlot hv = d.hslot[ho + h];
-    atomicMin(&d.vmin[ob + r], hv.aux);
-    atomicAdd(&d.vcnt[ob + r], hv.count);
+    // Voxels numbered one after the other mostly share a component: each run
+    // of equal roots in the wavefront reduces its minimum and count first
+    // (a segmented reduction down the run) and its leader adds them, so a
+    // large component's root takes one atomic pair per run, not per voxel.
+    const WaveRun run = wave_runs(r);
+    const int lane = lane_id();
+    const int end = run.leader + static_cast<int>(run.count);  // one past the run's last lane
+    int32_t mn = hv.aux;
+    uint32_t cnt = hv.count;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int32_t omn = __shfl_down(mn, o, kWave);
+        const uint32_t ocnt = __shfl_down(cnt, o, kWave);
+        if (lane + o < end) {
+            mn = min(mn, omn);
+            cnt += ocnt;
+        }
+    }
+    if (lane == run.leader) {
+        atomicMin(&d.vmin[ob + r], mn);
+        atomicAdd(&d.vcnt[ob + r], cnt);
+    }
 }
 
 __device__ __forceinline__ int32_t root_of_point(const DevBuf& d, size_t ob, size_t ho, int k) {

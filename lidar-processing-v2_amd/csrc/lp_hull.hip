@@ -596,26 +596,53 @@ __device__ __forceinline__ int upper_bound_u32(const uint32_t* a, int n, uint32_
     return lo;
 }
 
-// Chunk counts for round r: a cluster that is not final yet needs
-// ceil(cnt / 256) chunks (one if it fits: that chunk's hull is final).
-__global__ void k_hull_chunks(DevBuf d, Layout L, int r) {
-    const int f = blockIdx.y;
+// Chunk plan for round r, one 1024-thread workgroup per frame walking the
+// clusters in tiles with a running carry: a cluster that is not final yet
+// needs ceil(cnt / 256) chunks (one if it fits: that chunk's hull is final);
+// hc_choff receives the exclusive scan of the chunk counts and the frame's
+// total goes to kFsHullChunks.  Round 0 also clears the hull count and flag,
+// and every round clears the next round's candidate counts.
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanItems = 4;
+
+__global__ __launch_bounds__(kPlanThreads) void k_hull_plan(DevBuf d, Layout L, int r) {
+    __shared__ uint32_t lds[kPlanThreads / kWave + 1];
+    const int f = blockIdx.x;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= K) return;
     const size_t ob = (size_t)f * L.N;
     const size_t BN = (size_t)L.B * L.N;
-    uint32_t nch = 0;
-    if (r == 0) {
-        d.hull_count[ob + c] = 0;
-        d.hull_flag[ob + c] = 0;
+    uint32_t carry = 0;
+    for (int base = 0; base < K; base += kPlanThreads * kPlanItems) {
+        uint32_t v[kPlanItems];
+        uint32_t local = 0;
+        const int b = base + static_cast<int>(threadIdx.x) * kPlanItems;
+#pragma unroll
+        for (int q = 0; q < kPlanItems; ++q) {
+            const int c = b + q;
+            v[q] = 0;
+            if (c < K) {
+                if (r == 0) {
+                    d.hull_count[ob + c] = 0;
+                    d.hull_flag[ob + c] = 0;
+                }
+                if (r == 0 || d.hull_count[ob + c] == 0) {
+                    const uint32_t cnt = d.hc_count[(r & 1) * BN + ob + c];
+                    v[q] = cnt <= static_cast<uint32_t>(kHullChunk) ? 1u : (cnt + kHullChunk - 1) / kHullChunk;
+                }
+                d.hc_count[((r + 1) & 1) * BN + ob + c] = 0;
+                local += v[q];
+            }
+        }
+        uint32_t total;
+        uint32_t ex = block_exclusive_sum<kPlanThreads>(local, lds, &total) + carry;
+#pragma unroll
+        for (int q = 0; q < kPlanItems; ++q) {
+            if (b + q < K) d.hc_choff[ob + b + q] = ex;
+            ex += v[q];
+        }
+        carry += total;
     }
-    if (r == 0 || d.hull_count[ob + c] == 0) {
-        const uint32_t cnt = d.hc_count[(r & 1) * BN + ob + c];
-        nch = cnt <= static_cast<uint32_t>(kHullChunk) ? 1u : (cnt + kHullChunk - 1) / kHullChunk;
-    }
-    d.hc_chunks[ob + c] = nch;
-    d.hc_count[((r + 1) & 1) * BN + ob + c] = 0;
+    if (threadIdx.x == 0) d.fs[f * kFsCount + kFsHullChunks] = carry;
 }
 
 // Register budget for three wavefronts per SIMD (168 VGPRs, a few spills in
@@ -1384,7 +1411,6 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     DevBuf& d = c->d;
     const Layout& L = c->L;
     hipStream_t st = c->stream;
-    const int nb = (L.N + 255) / 256;
     // Stage marks around the single-kernel steps ("hull_round" three
     // times), so profiled runs time each round launch on its own.
     // Chunk wavefronts loop over the frame's chunks: about 2048 workgroups
@@ -1393,8 +1419,7 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     const int round_grid = std::min(512, std::max(8, 2048 / B));
     for (int r = 0; r < kHullRounds; ++r) {
         stage_mark(c, "hull");
-        hipLaunchKernelGGL(k_hull_chunks, dim3(nb, B), dim3(256), 0, st, d, L, r);
-        launch_scan_small(c, B, d.hc_chunks, d.hc_choff, L.N, kCountSlot, kFsClusters, kFsHullChunks);
+        hipLaunchKernelGGL(k_hull_plan, dim3(B), dim3(kPlanThreads), 0, st, d, L, r);
         stage_mark(c, "hull_round");
         hipLaunchKernelGGL(k_hull_round, dim3(round_grid, B), dim3(256), 0, st, d, L, r);
     }

@@ -181,7 +181,6 @@ struct DevBuf {
     double* hcy;
     int32_t* hci;            // candidate obstacle index
     uint32_t* hc_count;      // per cluster candidates (two buffers)
-    uint32_t* hc_chunks;     // per cluster chunks of the current round
     uint32_t* hc_choff;      // their exclusive scan
     int32_t* hull_idx;       // final hull (obstacle indices), frame stride N
     uint32_t* hull_count;    // per cluster
