@@ -157,7 +157,6 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.spos, B * N);
     go(x.nocc, B * kNoiseOccWords);
     go(x.ncell_slot, B * N);
-    go(x.ncell_start, B * N);
     go(x.pt_cell, B * N);
     go(x.pt_xyzc, B * N);
     go(x.pt_pix, B * N);

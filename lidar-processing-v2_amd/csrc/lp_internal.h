@@ -115,7 +115,6 @@ struct DevBuf {
     int32_t* nloc;           // = clabel (free until clustering): each point's place in its cell, then the listed points' input indices
     uint32_t* nocc;          // per frame: kNoiseOccWords words, one bit per occupied 1 m (x, y) column
     int32_t* ncell_slot;     // per frame (stride N): hash slot of each occupied noise cell, by dense id
-    uint32_t* ncell_start;   // per frame (stride N): first cell-sorted position of each cell, by dense id
     // per point
     int32_t* pt_cell;
     float4* pt_xyzc;         // per point (CSR): x, y, z and the polar cell (int bits), for gathers

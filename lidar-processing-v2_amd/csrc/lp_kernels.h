@@ -31,6 +31,10 @@ void launch_scan_small(lp_context* c, int B, const uint32_t* in, uint32_t* out, 
 // in[(f * in_stride + gather[f * stride + i]) * elem] (a field of a record array).
 void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, int elem, const int32_t* gather,
                         uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot);
+// The same, each result written back through the gather index into another
+// field of the record array (`field` is that field of record 0).
+void launch_scan_gather_into(lp_context* c, int B, const uint32_t* in, size_t in_stride, int elem,
+                             const int32_t* gather, uint32_t* field, size_t stride, ScanCount kind, int count_slot);
 
 void launch_mt19937(uint32_t* out, int count, hipStream_t st);
 // Resets the hash slots slots[f * N .. + fs[count_slot]) of every frame to

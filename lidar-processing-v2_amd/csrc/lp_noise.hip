@@ -220,9 +220,9 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     }
 }
 
-// ncell_start holds the exclusive scan of the cell counts by dense id on
-// entry.  Each point goes to its cell's start plus its place in the cell;
-// the point at place 0 writes the cell's start to the slot's aux field.
+// Each slot's aux holds its cell's first cell-sorted position on entry (the
+// exclusive scan of the cell counts by dense id); each point goes to that
+// start plus its place in the cell.
 __global__ void k_noise_scatter(DevBuf d, Layout L) {
     int tile, f;
     xcd_tile_frame(tile, f);
@@ -234,9 +234,7 @@ __global__ void k_noise_scatter(DevBuf d, Layout L) {
     const size_t ho = (size_t)f * L.hash_cap;
     const int32_t h = d.vslot[po + i];
     const int loc = d.nloc[po + i];
-    const int start = static_cast<int>(d.ncell_start[po + d.hslot[ho + h].dense]);
-    if (loc == 0) d.hslot[ho + h].aux = start;
-    const int pos = start + loc;
+    const int pos = d.hslot[ho + h].aux + loc;  // the cell's start plus the point's place
     d.dsort[po + pos] = make_float4(d.rx[gi], d.ry[gi], d.rz[gi], __int_as_float(h));
     d.spos[po + i] = pos;
 }
@@ -536,8 +534,10 @@ void launch_noise(lp_context* c, int B, bool compact) {
     stage_mark(c, "noise");
     hipLaunchKernelGGL(k_noise_clear, dim3(256, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_noise_prep, dim3((L.N + kPrepTile - 1) / kPrepTile, B), dim3(256), 0, st, d, L);
-    launch_scan_gather(c, B, &d.hslot[0].count, L.hash_cap, 4, d.ncell_slot, d.ncell_start, L.N, kCountSlot,
-                       kFsNoiseCells, -1);
+    // Cell starts (exclusive scan of the counts by dense id) straight into
+    // the slots' aux fields.
+    launch_scan_gather_into(c, B, &d.hslot[0].count, L.hash_cap, 4, d.ncell_slot,
+                            reinterpret_cast<uint32_t*>(&d.hslot[0].aux), L.N, kCountSlot, kFsNoiseCells);
     hipLaunchKernelGGL(k_noise_scatter, dim3(nb, B), dim3(256), 0, st, d, L);
     stage_mark(c, "noise_count");
     // The open list is a few percent of the points: a quarter of the point

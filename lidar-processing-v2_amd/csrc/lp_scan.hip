@@ -55,8 +55,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(DevBuf d, Layout L
 // Down pass: each tile adds the totals of the tiles before it (read straight
 // from the reduce pass, at most blk_stride of them) to its own exclusive
 // scan; the last tile writes the frame total.  Two launches per scan.
+// With out_gathered the result for element i goes where its input came from,
+// into the record field `out` points at (out is that field of record 0).
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, ScanIn si, uint32_t* out,
-                                                            size_t stride, ScanCount kind, int slot, int total_slot) {
+                                                            size_t stride, ScanCount kind, int slot, int total_slot,
+                                                            bool out_gathered) {
     __shared__ uint32_t lds[kScanThreads / kWave + 1];
     const int f = blockIdx.y;
     const int n = scan_count(d, L, f, kind, slot);
@@ -84,7 +87,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(DevBuf d, Layout L, 
     uint32_t ex = block_exclusive_sum<kScanThreads>(s, lds, &total) + pre_total;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if (b + k < n) o[b + k] = ex;
+        if (b + k < n) {
+            if (out_gathered)
+                out[((size_t)f * si.in_stride + si.gather[(size_t)f * stride + b + k]) * si.elem] = ex;
+            else
+                o[b + k] = ex;
+        }
         ex += v[k];
     }
     if (threadIdx.x == 0 && total_slot >= 0 && base + kScanTile >= n)
@@ -134,14 +142,14 @@ void launch_scan_small(lp_context* c, int B, const uint32_t* in, uint32_t* out, 
 }
 
 static void launch_scan_in(lp_context* c, int B, ScanIn si, uint32_t* out, size_t stride, ScanCount kind,
-                           int count_slot, int total_slot) {
+                           int count_slot, int total_slot, bool out_gathered = false) {
     const Layout& L = c->L;
     const int maxn = static_cast<int>(stride);
     const int nb = (maxn + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, si, stride, kind,
                        count_slot);
     hipLaunchKernelGGL(k_scan_down, dim3(nb, B), dim3(kScanThreads), 0, c->stream, c->d, L, si, out, stride, kind,
-                       count_slot, total_slot);
+                       count_slot, total_slot, out_gathered);
 }
 
 void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t stride, ScanCount kind,
@@ -152,6 +160,11 @@ void launch_scan(lp_context* c, int B, const uint32_t* in, uint32_t* out, size_t
 void launch_scan_gather(lp_context* c, int B, const uint32_t* in, size_t in_stride, int elem, const int32_t* gather,
                         uint32_t* out, size_t stride, ScanCount kind, int count_slot, int total_slot) {
     launch_scan_in(c, B, ScanIn{in, in_stride, gather, elem}, out, stride, kind, count_slot, total_slot);
+}
+
+void launch_scan_gather_into(lp_context* c, int B, const uint32_t* in, size_t in_stride, int elem,
+                             const int32_t* gather, uint32_t* field, size_t stride, ScanCount kind, int count_slot) {
+    launch_scan_in(c, B, ScanIn{in, in_stride, gather, elem}, field, stride, kind, count_slot, -1, true);
 }
 
 }  // namespace lp
