@@ -191,6 +191,84 @@ __device__ int half_chain(const double* X, const double* Y, const uint16_t* P, i
     return kk;
 }
 
+// The same chain as half_chain, computed by the whole wavefront by divide
+// and conquer: the sequence (L[0..n), reversed for the upper chain) is cut
+// into 64 consecutive blocks of at most ceil(n / 64) points, each lane runs
+// the chain over its block, and six levels merge neighbouring chains.  Two
+// chains of consecutive blocks meet at their bridge (a, b): starting from the
+// left chain's last and the right chain's first vertex, a is dropped while
+// the turn (a - 1, a, b) is not strictly positive and b while (a, b, b + 1)
+// is not, until both turns are; the merged chain is the left one up to a and
+// the right one from b.  With sign-exact tests the chain of a sorted
+// sequence is the unique strictly convex chain of its points, so the result
+// equals the sequential walk's; the walk's depth (a pop loop per point) is
+// replaced by six short merges, which pays on long, nearly convex halves
+// (float-rounded straight edges keep most of their points).  S receives the
+// chain (S has room for n entries); every lane returns its length.
+__device__ int merge_half_chain(const double* X, const double* Y, const uint16_t* L, int n, bool upper, uint16_t* S,
+                                uint32_t* inexact) {
+    const int lane = lane_id();
+    const int bs = (n + kWave - 1) / kWave;  // block size, at most 4 for a 256-point chunk
+    for (int k = lane; k < n; k += kWave) S[k] = L[upper ? n - 1 - k : k];
+    wave_sync();
+    // Level 0: each lane's block, in place at its start.
+    int len = 0;
+    {
+        const int b0 = lane * bs, b1 = min(b0 + bs, n);
+        for (int k = b0; k < b1; ++k) {
+            const int p = S[k];
+            while (len > 1) {
+                const int a = S[b0 + len - 2], b = S[b0 + len - 1];
+                int ns;
+                const int o = orient_exact(X[a], Y[a], X[b], Y[b], X[p], Y[p], &ns);
+                *inexact += (o != ns) ? 1u : 0u;
+                if (o > 0) break;
+                --len;
+            }
+            S[b0 + len++] = static_cast<uint16_t>(p);
+        }
+    }
+    wave_sync();
+    // Merge levels: lane g (a multiple of 2w) joins its chain with lane
+    // g + w's; the chain lengths travel by lane shuffles.
+    for (int w = 1; w < kWave; w <<= 1) {
+        const int right_len = __shfl(len, min(lane + w, kWave - 1), kWave);
+        if ((lane & (2 * w - 1)) == 0 && lane + w < kWave && (lane + w) * bs < n) {
+            const int lb = lane * bs, rb = (lane + w) * bs;
+            const int nl = len, nr = right_len;
+            int ia = nl - 1, ib = 0;
+            bool moved = true;
+            while (moved) {
+                moved = false;
+                while (ia > 0) {
+                    const int a0 = S[lb + ia - 1], a = S[lb + ia], b = S[rb + ib];
+                    int ns;
+                    const int o = orient_exact(X[a0], Y[a0], X[a], Y[a], X[b], Y[b], &ns);
+                    *inexact += (o != ns) ? 1u : 0u;
+                    if (o > 0) break;
+                    --ia;
+                    moved = true;
+                }
+                while (ib < nr - 1) {
+                    const int a = S[lb + ia], b = S[rb + ib], b1 = S[rb + ib + 1];
+                    int ns;
+                    const int o = orient_exact(X[a], Y[a], X[b], Y[b], X[b1], Y[b1], &ns);
+                    *inexact += (o != ns) ? 1u : 0u;
+                    if (o > 0) break;
+                    ++ib;
+                    moved = true;
+                }
+            }
+            // Right part moves left to follow a (its destination never
+            // passes its source).
+            for (int k = ib; k < nr; ++k) S[lb + ia + 1 + k - ib] = S[rb + k];
+            len = ia + 1 + nr - ib;
+        }
+        wave_sync();
+    }
+    return __shfl(len, 0, kWave);
+}
+
 // Splits the n >= 2 sorted points P by the line from P[0] to P[n-1]: LO gets
 // P[0], the points strictly below it (in order), P[n-1]; UP the same with the
 // points strictly above.  Only those can be vertices of the lower / upper
@@ -281,6 +359,7 @@ __device__ int prune_chain(const double* X, const double* Y, uint16_t* L, uint16
 }
 
 constexpr int kPrunePasses = 6;
+constexpr int kMergeChain = 32;  // half lists longer than this take the wavefront chain
 constexpr int kBigPrunePasses = 32;
 constexpr int kWavesPerBlock = 4;
 constexpr int kHullRounds = 3;
@@ -842,6 +921,14 @@ __global__ __launch_bounds__(256, 3) void k_hull_round(DevBuf d, Layout L, int r
                 if (lane == 0) {
                     nlo[w] = cl;
                     nup[w] = cu;
+                }
+            } else if (max(cl, cu) > kMergeChain) {
+                // Long halves: the wavefront's divide-and-conquer chain.
+                const int kl2 = merge_half_chain(X, Y, lol[w], cl, false, S, &inexact);
+                const int ku2 = merge_half_chain(X, Y, upl[w], cu, true, S + kHullChunk, &inexact);
+                if (lane == 0) {
+                    nlo[w] = kl2;
+                    nup[w] = ku2;
                 }
             } else if (lane < 2) {
                 // One call with per-lane arguments, so the two chains run in
