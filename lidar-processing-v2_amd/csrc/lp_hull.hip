@@ -1084,17 +1084,23 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
                 }
             }
             __syncthreads();
-            if (!nlist_done && tid < 2) {
+            if (!nlist_done && tid < 2 * kWave) {
+                // Wavefront 0 the lower chain, wavefront 1 the upper one, each
+                // by the divide-and-conquer chain.
+                const int wv = tid / kWave;
                 uint32_t inexact = 0;
-                const int len = half_chain(X, Y, tid == 0 ? LO : UP, nlist[tid], tid == 1,
-                                           tid == 1 ? S + kHullBig : S, &inexact);
-                if (tid == 0)
-                    nlo = len;
-                else
-                    nup = len;
-                if (inexact) {
-                    atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
-                    atomicOr(&d.hull_flag[ob + c], 1u);
+                const int len = merge_half_chain(X, Y, wv == 0 ? LO : UP, nlist[wv], wv == 1,
+                                                 wv == 1 ? S + kHullBig : S, &inexact);
+                inexact = wave_sum(inexact);
+                if (lane_id() == 0) {
+                    if (wv == 0)
+                        nlo = len;
+                    else
+                        nup = len;
+                    if (inexact) {
+                        atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                        atomicOr(&d.hull_flag[ob + c], 1u);
+                    }
                 }
             }
         }
