@@ -294,43 +294,12 @@ __global__ __launch_bounds__(256) void k_voxel_insert_q(DevBuf d, Layout L, int 
            [&](int f, int t) { voxel_insert_tile(d, L, f, t); });
 }
 
-__device__ __forceinline__ int32_t hash_find(const HashSlot* hs, uint32_t mask, int32_t key) {
-    uint32_t h = hash_slot(key, mask);
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const int32_t k = hs[h].key;
-        if (k == key) return static_cast<int32_t>(h);
-        if (k == -1) return -1;
-        h = (h + 1) & mask;
-    }
-    return -1;
-}
 
 // Half of the 26-neighbourhood (the other half is covered from the
 // neighbour's side): offsets (dr, da, de) with (de, da, dr) > (0, 0, 0).
 __constant__ static const int8_t kHalfNb[13][3] = {
     {1, 0, 0},  {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},  {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
     {-1, 0, 1}, {0, 0, 1},  {1, 0, 1},  {-1, 1, 1}, {0, 1, 1},   {1, 1, 1}};
-
-// Dense id of neighbour q of the voxel with flat key `key`
-// (clusterer.cpp:150-176 bounds: range and elevation in [0, num), azimuth
-// wraps by one step); -1 if that voxel is empty.
-__device__ __forceinline__ int32_t voxel_neighbour(const DevBuf& d, size_t ho, uint32_t mask, const Dims& dm,
-                                                   int32_t key, int q) {
-    const int32_t ri = key % dm.nr;
-    const int32_t rest = key / dm.nr;
-    const int32_t ai = rest % dm.na;
-    const int32_t ei = rest / dm.na;
-    const int32_t rr = ri + kHalfNb[q][0];
-    const int32_t ee = ei + kHalfNb[q][2];
-    if (rr < 0 || rr >= dm.nr || ee < 0 || ee >= dm.ne) return -1;
-    int32_t aa = ai + kHalfNb[q][1];
-    if (aa < 0)
-        aa += dm.na;
-    else if (aa >= dm.na)
-        aa -= dm.na;
-    const int32_t g = hash_find(d.hslot + ho, mask, dm.nr * (dm.na * ee + aa) + rr);
-    return g < 0 ? -1 : d.hslot[ho + g].dense;
-}
 
 // ---- union-find in LDS ----------------------------------------------------
 __device__ __forceinline__ int32_t lds_find(volatile int32_t* par, int32_t x) {
@@ -376,8 +345,39 @@ __global__ void k_voxel_neighbours(DevBuf d, Layout L) {
     const uint32_t mask = static_cast<uint32_t>(L.hash_cap) - 1;
     const int32_t key = d.hslot[ho + d.vlist[ob + v]].key;
     int32_t* nb = d.vnb + ob * 13;
+    // Neighbour q of the voxel (clusterer.cpp:150-176 bounds: range and
+    // elevation in [0, num), azimuth wraps by one step) is its dense id, or -1
+    // if that voxel is empty.  All 13 first probes are issued before any is
+    // waited on (collisions, rare at the table's load, probe on afterwards).
+    const int32_t ri = key % dm.nr;
+    const int32_t rest = key / dm.nr;
+    const int32_t ai = rest % dm.na;
+    const int32_t ei = rest / dm.na;
+    int32_t nk[13];
+    uint32_t h[13];
+    HashSlot rec[13];
 #pragma unroll
-    for (int q = 0; q < 13; ++q) nb[(size_t)q * L.N + v] = voxel_neighbour(d, ho, mask, dm, key, q);
+    for (int q = 0; q < 13; ++q) {
+        const int32_t rr = ri + kHalfNb[q][0];
+        const int32_t ee = ei + kHalfNb[q][2];
+        int32_t aa = ai + kHalfNb[q][1];
+        if (aa < 0)
+            aa += dm.na;
+        else if (aa >= dm.na)
+            aa -= dm.na;
+        nk[q] = (rr < 0 || rr >= dm.nr || ee < 0 || ee >= dm.ne) ? -1 : dm.nr * (dm.na * ee + aa) + rr;
+        h[q] = hash_slot(nk[q], mask);
+        rec[q] = HashSlot{-1, 0u, -1, 0};
+        if (nk[q] >= 0) rec[q] = d.hslot[ho + h[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+        for (uint32_t probe = 1; probe <= mask && nk[q] >= 0 && rec[q].key != nk[q] && rec[q].key != -1; ++probe) {
+            h[q] = (h[q] + 1) & mask;
+            rec[q] = d.hslot[ho + h[q]];
+        }
+        nb[(size_t)q * L.N + v] = (nk[q] >= 0 && rec[q].key == nk[q]) ? rec[q].dense : -1;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_voxel_cc(DevBuf d, Layout L) {
