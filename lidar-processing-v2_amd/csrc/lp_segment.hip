@@ -701,43 +701,55 @@ __global__ __launch_bounds__(64) void k_ransac_hyp(DevBuf d, Layout L) {
 // Inlier counts of all 60 hypotheses at once: one candidate per thread,
 // ballot + popcount per hypothesis, LDS totals, one atomic per block and
 // hypothesis (segmenter.cpp:420-431).
+constexpr int kScoreRows = 4;  // candidates per thread: rows of 256, each plane read once for all of them
+
+// Every candidate scores all hypotheses (segmenter.cpp:410-430: |a x + b y +
+// c z - d| < thr, the same float expression).  The planes are read as
+// wavefront-uniform values (scalar loads, no LDS); each wavefront counts a
+// plane's hits with ballots over its rows, lane `it` keeps plane it's count,
+// and the block's counts meet in LDS before one atomic per plane.
 __global__ __launch_bounds__(256) void k_ransac_count(DevBuf d, Layout L) {
-    __shared__ float hp[kRansacIters][4];
-    __shared__ uint32_t hv[kRansacIters];
     __shared__ uint32_t acc[kRansacIters];
+    static_assert(kRansacIters <= kWave, "one lane per hypothesis holds its count");
     const int f = blockIdx.y;
     const SegConst& s = d.frames[f].sc;
     const uint32_t n = d.fs[f * kFsCount + kFsCands];
-    if (n < 2 || blockIdx.x * 256u >= n) return;
+    const uint32_t base = blockIdx.x * (256u * kScoreRows);
+    if (n < 2 || base >= n) return;
     if ((d.fs[f * kFsCount + kFsErr] & (kErrRansacSingle | kErrRngExhausted)) != 0) return;
     const int tid = threadIdx.x;
-    if (tid < kRansacIters) {
-        const float* h = d.hyp + ((size_t)f * kRansacIters + tid) * 4;
-        hp[tid][0] = h[0];
-        hp[tid][1] = h[1];
-        hp[tid][2] = h[2];
-        hp[tid][3] = h[3];
-        hv[tid] = d.hyp_valid[f * kRansacIters + tid];
-        acc[tid] = 0;
-    }
-    __syncthreads();
-    const uint32_t t = blockIdx.x * 256u + tid;
+    const int lane = lane_id();
+    if (tid < kRansacIters) acc[tid] = 0;
     const size_t ob = (size_t)f * L.N;
-    const bool act = t < n;
-    float x = 0, y = 0, z = 0;
-    if (act) {
-        x = d.cx[ob + t];
-        y = d.cy[ob + t];
-        z = d.cz[ob + t];
+    float x[kScoreRows], y[kScoreRows], z[kScoreRows];
+    bool act[kScoreRows];
+#pragma unroll
+    for (int r = 0; r < kScoreRows; ++r) {
+        const uint32_t t = base + r * 256u + tid;
+        act[r] = t < n;
+        x[r] = act[r] ? d.cx[ob + t] : 0.0f;
+        y[r] = act[r] ? d.cy[ob + t] : 0.0f;
+        z[r] = act[r] ? d.cz[ob + t] : 0.0f;
     }
+    const float4* hyp = reinterpret_cast<const float4*>(d.hyp) + (size_t)f * kRansacIters;
+    const uint32_t* valid = d.hyp_valid + f * kRansacIters;
+    const float thr = s.thr;
+    uint32_t mine = 0;
     for (int it = 0; it < kRansacIters; ++it) {
-        if (!hv[it]) continue;
-        const float od = fabsf((hp[it][0] * x) + (hp[it][1] * y) + (hp[it][2] * z) - hp[it][3]);
-        const uint64_t m = __ballot(act && od < s.thr);
-        if (lane_id() == 0) atomicAdd(&acc[it], static_cast<uint32_t>(__popcll(m)));
+        if (!valid[it]) continue;
+        const float4 p = hyp[it];
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < kScoreRows; ++r) {
+            const float od = fabsf((p.x * x[r]) + (p.y * y[r]) + (p.z * z[r]) - p.w);
+            c += static_cast<uint32_t>(__popcll(__ballot(act[r] && od < thr)));
+        }
+        mine += lane == it ? c : 0u;
     }
     __syncthreads();
-    if (tid < kRansacIters && hv[tid] && acc[tid]) atomicAdd(&d.hyp_count[f * kRansacIters + tid], acc[tid]);
+    if (lane < kRansacIters && mine) atomicAdd(&acc[lane], mine);
+    __syncthreads();
+    if (tid < kRansacIters && valid[tid] && acc[tid]) atomicAdd(&d.hyp_count[f * kRansacIters + tid], acc[tid]);
 }
 
 // Best hypothesis: strict `>` in iteration order, then the sign flip
@@ -836,7 +848,8 @@ void launch_segment_front(lp_context* c, int B) {
     hipLaunchKernelGGL(k_cand_scan, dim3(B), dim3(1024), 0, st, d, L);
     hipLaunchKernelGGL(k_cand_fill, dim3(128, B), dim3(256), 0, st, d, L);
     hipLaunchKernelGGL(k_ransac_hyp, dim3(B), dim3(64), 0, st, d, L);
-    hipLaunchKernelGGL(k_ransac_count, dim3(nb, B), dim3(256), 0, st, d, L);
+    hipLaunchKernelGGL(k_ransac_count, dim3((L.N + 256 * kScoreRows - 1) / (256 * kScoreRows), B), dim3(256), 0, st,
+                       d, L);
     hipLaunchKernelGGL(k_ransac_best, dim3((B + 63) / 64), dim3(64), 0, st, d, L);
     stage_mark(c, "pixels");
     hipLaunchKernelGGL(k_pixel_label, dim3((L.P + 255) / 256, B), dim3(256), 0, st, d, L);
