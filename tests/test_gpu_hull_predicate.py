@@ -100,3 +100,31 @@ def test_certified_lists_keep_the_exact_path(oracle):
     for (x, y), h in zip(L, hulls):
         ref = oracle.convex_hull(x, y)
         assert np.array_equal(np.stack([x[h], y[h]], 1), np.stack([x[ref], y[ref]], 1))
+
+
+def test_long_nearly_convex_lists_match_the_reference(oracle):
+    """Float-rounded straight edges and arcs keep most of their points as hull
+    vertices, so pruning barely shrinks them and the chunk hulls take the
+    wavefront's block-and-merge chain (and, past three rounds, the
+    big-cluster workgroup's).  Both must give the oracle's vertex lists."""
+    rng = np.random.default_rng(31)
+    lists = []
+    for n in (200, 257, 700, 3000):
+        t = np.sort(rng.uniform(0.0, 1.0, n))
+        # a wall seen from afar: a straight segment in float32
+        x = (10.0 + 40.0 * t).astype(np.float32).astype(np.float64)
+        y = (5.0 + 17.3 * t).astype(np.float32).astype(np.float64)
+        lists.append((x, y))
+        # a pole's front: an arc, float-rounded
+        a = np.pi * (0.2 + 0.6 * t)
+        lists.append(((30.0 + 0.15 * np.cos(a)).astype(np.float32).astype(np.float64),
+                      (-12.0 + 0.15 * np.sin(a)).astype(np.float32).astype(np.float64)))
+        # both, shuffled
+        k = rng.permutation(2 * n)
+        lists.append((np.concatenate([x, lists[-1][0]])[k], np.concatenate([y, lists[-1][1]])[k]))
+    with N.Context(max_frames=1, max_points=32768) as ctx:
+        st, hulls = ctx.convex_hulls(lists)
+    assert st == N.OK
+    for (x, y), h in zip(lists, hulls):
+        ref = oracle.convex_hull(x, y)
+        assert np.array_equal(np.stack([x[h], y[h]], 1), np.stack([x[ref], y[ref]], 1)), (len(x), len(h), len(ref))
