@@ -163,17 +163,40 @@ __device__ __forceinline__ void spherical_tile(const DevBuf& d, const Layout& L,
         ma = max_bits(az);
         me = max_bits(el);
     }
+    // The frame maxima: wavefront maxima (DPP inside 16-lane rows, then two
+    // shuffles), the workgroup's in LDS, one atomic per workgroup and value
+    // (per wavefront, the per-frame counters were a contention point).
+    __shared__ uint32_t red[3][1024 / kWave];
 #pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        mr = max(mr, __shfl_xor(mr, o, kWave));
-        ma = max(ma, __shfl_xor(ma, o, kWave));
-        me = max(me, __shfl_xor(me, o, kWave));
+    for (int o = 1; o < kWave; o <<= 1) {
+        if (o < 16) {
+            mr = max(mr, lane_xor(mr, o));
+            ma = max(ma, lane_xor(ma, o));
+            me = max(me, lane_xor(me, o));
+        } else {
+            mr = max(mr, static_cast<uint32_t>(__shfl_xor(mr, o, kWave)));
+            ma = max(ma, static_cast<uint32_t>(__shfl_xor(ma, o, kWave)));
+            me = max(me, static_cast<uint32_t>(__shfl_xor(me, o, kWave)));
+        }
     }
+    const int w = static_cast<int>(threadIdx.x) / kWave;
     if (lane_id() == 0) {
+        red[0][w] = mr;
+        red[1][w] = ma;
+        red[2][w] = me;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < static_cast<int>(blockDim.x) / kWave; ++q) {
+            mr = max(mr, red[0][q]);
+            ma = max(ma, red[1][q]);
+            me = max(me, red[2][q]);
+        }
         atomicMax(&d.fs[f * kFsCount + kFsMaxR], mr);
         atomicMax(&d.fs[f * kFsCount + kFsMaxA], ma);
         atomicMax(&d.fs[f * kFsCount + kFsMaxE], me);
     }
+    __syncthreads();  // red is reused by the workgroup's next tile
 }
 
 __global__ void k_spherical(DevBuf d, Layout L) { spherical_tile(d, L, blockIdx.y, blockIdx.x); }
