@@ -288,7 +288,8 @@ __device__ __forceinline__ void voxel_insert_tile(const DevBuf& d, const Layout&
     }
     // Dense ids of the new voxels, in any order.
     uint32_t pos[kVoxRows];
-    wave_reserve<kVoxRows>(&d.fs[f * kFsCount + kFsVoxels], fresh, pos);
+    __shared__ uint32_t rsv[1024 / kWave + 1];
+    block_reserve<kVoxRows>(&d.fs[f * kFsCount + kFsVoxels], fresh, pos, rsv);
 #pragma unroll
     for (int s = 0; s < kVoxRows; ++s) {
         if (fresh[s]) {

@@ -406,6 +406,43 @@ __device__ __forceinline__ void wave_reserve(uint32_t* ctr, const bool (&flag)[R
     }
 }
 
+// wave_reserve for a whole workgroup: one atomic add per workgroup
+// (per-frame counters hit by every wavefront are a contention point).  All
+// threads of the workgroup call it; `lds` holds blockDim.x / 64 + 1 words.
+// Places are in (wavefront, row, lane) order.
+template <int R>
+__device__ __forceinline__ void block_reserve(uint32_t* ctr, const bool (&flag)[R], uint32_t (&out)[R],
+                                              uint32_t* lds) {
+    const int NW = static_cast<int>(blockDim.x) / kWave;
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    uint64_t m[R];
+    uint32_t total = 0;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        m[s] = __ballot(flag[s]);
+        total += static_cast<uint32_t>(__popcll(m[s]));
+    }
+    if (lane_id() == 0) lds[w] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int q = 0; q < NW; ++q) {
+            const uint32_t t = lds[q];
+            lds[q] = run;
+            run += t;
+        }
+        lds[NW] = run ? atomicAdd(ctr, run) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = lds[NW] + lds[w];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        out[s] = base + lanes_below(m[s]);
+        base += static_cast<uint32_t>(__popcll(m[s]));
+    }
+    __syncthreads();
+}
+
 // Stream compaction inside one tile of NT * R elements held row by row
 // (element k * NT + threadIdx.x of the tile is the thread's element of row
 // k, so every row is read and written with coalesced accesses).  bits has

@@ -118,6 +118,7 @@ constexpr int kPrepTile = 256 * kPrepRows;
 // round trips, and two points per thread halve the waits per point.
 __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     __shared__ float4 tile_pts[kPrepTile + 2 * kNearWindow];
+    __shared__ uint32_t rsv[256 / kWave + 1];
     int tile, f;
     xcd_tile_frame(tile, f);
     const FrameDesc& fd = d.frames[f];
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
         key[s] = in_frame ? noise_key(cxs[s], cys[s], cz) : -2;
     }
     uint32_t q[kPrepRows];
-    wave_reserve<kPrepRows>(&d.fs[f * kFsCount + kFsNoiseOpen], open, q);
+    block_reserve<kPrepRows>(&d.fs[f * kFsCount + kFsNoiseOpen], open, q, rsv);
 #pragma unroll
     for (int s = 0; s < kPrepRows; ++s)
         if (open[s]) d.nopen[po + q[s]] = i[s];
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(256) void k_noise_prep(DevBuf d, Layout L) {
     // Every occupied cell gets a dense id (in any order): the cell counts are
     // scanned over the ids, not over the whole table.
     uint32_t id[kPrepRows];
-    wave_reserve<kPrepRows>(&d.fs[f * kFsCount + kFsNoiseCells], fresh, id);
+    block_reserve<kPrepRows>(&d.fs[f * kFsCount + kFsNoiseCells], fresh, id, rsv);
 #pragma unroll
     for (int s = 0; s < kPrepRows; ++s) {
         if (fresh[s]) {
