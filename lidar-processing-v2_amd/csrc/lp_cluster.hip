@@ -617,11 +617,10 @@ __global__ void k_component_label(DevBuf d, Layout L) {
 // two never pass this test, so each run keeps two copies (a cluster of one
 // repeated point keeps the two copies the reference returns for it).
 __device__ __forceinline__ bool column_repeat(const DevBuf& d, const Layout& L, const FrameDesc& fd, size_t ob,
-                                              int i, int32_t lab) {
+                                              int i, float x, float y, int32_t lab) {
     const int pix = d.pt_pix[fd.pt_off + i];
     const int W = fd.sc.W;
     if (pix < 2 * W) return false;
-    const float x = d.x[fd.pt_off + i], y = d.y[fd.pt_off + i];
     const unsigned long long imask = (1ull << L.idx_bits) - 1;
 #pragma unroll
     for (int dr = 1; dr <= 2; ++dr) {
@@ -649,10 +648,11 @@ __global__ void k_cluster_scatter(DevBuf d, Layout L, int from_split) {
     const uint32_t off = d.cl_off[ob + lab];
     const uint32_t pos = wave_atomic_inc(&d.cl_cursor[ob + lab]);
     d.cl_members[ob + off + pos] = k;
-    if (from_split && column_repeat(d, L, d.frames[f], ob, d.obs_src[ob + k], lab)) return;
+    const float x = d.ox[ob + k], y = d.oy[ob + k];
+    if (from_split && column_repeat(d, L, d.frames[f], ob, d.obs_src[ob + k], x, y, lab)) return;
     const uint32_t hp = off + wave_atomic_inc(&d.hc_count[ob + lab]);
-    d.hcx[ob + hp] = static_cast<double>(d.ox[ob + k]);
-    d.hcy[ob + hp] = static_cast<double>(d.oy[ob + k]);
+    d.hcx[ob + hp] = static_cast<double>(x);
+    d.hcy[ob + hp] = static_cast<double>(y);
     d.hci[ob + hp] = k;
 }
 
