@@ -147,17 +147,6 @@ __global__ __launch_bounds__(kQueueThreads) void k_jcp_fill(DevBuf d, Layout L) 
     if (threadIdx.x == 0 && base + kQueueTile >= P) d.fs[f * kFsCount + kFsQueue] = pre_total + total;
 }
 
-__device__ __forceinline__ int lower_bound_i32(const int32_t* a, int n, int v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < v)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
 
 struct SlotVal {
     float u;
@@ -203,6 +192,7 @@ __global__ void k_jcp_slots(DevBuf d, Layout L) {
     const size_t po = fd.pix_off;
     const size_t qo = (size_t)f * L.P;
     const int32_t* ql = d.qlist + qo;
+    const int32_t* qrow = d.qrow + (size_t)f * (L.H + 1);
     const int p = ql[j];
     const int r = p / W, c = p % W;
     const float4 core = d.pxyz[po + p];
@@ -224,8 +214,9 @@ __global__ void k_jcp_slots(DevBuf d, Layout L) {
             const int rlo = max(0, -dr), rhi = min(H, H - dr);
             const int clo = max(0, -dc), chi = min(W, W - dc);
             int khi = j - 1;
-            if (rhi < H) khi = min(khi, lower_bound_i32(ql, nq, rhi * W) - 1);
-            const int klo = rlo > 0 ? lower_bound_i32(ql, nq, rlo * W) : 0;
+            // First queue index at or after a row's first pixel: qrow.
+            if (rhi < H) khi = min(khi, qrow[rhi] - 1);
+            const int klo = rlo > 0 ? qrow[rlo] : 0;
             int k = khi;
             for (; k >= klo; --k) {
                 const int qc = ql[k] % W;
