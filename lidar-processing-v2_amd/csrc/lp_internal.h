@@ -149,7 +149,7 @@ struct DevBuf {
     unsigned long long* pix_key;
     uint8_t* state;
     int32_t* mapping;
-    float4* pxyz;
+    float4* pxyz;            // per pixel: the winner's x, y, z and its index (int bits, -1 when empty)
     uint8_t* qflag;          // 1: the pixel joins the JCP queue
     int32_t* qmap;
     // JCP queue (frame stride pix_stride)

@@ -159,8 +159,8 @@ struct SlotVal {
 __device__ __forceinline__ SlotVal eval_slot(const DevBuf& d, size_t po, size_t qo, const SegConst& s, int pe,
                                              float4 core, int np) {
     SlotVal v{0.0f, kSrcUnknown, false};
-    if (d.mapping[po + np] == -1) return v;
-    const float4 nb = d.pxyz[po + np];
+    const float4 nb = d.pxyz[po + np];  // w holds the pixel's winner, -1 when empty
+    if (__float_as_int(nb.w) == -1) return v;
     const float dx = core.x - nb.x;
     const float dy = core.y - nb.y;
     const float dz = core.z - nb.z;

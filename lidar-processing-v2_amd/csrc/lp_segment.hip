@@ -800,6 +800,7 @@ __global__ void k_pixel_label(DevBuf d, Layout L) {
     if (key == ~0ull) {
         d.state[po] = kEmpty;
         d.mapping[po] = -1;
+        d.pxyz[po] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));  // w: the winner, -1 for none
         return;
     }
     const int i = static_cast<int>(key & ((1ull << L.idx_bits) - 1));
@@ -816,7 +817,7 @@ __global__ void k_pixel_label(DevBuf d, Layout L) {
     }
     d.state[po] = lab;
     d.mapping[po] = i;
-    d.pxyz[po] = make_float4(x, y, z, 0.0f);
+    d.pxyz[po] = make_float4(x, y, z, __int_as_float(i));
 }
 
 // ---------------------------------------------------------------------------
