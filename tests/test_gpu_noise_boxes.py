@@ -148,6 +148,54 @@ def test_fused_filter_segment_cluster_hull_boxes(ctx, oracle):
         ctx.set_oriented_boxes(False)
 
 
+def _all_noise_frame(like, n=60):
+    """Points on a widening spiral, each metres from the next: all DROR noise."""
+    ang = np.arange(n) * (2 * np.pi / n)
+    rad = 10.0 + 1.5 * np.arange(n)
+    x = (rad * np.cos(ang)).astype(np.float32)
+    y = (rad * np.sin(ang)).astype(np.float32)
+    z = np.full(n, -1.0, np.float32)
+    ring = (np.arange(n) % like.height).astype(np.uint16)
+    return synth.Frame(x, y, z, z.copy(), ring, like.height, like.width, like.fov_up_deg, like.fov_down_deg)
+
+
+def test_fused_batch_with_empty_and_all_noise_frames(ctx, oracle):
+    """Frames with no input and with no kept point share a batch with a full
+    frame: the keep, obstacle, queue and cluster compactions see zero items
+    for them, and the full frame's results do not change."""
+    full = synth.workload_c2(1)
+    e = np.zeros(0, np.float32)
+    empty = synth.Frame(e, e, e, e, np.zeros(0, np.uint16), full.height, full.width, full.fov_up_deg,
+                        full.fov_down_deg)
+    frames = [empty, full, _all_noise_frame(full)]
+    ctx.set_noise_filter(N.noise_config())
+    ctx.set_oriented_boxes(True)
+    try:
+        for use_graph in (False, True, True):
+            outs = ctx.process([(f.x, f.y, f.z, f.ring) for f in frames], [cfgs(oracle, f)[1] for f in frames],
+                               use_graph=use_graph)
+            for f, out in zip(frames, outs):
+                sco, _ = cfgs(oracle, f)
+                noise, (x, y), ref, hulls, boxes = full_frame_reference(oracle, f, sco)
+                assert out["status"] == 0
+                assert np.array_equal(out["noise_labels"], noise)
+                assert out["num_points"] == len(x) and out["num_noise"] == int(noise.sum())
+                assert np.array_equal(out["seg_labels"], ref["seg_labels"])
+                assert np.array_equal(out["cluster_labels"], ref["cluster_labels"])
+                assert out["num_clusters"] == ref["num_clusters"]
+                obs = out["obstacle_index"]
+                gh = hull_coords(x[obs].astype(np.float64), y[obs].astype(np.float64), out["cluster_labels"],
+                                 out["num_clusters"], out["hull_offsets"], out["hull_indices"])
+                assert len(gh) == len(hulls)
+                assert all(np.array_equal(a, b) for a, b in zip(gh, hulls))
+                assert all(box_equal(a, b) for a, b in zip(out["boxes"], boxes))
+            assert outs[0]["num_points"] == 0 and outs[2]["num_points"] == 0
+            assert outs[2]["num_noise"] == frames[2].n
+    finally:
+        ctx.set_noise_filter(None)
+        ctx.set_oriented_boxes(False)
+
+
 def test_filter_off_again_restores_plain_path(ctx, oracle):
     f = synth.workload_c1(0)
     sco, scg = cfgs(oracle, f)

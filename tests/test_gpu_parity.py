@@ -91,6 +91,47 @@ def test_full_frame_bit_exact(ctx, oracle, name):
     assert not bad, f"{len(bad)} hulls differ, first {bad[:5]}"
 
 
+def _posts():
+    """Vertical posts: every beam of a column returns the same float (x, y).
+    (azimuth deg, distance m, columns, first ring, last ring)."""
+    xs, ys, zs, rs = [], [], [], []
+    for az, d, ncol, r0, r1 in [(30.0, 12.0, 1, 20, 50), (75.0, 9.0, 2, 25, 45), (130.0, 15.0, 4, 22, 40),
+                                (200.0, 7.0, 3, 30, 31), (250.0, 11.0, 1, 24, 44), (300.0, 6.0, 1, 33, 34)]:
+        for c in range(ncol):
+            a = np.deg2rad(az + 0.4 * c)
+            x, y = np.float32(d * np.cos(a)), np.float32(d * np.sin(a))
+            for r in range(r0, r1 + 1):
+                xs.append(x)
+                ys.append(y)
+                zs.append(np.float32(-1.2 + 0.08 * (r - r0)))
+                rs.append(r)
+    return np.array(xs, np.float32), np.array(ys, np.float32), np.array(zs, np.float32), np.array(rs, np.uint16)
+
+
+def test_posts_of_repeated_points_bit_exact(ctx, oracle):
+    """Clusters whose points repeat one (x, y) down a range-image column: a
+    one-column post's hull is its point listed twice, as in the reference's
+    chain; posts of 2-4 columns give 2-4 vertices."""
+    f = frame("c1")
+    px, py, pz, pr = _posts()
+    x, y, z, ring = np.r_[f.x, px], np.r_[f.y, py], np.r_[f.z, pz], np.r_[f.ring, pr]
+    sco, scg = cfgs(oracle, f)
+    ref = oracle.process_frame(x, y, z, ring, scfg=sco)
+    out = ctx.process([(x, y, z, ring)], scg)[0]
+    assert ref["status"] == 0 and out["status"] == 0
+    assert np.array_equal(out["seg_labels"], ref["seg_labels"])
+    assert np.array_equal(out["cluster_labels"], ref["cluster_labels"])
+    obs = out["obstacle_index"]
+    ox, oy = x[obs].astype(np.float64), y[obs].astype(np.float64)
+    K = ref["num_clusters"]
+    hg = hull_coords(ox, oy, out["cluster_labels"], K, out["hull_offsets"], out["hull_indices"])
+    hr = hull_coords(ox, oy, ref["cluster_labels"], K, ref["hull_offsets"], ref["hull_indices"])
+    bad = [c for c in range(K) if not np.array_equal(hg[c], hr[c])]
+    assert not bad, f"{len(bad)} hulls differ, first {bad[:5]}"
+    # The posts reach the hull stage: one hull is a single point twice.
+    assert any(len(h) == 2 and np.array_equal(h[0], h[1]) for h in hr)
+
+
 def _known_answer_cases():
     return [
         # in-order updates (Gauss-Seidel), expected labels hand-derived
