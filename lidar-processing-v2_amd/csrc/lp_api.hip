@@ -233,6 +233,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.cc, 1);
     go(x.nc, 1);
     go(x.boxes, B * static_cast<size_t>(L.box_stride));
+    go(x.box_large, B * static_cast<size_t>(L.box_stride));
     go(x.vclass, B * static_cast<size_t>(L.box_stride));
     return static_cast<size_t>(p - base);
 }

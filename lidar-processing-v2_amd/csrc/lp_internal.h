@@ -193,6 +193,7 @@ struct DevBuf {
     CluConst* cc;            // [1]
     NoiseConst* nc;          // [1]
     lp_bounding_box* boxes;  // [B][box_stride]
+    int32_t* box_large;      // [B][box_stride] polygons of more than 64 vertices (box stage)
     int32_t* vclass;         // [B][box_stride] vehicle class per cluster
     // label clouds (allocated when switched on): ground flags and the
     // ground-then-unknown index lists (frame stride N)
@@ -242,6 +243,7 @@ enum FsSlot : int {
     kFsCcGlobal,       // 1: too many voxels for the LDS union-find
     kFsErr,
     kFsBoxes,          // clusters with a box
+    kFsBoxLarge,       // polygons listed for the workgroup box kernel
     kFsCellsUsed,      // non-empty polar cells
     kFsGround,         // GROUND points (label clouds on)
     kFsVehicles,       // clusters classified as vehicles

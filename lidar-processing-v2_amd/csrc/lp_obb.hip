@@ -10,15 +10,16 @@
 // minimum is the box's float `area` member, so near-ties are decided against
 // the float-rounded value exactly as the reference does.
 //
-// One workgroup per polygon.  One lane walks the antipodal pairs while
-// another sums the centroid in the reference's order.  The candidates
-// (i, -1) and (i - 1, +1) share one edge with opposite directions, which
-// negates every rotated coordinate exactly and leaves the area
-// bit-identical, so the workgroup evaluates one area per edge, O(n) each;
-// the first wavefront then applies the sequential selection rule over
-// the candidates in the reference's order (64 at a time, with a ballot for
-// those below the running minimum) and recomputes the chosen candidate's
-// extents in its own direction for the corners.
+// Polygons of up to 64 vertices (nearly all hulls) take one wavefront each,
+// larger ones a workgroup.  One lane walks the antipodal pairs and the
+// centroid is summed in the reference's order.  The candidates (i, -1) and
+// (i - 1, +1) share one edge with opposite directions, which negates every
+// rotated coordinate exactly and leaves the area bit-identical, so one area
+// is evaluated per edge, O(n) each, a lane per edge; one wavefront then
+// applies the sequential selection rule over the candidates in the
+// reference's order (64 at a time, through a prefix minimum of the areas) and
+// recomputes the chosen candidate's extents in its own direction for the
+// corners.
 #include <algorithm>
 
 #include "lp_device.h"
@@ -33,11 +34,8 @@ namespace {
 
 constexpr int kBoxThreads = 256;
 // Polygons of up to kBoxVerts vertices are held in LDS (larger: capacity
-// error).  The main launch takes those of up to kBoxSmall vertices with half
-// of the LDS, so more than twice as many workgroups fit a CU; a second
-// launch takes the larger ones.  Antipodal pairs: at most 3n/2, room for 2n.
+// error).  Antipodal pairs: at most 3n/2, room for 2n.
 constexpr int kBoxVerts = 1024;
-constexpr int kBoxSmall = 512;
 
 // areaOfTriangle (polygonizer.hpp:228-231), same operand order.
 __device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, double by, double cx, double cy) {
@@ -46,33 +44,56 @@ __device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, d
 
 __device__ __forceinline__ int wrap(int k, int n) { return k >= n ? k - n : k; }
 
-// Antipodal pairs of polygon (X, Y, n >= 3) in the reference's order
-// (polygonizer.cpp:93-161), walked by one lane: the areas of the running
-// pair are carried from step to step and the next vertex is loaded while the
-// current comparison runs.  Returns the count, or -1 when the walk exceeds
-// 4n + 8 steps or `cap` pairs (the reference's loop need not end on
+// Antipodal pairs of polygon (n >= 3 vertices) in the reference's order
+// (polygonizer.cpp:93-161), walked as one sequential chain: the areas of the
+// running pair are carried from step to step, and the vertices up to j + 3
+// and i + 3 are fetched ahead, so a step's fetch has a whole step to arrive
+// before a later step moves it into place.  vert(k, x, y) fetches vertex k;
+// emit(c, i, j) records pair c.  Returns the count, or -1 when the walk
+// exceeds 4n + 8 steps or `cap` pairs (the reference's loop need not end on
 // degenerate input).
-__device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pairs, int cap) {
+template <typename Vert, typename Emit>
+__device__ __forceinline__ int antipodal_walk(int n, int cap, Vert vert, Emit emit) {
     const int i0 = n - 1;
     const int limit = 4 * n + 8;
     int guard = 0, count = 0;
     int i = 0, j = 1;
-    double xi = X[0], yi = Y[0], xi1 = X[1], yi1 = Y[1];
-    double xj = X[1], yj = Y[1];
+    double xi, yi, xi1, yi1, xi2, yi2, xi3, yi3, xj, yj, xn, yn, xn2, yn2, xn3, yn3;
+    vert(0, xi, yi);
+    vert(1, xi1, yi1);
+    int i2 = wrap(2, n);  // vertices i + 2 and i + 3, fetched ahead
+    int i3 = wrap(i2 + 1, n);
+    vert(i2, xi2, yi2);
+    vert(i3, xi3, yi3);
+    xj = xi1;
+    yj = yi1;
     int jn = wrap(2, n);
-    double xn = X[jn], yn = Y[jn];
+    int jn2 = wrap(jn + 1, n);  // vertices j + 2 and j + 3, fetched ahead
+    int jn3 = wrap(jn2 + 1, n);
+    xn = xi2;
+    yn = yi2;
+    vert(jn2, xn2, yn2);
+    vert(jn3, xn3, yn3);
+    auto step_j = [&] {
+        j = jn;
+        xj = xn;
+        yj = yn;
+        jn = jn2;
+        xn = xn2;
+        yn = yn2;
+        jn2 = jn3;
+        xn2 = xn3;
+        yn2 = yn3;
+        jn3 = wrap(jn3 + 1, n);
+        vert(jn3, xn3, yn3);
+    };
     double a_cur = tri_area_xy(xi, yi, xi1, yi1, xj, yj);
     // while (area(i, i+1, j+1) > area(i, i+1, j)) ++j
     for (;;) {
         const double a_nx = tri_area_xy(xi, yi, xi1, yi1, xn, yn);
         if (!(a_nx > a_cur)) break;
-        j = jn;
-        xj = xn;
-        yj = yn;
+        step_j();
         a_cur = a_nx;
-        jn = wrap(j + 1, n);
-        xn = X[jn];
-        yn = Y[jn];
         if (++guard > limit) return -1;
     }
     const int j0 = j;
@@ -80,31 +101,34 @@ __device__ int antipodal_walk(const double* X, const double* Y, int n, int2* pai
         i = wrap(i + 1, n);
         xi = xi1;
         yi = yi1;
-        const int i1 = wrap(i + 1, n);
-        xi1 = X[i1];
-        yi1 = Y[i1];
+        const int i1 = i2;
+        xi1 = xi2;
+        yi1 = yi2;
+        i2 = i3;
+        xi2 = xi3;
+        yi2 = yi3;
+        i3 = wrap(i3 + 1, n);
+        vert(i3, xi3, yi3);
         if (count >= cap) return -1;
-        pairs[count++] = make_int2(i, j);
+        emit(count++, i, j);
         a_cur = tri_area_xy(xi, yi, xi1, yi1, xj, yj);
         for (;;) {
             const double a_nx = tri_area_xy(xi, yi, xi1, yi1, xn, yn);
             if (!(a_nx > a_cur)) break;
-            j = jn;
-            xj = xn;
-            yj = yn;
+            step_j();
             a_cur = a_nx;
-            jn = wrap(j + 1, n);
-            xn = X[jn];
-            yn = Y[jn];
             if (i == j0 && j == i0) return count;
             if (count >= cap) return -1;
-            pairs[count++] = make_int2(i, j);
+            emit(count++, i, j);
             if (++guard > limit) return -1;
         }
         // areaOfTriangle(j, i+1, j+1) == areaOfTriangle(i, i+1, j)
         if (tri_area_xy(xj, yj, xi1, yi1, xn, yn) == a_cur) {
             if (count >= cap) return -1;
-            pairs[count++] = (i == j0 && j == i0) ? make_int2(i1, j) : make_int2(i, jn);
+            if (i == j0 && j == i0)
+                emit(count++, i1, j);
+            else
+                emit(count++, i, jn);
         }
         if (++guard > limit) return -1;
     }
@@ -147,18 +171,215 @@ __device__ __forceinline__ bool cand_dir(const double* X, const double* Y, int n
     return true;
 }
 
-// Polygon k of frame f: vertices VX/VY[ob + off[k] .. off[k+1]) with the
-// offsets' end given by `total`.
-// This instantiation boxes the polygons of kMin < n <= kVerts vertices (and,
-// with kMin = 0, the degenerate ones); kVerts = kBoxVerts also flags larger
-// polygons as a capacity error.
-template <int kMin, int kVerts>
+// Area of the box in the frame of edge e (e -> e+1), or NaN when the edge is
+// shorter than 1e-6.  The candidates (e, +1) and (e + 1, -1) share it.
+// vert(q, x, y) fetches vertex q for the extents (the same for every lane).
+template <typename Vert>
+__device__ __forceinline__ double edge_box_area(const double* X, const double* Y, int n, int e, double cx, double cy,
+                                                Vert vert) {
+    double ux, uy;
+    if (!cand_dir(X, Y, n, e, 1, ux, uy)) return __builtin_nan("");
+    Extent ex = extent_init();
+    // Unrolled so the vertex fetches of four steps are in flight together;
+    // the order of the updates stays.
+#pragma unroll 4
+    for (int q = 0; q < n; ++q) {
+        double xq, yq;
+        vert(q, xq, yq);
+        const double tx = xq - cx;
+        const double ty = yq - cy;
+        extent_add(ex, tx * ux + ty * uy, -tx * uy + ty * ux);
+    }
+    return (ex.mxx - ex.mnx) * (ex.mxy - ex.mny);
+}
+
+// The reference's sequential choice among the 4 * np candidates of the
+// antipodal pairs, run by one whole wavefront (every lane returns the same
+// box).  Candidate c: pair c/4, vertex (c & 2 ? j : i), offset (c & 1 ? +1 :
+// -1); a candidate replaces the box when its area is below the float running
+// minimum (polygonizer.cpp:257).  `area` holds the area of every edge.
+__device__ void choose_box(const double* X, const double* Y, int n, const double* area, const int2* pairs, int np,
+                           double cx, double cy, lp_bounding_box& box) {
+    const int lane = lane_id();
+    // The running minimum after candidate c is float(min of the areas up to
+    // c): float rounding is monotone, so a candidate that does not replace
+    // the box lies between the smallest area so far and its float value and
+    // rounds to the same float.  So candidate c replaces the box exactly when
+    // its area is below float(min of the areas before it), which a prefix
+    // minimum over the lanes gives for 64 candidates at once; the box is the
+    // last replacing candidate.  NaN areas (short edges, no candidate) never
+    // replace and do not enter the minimum.
+    float best = __builtin_inff();  // float(DBL_MAX)
+    int sel = -1;
+    const int nc = 4 * np;
+    for (int base = 0; base < nc; base += kWave) {
+        const int c = base + lane;
+        double a = __builtin_nan("");
+        if (c < nc) {
+            const int2 pr = pairs[c >> 2];
+            const int idx = (c & 2) ? pr.y : pr.x;
+            a = area[(c & 1) ? idx : (idx == 0 ? n - 1 : idx - 1)];
+        }
+        double incl = (a == a) ? a : __builtin_inf();
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const double u = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl = fmin(incl, u);
+        }
+        double excl = __shfl_up(incl, 1, kWave);
+        if (lane == 0) excl = __builtin_inf();
+        const float before = fminf(best, static_cast<float>(excl));
+        const uint64_t rep = __ballot(a < static_cast<double>(before));
+        if (rep) sel = base + 63 - __clzll(static_cast<long long>(rep));
+        best = fminf(best, static_cast<float>(__shfl(incl, kWave - 1, kWave)));
+    }
+    if (sel < 0) return;
+    // The chosen candidate in its own direction.
+    const int2 pr = pairs[sel >> 2];
+    const int idx = (sel & 2) ? pr.y : pr.x;
+    double ux = 0.0, uy = 0.0;
+    cand_dir(X, Y, n, idx, (sel & 1) ? 1 : -1, ux, uy);
+    // Lanes take every 64th vertex; the reduction keeps, among equal values,
+    // the one of the lowest vertex index, which is what the sequential
+    // std::min / std::max keep (equal values differ at most in the sign of a
+    // zero).
+    Extent ex = extent_init();
+    int qa = n, qb = n, qc = n, qd = n;
+    for (int q = lane; q < n; q += kWave) {
+        const double tx = X[q] - cx;
+        const double ty = Y[q] - cy;
+        const double rx = tx * ux + ty * uy, ry = -tx * uy + ty * ux;
+        if (rx < ex.mnx) { ex.mnx = rx; qa = q; }
+        if (ex.mxx < rx) { ex.mxx = rx; qb = q; }
+        if (ry < ex.mny) { ex.mny = ry; qc = q; }
+        if (ex.mxy < ry) { ex.mxy = ry; qd = q; }
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double a0 = __shfl_xor(ex.mnx, o, kWave), a1 = __shfl_xor(ex.mxx, o, kWave);
+        const double a2 = __shfl_xor(ex.mny, o, kWave), a3 = __shfl_xor(ex.mxy, o, kWave);
+        const int b0 = __shfl_xor(qa, o, kWave), b1 = __shfl_xor(qb, o, kWave);
+        const int b2 = __shfl_xor(qc, o, kWave), b3 = __shfl_xor(qd, o, kWave);
+        if (a0 < ex.mnx || (a0 == ex.mnx && b0 < qa)) { ex.mnx = a0; qa = b0; }
+        if (ex.mxx < a1 || (a1 == ex.mxx && b1 < qb)) { ex.mxx = a1; qb = b1; }
+        if (a2 < ex.mny || (a2 == ex.mny && b2 < qc)) { ex.mny = a2; qc = b2; }
+        if (ex.mxy < a3 || (a3 == ex.mxy && b3 < qd)) { ex.mxy = a3; qd = b3; }
+    }
+    box.area = best;
+    box.corners[0] = ex.mnx * ux - ex.mny * uy + cx;
+    box.corners[1] = ex.mnx * uy + ex.mny * ux + cy;
+    box.corners[2] = ex.mxx * ux - ex.mny * uy + cx;
+    box.corners[3] = ex.mxx * uy + ex.mny * ux + cy;
+    box.corners[4] = ex.mxx * ux - ex.mxy * uy + cx;
+    box.corners[5] = ex.mxx * uy + ex.mxy * ux + cy;
+    box.corners[6] = ex.mnx * ux - ex.mxy * uy + cx;
+    box.corners[7] = ex.mnx * uy + ex.mxy * ux + cy;
+    box.angle_rad = atan2_approx(static_cast<float>(uy), static_cast<float>(ux));
+    box.is_valid = 1;
+}
+
+__device__ __forceinline__ lp_bounding_box empty_box() {
+    lp_bounding_box box;
+    for (int q = 0; q < 8; ++q) box.corners[q] = 0.0;
+    box.area = 0.0f;
+    box.angle_rad = 0.0f;
+    box.is_valid = 0;
+    box.reserved = 0;
+    return box;
+}
+
+// Polygons of up to one vertex per lane: one wavefront each, four per
+// workgroup (a grid-stride loop over the frame's polygons), the vertices in
+// the wavefront's own LDS slice.  Polygon k of frame f: vertices
+// VX/VY[ob + off[k] .. off[k+1]) with the offsets' end given by `total`.
+// Larger polygons are listed for k_boxes.
+constexpr int kBoxWaves = kBoxThreads / kWave;
+
+__global__ __launch_bounds__(kBoxThreads) void k_boxes_wave(DevBuf d, Layout L, const double* VX, const double* VY,
+                                                            const uint32_t* off) {
+    __shared__ double Xs[kBoxWaves][kWave];
+    __shared__ double Ys[kBoxWaves][kWave];
+    __shared__ double areas[kBoxWaves][kWave];
+    __shared__ int2 pairs_s[kBoxWaves][2 * kWave];
+    int tile, f;
+    xcd_tile_frame(tile, f);
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    double* X = Xs[w];
+    double* Y = Ys[w];
+    double* area = areas[w];
+    int2* pairs = pairs_s[w];
+    const uint32_t* fs = d.fs + f * kFsCount;
+    const int K = static_cast<int>(fs[kFsClusters]);
+    const uint32_t total = fs[kFsHullTotal];
+    const size_t ob = (size_t)f * L.N;
+    if (K > L.box_stride) {
+        if (tile == 0 && threadIdx.x == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
+        return;
+    }
+    for (int k = tile * kBoxWaves + w; k < K; k += gridDim.x * kBoxWaves) {
+        const uint32_t v0 = off[ob + k];
+        const uint32_t v1 = k + 1 < K ? off[ob + k + 1] : total;
+        const int n = static_cast<int>(v1 - v0);
+        if (n > kWave) {
+            if (lane == 0) {
+                const uint32_t q = atomicAdd(&d.fs[f * kFsCount + kFsBoxLarge], 1u);
+                d.box_large[(size_t)f * L.box_stride + q] = k;
+            }
+            continue;
+        }
+        lp_bounding_box box = empty_box();
+        if (n >= 3) {
+            if (lane < n) {
+                X[lane] = VX[ob + v0 + lane];
+                Y[lane] = VY[ob + v0 + lane];
+            }
+            wave_sync();
+            int np = 0;
+            if (lane == 0)
+                np = antipodal_walk(
+                    n, 2 * kWave, [&](int q, double& vx, double& vy) { vx = X[q]; vy = Y[q]; },
+                    [&](int c, int a, int b) { pairs[c] = make_int2(a, b); });
+            np = __shfl(np, 0, kWave);
+            wave_sync();
+            // Centroid: the vertices summed in order, as the reference does.
+            double cx = 0.0, cy = 0.0;
+            for (int q = 0; q < n; ++q) {
+                cx += X[q];
+                cy += Y[q];
+            }
+            cx /= n;
+            cy /= n;
+            if (np < 0) {
+                if (lane == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
+            } else {
+                if (lane < n)
+                    area[lane] = edge_box_area(X, Y, n, lane, cx, cy, [&](int q, double& vx, double& vy) {
+                        vx = X[q];
+                        vy = Y[q];
+                    });
+                wave_sync();
+                choose_box(X, Y, n, area, pairs, np, cx, cy, box);
+            }
+        }
+        if (lane == 0) {
+            d.boxes[(size_t)f * L.box_stride + k] = box;
+            if (box.is_valid) atomicAdd(&d.fs[f * kFsCount + kFsBoxes], 1u);
+        }
+        wave_sync();
+    }
+}
+
+// Polygons of more than one vertex per lane, one workgroup each (up to
+// kBoxVerts vertices; larger: capacity error): workgroup `tile` takes the
+// entries tile, tile + gridDim.x, ... of the frame's list written by
+// k_boxes_wave.
 __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const double* VX, const double* VY,
                                                       const uint32_t* off) {
-    __shared__ double X[kVerts];
-    __shared__ double Y[kVerts];
-    __shared__ double area[kVerts];  // per edge e (e -> e+1); NaN when too short
-    __shared__ int2 pairs[2 * kVerts];
+    __shared__ double X[kBoxVerts];
+    __shared__ double Y[kBoxVerts];
+    __shared__ double area[kBoxVerts];  // per edge e (e -> e+1); NaN when too short
+    __shared__ int2 pairs[2 * kBoxVerts];
     __shared__ int np_s;
     __shared__ double cen_s[2];
     int tile, f;
@@ -169,138 +390,55 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
     const uint32_t* fs = d.fs + f * kFsCount;
     const int K = static_cast<int>(fs[kFsClusters]);
     const uint32_t total = fs[kFsHullTotal];
+    const int m = static_cast<int>(fs[kFsBoxLarge]);
     const size_t ob = (size_t)f * L.N;
-    if (K > L.box_stride) {
-        if (tile == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
-        return;
-    }
-    for (int k = tile; k < K; k += gridDim.x) {
+    if (K > L.box_stride) return;  // flagged by k_boxes_wave
+    for (int j = tile; j < m; j += gridDim.x) {
+        const int k = d.box_large[(size_t)f * L.box_stride + j];
         const uint32_t v0 = off[ob + k];
         const uint32_t v1 = k + 1 < K ? off[ob + k + 1] : total;
         const int n = static_cast<int>(v1 - v0);
-        if ((kMin > 0 && n <= kMin) || (n > kVerts && kVerts < kBoxVerts)) continue;  // the other launch's
-        lp_bounding_box box;
-        for (int q = 0; q < 8; ++q) box.corners[q] = 0.0;
-        box.area = 0.0f;
-        box.angle_rad = 0.0f;
-        box.is_valid = 0;
-        box.reserved = 0;
-        if (n > kVerts) {
+        lp_bounding_box box = empty_box();
+        if (n > kBoxVerts) {
             if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
-        } else if (n >= 3) {
+        } else {
             for (int q = tid; q < n; q += kBoxThreads) {
                 X[q] = VX[ob + v0 + q];
                 Y[q] = VY[ob + v0 + q];
             }
             __syncthreads();
-            if (tid == 0) {
-                np_s = antipodal_walk(X, Y, n, pairs, 2 * kVerts);
-            } else if (w == 1 && lane == 0) {
+            // The first wavefront walks the antipodal pairs while the
+            // others sum the centroid (each on its own, in the
+            // reference's order) and evaluate the edge areas.
+            if (w == 0) {
+                if (lane == 0)
+                    np_s = antipodal_walk(
+                        n, 2 * kBoxVerts, [&](int q, double& x, double& y) { x = X[q]; y = Y[q]; },
+                        [&](int c, int a, int b) { pairs[c] = make_int2(a, b); });
+            } else {
                 double cx = 0.0, cy = 0.0;
                 for (int q = 0; q < n; ++q) {
                     cx += X[q];
                     cy += Y[q];
                 }
-                cen_s[0] = cx / n;
-                cen_s[1] = cy / n;
+                cx /= n;
+                cy /= n;
+                if (tid == kWave) {
+                    cen_s[0] = cx;
+                    cen_s[1] = cy;
+                }
+                for (int e = tid - kWave; e < n; e += kBoxThreads - kWave)
+                    area[e] = edge_box_area(X, Y, n, e, cx, cy, [&](int q, double& vx, double& vy) {
+                        vx = X[q];
+                        vy = Y[q];
+                    });
             }
             __syncthreads();
             const int np = np_s;
-            const double cx = cen_s[0], cy = cen_s[1];
             if (np < 0) {
                 if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrBoxCapacity);
-            } else {
-                // Area of every edge in its forward direction.
-                for (int e = tid; e < n; e += kBoxThreads) {
-                    double ux, uy;
-                    double a = __builtin_nan("");
-                    if (cand_dir(X, Y, n, e, 1, ux, uy)) {
-                        Extent ex = extent_init();
-                        // Unrolled so the vertex loads of four steps are in
-                        // flight together; the order of the updates stays.
-#pragma unroll 4
-                        for (int q = 0; q < n; ++q) {
-                            const double tx = X[q] - cx;
-                            const double ty = Y[q] - cy;
-                            extent_add(ex, tx * ux + ty * uy, -tx * uy + ty * ux);
-                        }
-                        a = (ex.mxx - ex.mnx) * (ex.mxy - ex.mny);
-                    }
-                    area[e] = a;
-                }
-                __syncthreads();
-                if (w == 0) {
-                    // Candidate c: pair c/4, vertex (c & 2 ? j : i), offset (c & 1 ? +1 : -1);
-                    // a candidate replaces the box when its area is below
-                    // the float running minimum (polygonizer.cpp:257).
-                    float best = __builtin_inff();  // float(DBL_MAX)
-                    int sel = -1;
-                    const int nc = 4 * np;
-                    for (int base = 0; base < nc; base += kWave) {
-                        const int c = base + lane;
-                        double a = __builtin_nan("");
-                        if (c < nc) {
-                            const int2 pr = pairs[c >> 2];
-                            const int idx = (c & 2) ? pr.y : pr.x;
-                            a = area[(c & 1) ? idx : (idx == 0 ? n - 1 : idx - 1)];
-                        }
-                        uint64_t beat = __ballot(a < static_cast<double>(best));
-                        while (beat) {
-                            const int q = __ffsll(static_cast<unsigned long long>(beat)) - 1;
-                            beat &= beat - 1;
-                            const double aq = __shfl(a, q, kWave);
-                            if (aq < static_cast<double>(best)) {
-                                best = static_cast<float>(aq);
-                                sel = base + q;
-                            }
-                        }
-                    }
-                    if (sel >= 0) {
-                        // The chosen candidate in its own direction.
-                        const int2 pr = pairs[sel >> 2];
-                        const int idx = (sel & 2) ? pr.y : pr.x;
-                        double ux = 0.0, uy = 0.0;
-                        cand_dir(X, Y, n, idx, (sel & 1) ? 1 : -1, ux, uy);
-                        // Lanes take every 64th vertex; the reduction keeps,
-                        // among equal values, the one of the lowest vertex
-                        // index, which is what the sequential std::min /
-                        // std::max keep (equal values differ at most in the
-                        // sign of a zero).
-                        Extent ex = extent_init();
-                        int qa = n, qb = n, qc = n, qd = n;
-                        for (int q = lane; q < n; q += kWave) {
-                            const double tx = X[q] - cx;
-                            const double ty = Y[q] - cy;
-                            const double rx = tx * ux + ty * uy, ry = -tx * uy + ty * ux;
-                            if (rx < ex.mnx) { ex.mnx = rx; qa = q; }
-                            if (ex.mxx < rx) { ex.mxx = rx; qb = q; }
-                            if (ry < ex.mny) { ex.mny = ry; qc = q; }
-                            if (ex.mxy < ry) { ex.mxy = ry; qd = q; }
-                        }
-#pragma unroll
-                        for (int o = kWave / 2; o > 0; o >>= 1) {
-                            const double a0 = __shfl_xor(ex.mnx, o, kWave), a1 = __shfl_xor(ex.mxx, o, kWave);
-                            const double a2 = __shfl_xor(ex.mny, o, kWave), a3 = __shfl_xor(ex.mxy, o, kWave);
-                            const int b0 = __shfl_xor(qa, o, kWave), b1 = __shfl_xor(qb, o, kWave);
-                            const int b2 = __shfl_xor(qc, o, kWave), b3 = __shfl_xor(qd, o, kWave);
-                            if (a0 < ex.mnx || (a0 == ex.mnx && b0 < qa)) { ex.mnx = a0; qa = b0; }
-                            if (ex.mxx < a1 || (a1 == ex.mxx && b1 < qb)) { ex.mxx = a1; qb = b1; }
-                            if (a2 < ex.mny || (a2 == ex.mny && b2 < qc)) { ex.mny = a2; qc = b2; }
-                            if (ex.mxy < a3 || (a3 == ex.mxy && b3 < qd)) { ex.mxy = a3; qd = b3; }
-                        }
-                        box.area = best;
-                        box.corners[0] = ex.mnx * ux - ex.mny * uy + cx;
-                        box.corners[1] = ex.mnx * uy + ex.mny * ux + cy;
-                        box.corners[2] = ex.mxx * ux - ex.mny * uy + cx;
-                        box.corners[3] = ex.mxx * uy + ex.mny * ux + cy;
-                        box.corners[4] = ex.mxx * ux - ex.mxy * uy + cx;
-                        box.corners[5] = ex.mxx * uy + ex.mxy * ux + cy;
-                        box.corners[6] = ex.mnx * ux - ex.mxy * uy + cx;
-                        box.corners[7] = ex.mnx * uy + ex.mxy * ux + cy;
-                        box.angle_rad = atan2_approx(static_cast<float>(uy), static_cast<float>(ux));
-                        box.is_valid = 1;
-                    }
-                }
+            } else if (w == 0) {
+                choose_box(X, Y, n, area, pairs, np, cen_s[0], cen_s[1], box);
             }
         }
         if (tid == 0) {
@@ -609,10 +747,15 @@ void launch_boxes(lp_context* c, int B) {
     hipStream_t st = c->stream;
     const size_t BN = (size_t)L.B * L.N;
     stage_mark(c, "boxes");
-    hipLaunchKernelGGL((k_boxes<0, kBoxSmall>), dim3(128, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN,
-                       d.hcy + BN, d.hull_off);
-    hipLaunchKernelGGL((k_boxes<kBoxSmall, kBoxVerts>), dim3(4, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN,
-                       d.hcy + BN, d.hull_off);
+    // Four polygons per workgroup; about 128 wavefronts per frame on large
+    // batches, more for few frames (standalone polygon lists).
+    const int tiles = std::min(1024, std::max(32, 4096 / B));
+    hipLaunchKernelGGL(k_boxes_wave, dim3(tiles, B), dim3(kBoxThreads), 0, st, d, L, d.hcx + BN, d.hcy + BN,
+                       d.hull_off);
+    // Enough workgroups that each takes at most one of a frame's few large
+    // polygons, whose walk is the longest chain of the stage.
+    hipLaunchKernelGGL(k_boxes, dim3(std::min(256, std::max(4, 1024 / B)), B), dim3(kBoxThreads), 0, st, d, L,
+                       d.hcx + BN, d.hcy + BN, d.hull_off);
 }
 
 }  // namespace lp
