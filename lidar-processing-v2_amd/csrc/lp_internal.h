@@ -136,7 +136,7 @@ struct DevBuf {
     float* zmin;
     float* elev;
     uint32_t* big_cells;     // per frame list of cells needing a workgroup sort
-    uint32_t* cell_list;     // per frame: the non-empty cells, in cell order
+    uint4* cell_list;        // per frame: the non-empty cells in cell order, {cell, start, count, 0}
     // RANSAC (per frame)
     uint32_t* cand_count;    // frame stride cand_stride
     uint32_t* cand_start;

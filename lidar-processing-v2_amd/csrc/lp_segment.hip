@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void k_project_q(DevBuf d, Layout L, int B) {
 
 // Exclusive scan of the cell histogram, one 1024-thread workgroup per frame.
 // Cell starts (exclusive scan of the counts), the list of non-empty cells
-// for k_cell_sort, and the empty marker of every empty cell's minimum.
+// with their starts and counts for k_cell_sort, and the empty marker of every
+// empty cell's minimum.
 __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
     __shared__ uint64_t lds[1024 / kWave + 1];
     const int f = blockIdx.x;
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
     const size_t co = (size_t)f * L.cell_stride;
     const uint32_t* cnt = d.cell_count + co;
     uint32_t* st = d.cell_start + (size_t)f * (L.cell_stride + 1);
-    uint32_t* list = d.cell_list + co;
+    uint4* list = d.cell_list + co;
     float* zmin = d.zmin + co;
     uint32_t carry = 0, used = 0;
     for (int base = 0; base < cells; base += 1024 * 8) {
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(1024) void k_cell_scan(DevBuf d, Layout L) {
             if (b + k < cells) {
                 st[b + k] = ex;
                 if (v[k])
-                    list[lx++] = static_cast<uint32_t>(b + k);
+                    list[lx++] = make_uint4(static_cast<uint32_t>(b + k), ex, v[k], 0u);
                 else
                     zmin[b + k] = u2f(kEmptyZ);
             }
@@ -302,11 +303,11 @@ __device__ __forceinline__ void sort_cell_row(int32_t* items, const uint32_t* zk
 // One wavefront per non-empty cell (a grid-stride loop over the cell list):
 // sorts the cell's point indices ascending (so the cell lists follow the
 // reference's input order) for the RANSAC cells, and sorts z for the robust
-// minimum.  A wavefront takes four consecutive cells of the list: those of
-// up to 16 points are sorted by one 16-lane row each (DPP bitonic), the
-// others one after the other by the whole wavefront.  Cells up to
-// kSmallCell (512) points sort in registers; larger ones are queued for
-// k_cell_sort_big.
+// minimum.  A wavefront takes four consecutive cells of the list (each entry
+// carries the cell's start and count): those of up to 16 points are sorted
+// by one 16-lane row each (DPP bitonic), the others one after the other by
+// the whole wavefront.  Cells up to kSmallCell (512) points sort in
+// registers; larger ones are queued for k_cell_sort_big.
 __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     static_assert(kSmallCell == 8 * kWave, "the register sorts cover cells up to kSmallCell points");
     int tile, f;
@@ -319,19 +320,21 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
     const int used = static_cast<int>(d.fs[f * kFsCount + kFsCellsUsed]);
     const int waves = gridDim.x * 4;
     float* zmin = d.zmin + co;
-    for (int q4 = (tile * 4 + w) * 4; q4 < used; q4 += waves * 4) {
-        // Row g of the wavefront takes list entry q4 + g.
-        const int g = lane >> 4;
-        const int qg = q4 + g;
-        int cell_g = -1;
-        uint32_t n_g = 0;
-        if (qg < used) {
-            cell_g = static_cast<int>(d.cell_list[co + qg]);
-            n_g = d.cell_count[co + cell_g];
-        }
+    // Row g of the wavefront takes list entry q4 + g; the entry of its next
+    // iteration is loaded one iteration ahead.
+    const int g = lane >> 4;
+    const int step = waves * 4;
+    int q4 = (tile * 4 + w) * 4;
+    const uint4 none = make_uint4(0xffffffffu, 0u, 0u, 0u);
+    uint4 next = q4 + g < used ? d.cell_list[co + q4 + g] : none;
+    for (; q4 < used; q4 += step) {
+        const uint4 e = next;
+        next = q4 + step + g < used ? d.cell_list[co + q4 + step + g] : none;
+        const int cell_g = static_cast<int>(e.x);  // -1: no entry
+        const uint32_t start_g = e.y;
+        const uint32_t n_g = e.z;
         if (cell_g >= 0 && n_g <= 16u) {
-            const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell_g];
-            sort_cell_row(d.cell_items + (size_t)f * L.N + start, d.cell_zkey + (size_t)f * L.N + start,
+            sort_cell_row(d.cell_items + (size_t)f * L.N + start_g, d.cell_zkey + (size_t)f * L.N + start_g,
                           static_cast<int>(n_g), cell_g % s.rings, zmin + cell_g);
         }
         // Rows whose cell is larger: the whole wavefront, one cell at a time.
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(DevBuf d, Layout L) {
                 }
                 continue;
             }
-            const uint32_t start = d.cell_start[(size_t)f * (L.cell_stride + 1) + cell];
+            const uint32_t start = static_cast<uint32_t>(__shfl(static_cast<int>(start_g), gg * 16, kWave));
             int32_t* items = d.cell_items + (size_t)f * L.N + start;
             const uint32_t* zk = d.cell_zkey + (size_t)f * L.N + start;
             const int radial = cell % s.rings;
