@@ -45,55 +45,37 @@ __device__ __forceinline__ double tri_area_xy(double ax, double ay, double bx, d
 __device__ __forceinline__ int wrap(int k, int n) { return k >= n ? k - n : k; }
 
 // Antipodal pairs of polygon (n >= 3 vertices) in the reference's order
-// (polygonizer.cpp:93-161), walked as one sequential chain: the areas of the
-// running pair are carried from step to step, and the vertices up to j + 3
-// and i + 3 are fetched ahead, so a step's fetch has a whole step to arrive
-// before a later step moves it into place.  vert(k, x, y) fetches vertex k;
-// emit(c, i, j) records pair c.  Returns the count, or -1 when the walk
-// exceeds 4n + 8 steps or `cap` pairs (the reference's loop need not end on
-// degenerate input).
+// (polygonizer.cpp:93-161), walked as one sequential chain with the areas of
+// the running pair carried from step to step.  vert(stream, k, x, y) fetches
+// vertex k; stream 0 serves i and stream 1 serves j, and each stream's
+// vertex indices advance by one (mod n) from call to call.  emit(c, i, j)
+// records pair c.  Returns the count, or -1 when the walk exceeds 4n + 8
+// steps or `cap` pairs (the reference's loop need not end on degenerate
+// input).
 template <typename Vert, typename Emit>
 __device__ __forceinline__ int antipodal_walk(int n, int cap, Vert vert, Emit emit) {
     const int i0 = n - 1;
     const int limit = 4 * n + 8;
     int guard = 0, count = 0;
     int i = 0, j = 1;
-    double xi, yi, xi1, yi1, xi2, yi2, xi3, yi3, xj, yj, xn, yn, xn2, yn2, xn3, yn3;
-    vert(0, xi, yi);
-    vert(1, xi1, yi1);
-    int i2 = wrap(2, n);  // vertices i + 2 and i + 3, fetched ahead
-    int i3 = wrap(i2 + 1, n);
-    vert(i2, xi2, yi2);
-    vert(i3, xi3, yi3);
+    double xi, yi, xi1, yi1, xj, yj, xn, yn;
+    vert(0, 0, xi, yi);
+    vert(0, 1, xi1, yi1);
     xj = xi1;
     yj = yi1;
     int jn = wrap(2, n);
-    int jn2 = wrap(jn + 1, n);  // vertices j + 2 and j + 3, fetched ahead
-    int jn3 = wrap(jn2 + 1, n);
-    xn = xi2;
-    yn = yi2;
-    vert(jn2, xn2, yn2);
-    vert(jn3, xn3, yn3);
-    auto step_j = [&] {
-        j = jn;
-        xj = xn;
-        yj = yn;
-        jn = jn2;
-        xn = xn2;
-        yn = yn2;
-        jn2 = jn3;
-        xn2 = xn3;
-        yn2 = yn3;
-        jn3 = wrap(jn3 + 1, n);
-        vert(jn3, xn3, yn3);
-    };
+    vert(1, jn, xn, yn);
     double a_cur = tri_area_xy(xi, yi, xi1, yi1, xj, yj);
     // while (area(i, i+1, j+1) > area(i, i+1, j)) ++j
     for (;;) {
         const double a_nx = tri_area_xy(xi, yi, xi1, yi1, xn, yn);
         if (!(a_nx > a_cur)) break;
-        step_j();
+        j = jn;
+        xj = xn;
+        yj = yn;
         a_cur = a_nx;
+        jn = wrap(j + 1, n);
+        vert(1, jn, xn, yn);
         if (++guard > limit) return -1;
     }
     const int j0 = j;
@@ -101,22 +83,20 @@ __device__ __forceinline__ int antipodal_walk(int n, int cap, Vert vert, Emit em
         i = wrap(i + 1, n);
         xi = xi1;
         yi = yi1;
-        const int i1 = i2;
-        xi1 = xi2;
-        yi1 = yi2;
-        i2 = i3;
-        xi2 = xi3;
-        yi2 = yi3;
-        i3 = wrap(i3 + 1, n);
-        vert(i3, xi3, yi3);
+        const int i1 = wrap(i + 1, n);
+        vert(0, i1, xi1, yi1);
         if (count >= cap) return -1;
         emit(count++, i, j);
         a_cur = tri_area_xy(xi, yi, xi1, yi1, xj, yj);
         for (;;) {
             const double a_nx = tri_area_xy(xi, yi, xi1, yi1, xn, yn);
             if (!(a_nx > a_cur)) break;
-            step_j();
+            j = jn;
+            xj = xn;
+            yj = yn;
             a_cur = a_nx;
+            jn = wrap(j + 1, n);
+            vert(1, jn, xn, yn);
             if (i == j0 && j == i0) return count;
             if (count >= cap) return -1;
             emit(count++, i, j);
@@ -180,9 +160,9 @@ __device__ __forceinline__ double edge_box_area(const double* X, const double* Y
     double ux, uy;
     if (!cand_dir(X, Y, n, e, 1, ux, uy)) return __builtin_nan("");
     Extent ex = extent_init();
-    // Unrolled so the vertex fetches of four steps are in flight together;
+    // Unrolled so the vertex fetches of two steps are in flight together;
     // the order of the updates stays.
-#pragma unroll 4
+#pragma unroll 2
     for (int q = 0; q < n; ++q) {
         double xq, yq;
         vert(q, xq, yq);
@@ -338,7 +318,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes_wave(DevBuf d, Layout L, 
             int np = 0;
             if (lane == 0)
                 np = antipodal_walk(
-                    n, 2 * kWave, [&](int q, double& vx, double& vy) { vx = X[q]; vy = Y[q]; },
+                    n, 2 * kWave, [&](int, int q, double& vx, double& vy) { vx = X[q]; vy = Y[q]; },
                     [&](int c, int a, int b) { pairs[c] = make_int2(a, b); });
             np = __shfl(np, 0, kWave);
             wave_sync();
@@ -413,7 +393,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_boxes(DevBuf d, Layout L, const
             if (w == 0) {
                 if (lane == 0)
                     np_s = antipodal_walk(
-                        n, 2 * kBoxVerts, [&](int q, double& x, double& y) { x = X[q]; y = Y[q]; },
+                        n, 2 * kBoxVerts, [&](int, int q, double& x, double& y) { x = X[q]; y = Y[q]; },
                         [&](int c, int a, int b) { pairs[c] = make_int2(a, b); });
             } else {
                 double cx = 0.0, cy = 0.0;
