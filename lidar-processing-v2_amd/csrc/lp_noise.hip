@@ -24,9 +24,8 @@
 // thread per open point, checks the point's own cell outward from its own
 // position (points next to each other in scan order are mostly neighbours in
 // space), so almost every open point of a dense surface reaches
-// min_neighbours after a few distance tests; a point whose cell has been
-// counted whole and whose ball reaches no occupied adjacent cell (an
-// isolated return) is decided there too.  The rest (rain, isolated returns, points
+// min_neighbours after a few distance tests; a point whose ball lies inside
+// its cell is decided there too.  The rest (rain, isolated returns, points
 // near a cell face) are listed and the second pass, one wavefront per listed
 // point, visits the cube of cells within ceil(r) + 1 nearest first: the slow
 // searches run side by side instead of holding up wavefronts of easy points.
@@ -263,7 +262,7 @@ constexpr int kOwnCellBudget = 64;  // distance tests of the own-cell pass
 
 // Own-cell test of point i at cell-sorted position pos (see k_noise_count).
 __device__ __forceinline__ void noise_own_cell(const DevBuf& d, const NoiseConst& nc, int f, size_t po, size_t ho,
-                                               uint32_t mask, int i, int pos) {
+                                               int i, int pos) {
     const float4 t = d.dsort[po + pos];
     const float r2 = noise_radius_sqr(nc, t.x, t.y);
     const uint32_t k = nc.k;
@@ -294,28 +293,17 @@ __device__ __forceinline__ void noise_own_cell(const DevBuf& d, const NoiseConst
             state = 2;
             carry = cnt | static_cast<uint32_t>(up - pos) << 8 | static_cast<uint32_t>(pos - 1 - dn) << 16;
             const float r = sqrtf(r2);
+            int cx, cy, cz;
+            noise_cell(t.x, t.y, t.z, cx, cy, cz);
             const bool whole = up >= s1 && dn < s0;
-            // The whole own cell is counted and the radius reaches no further
-            // than the 26 adjacent cells (the first shell of k_noise_search,
-            // with its cull): when none of those within reach holds a point,
-            // that pass would count nothing more, so the point is NOISE here.
-            if (whole && r < static_cast<float>(kNoiseMaxExtent - 1) &&
-                static_cast<int>(floorf(r + kNoiseCellSlack)) == 0) {
-                int cx, cy, cz;
-                noise_cell(t.x, t.y, t.z, cx, cy, cz);
+            const bool interior = cx > 0 && cx < 1023 && cy > 0 && cy < 1023 && cz > 0 && cz < 2047;
+            if (whole && interior && r < static_cast<float>(kNoiseMaxExtent - 1)) {
                 const float fx = t.x - floorf(t.x), fy = t.y - floorf(t.y), fz = t.z - floorf(t.z);
                 const float reach = (r + kNoiseCellSlack) * (r + kNoiseCellSlack);
-                const uint32_t* occ = d.nocc + (size_t)f * kNoiseOccWords;
-                bool near = false;
-                for (int ci = 0; ci < 27 && !near; ++ci) {
-                    const int dz = ci / 9 - 1, dy = (ci / 3) % 3 - 1, dx = ci % 3 - 1;
-                    const int ax = cx + dx, ay = cy + dy, az = cz + dz;
-                    if (ci != 13 && cell_gap_sqr(fx, fy, fz, dx, dy, dz) <= reach && ax >= 0 && ax <= 1023 &&
-                        ay >= 0 && ay <= 1023 && az >= 0 && az <= 2047 &&
-                        ((occ[ay * 32 + (ax >> 5)] >> (ax & 31)) & 1u))
-                        near = noise_find(d.hslot + ho, mask, noise_key(ax, ay, az)) >= 0;
-                }
-                if (!near) state = 1;
+                if (cell_gap_sqr(fx, fy, fz, -1, 0, 0) > reach && cell_gap_sqr(fx, fy, fz, 1, 0, 0) > reach &&
+                    cell_gap_sqr(fx, fy, fz, 0, -1, 0) > reach && cell_gap_sqr(fx, fy, fz, 0, 1, 0) > reach &&
+                    cell_gap_sqr(fx, fy, fz, 0, 0, -1) > reach && cell_gap_sqr(fx, fy, fz, 0, 0, 1) > reach)
+                    state = 1;
             }
         }
     }
@@ -332,7 +320,7 @@ __device__ __forceinline__ void noise_own_cell(const DevBuf& d, const NoiseConst
 // Own-cell pass over the open list (a grid-stride loop).  A point is VALID
 // once min_neighbours points of its cell are within its radius (the count
 // includes itself, and the search order does not change the outcome), NOISE
-// when its whole cell has been checked and no occupied cell comes within its
+// when its whole cell has been checked and no other cell comes within its
 // radius, and listed for the neighbour-cell pass otherwise.
 __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
     int tile, f;
@@ -344,7 +332,7 @@ __global__ __launch_bounds__(256) void k_noise_count(DevBuf d, Layout L) {
     for (int q = tile * blockDim.x + threadIdx.x; q < nopen; q += gridDim.x * blockDim.x) {
         const int i = d.nopen[po + q];
         const int pos = d.spos[po + i];
-        noise_own_cell(d, nc, f, po, ho, static_cast<uint32_t>(L.hash_cap) - 1, i, pos);
+        noise_own_cell(d, nc, f, po, ho, i, pos);
     }
 }
 
