@@ -663,23 +663,12 @@ __device__ int sort_dedupe_chunk(double* X, double* Y, int32_t* I, uint16_t* P, 
     return nd;
 }
 
-__device__ __forceinline__ int upper_bound_u32(const uint32_t* a, int n, uint32_t v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] <= v)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Chunk plan for round r, one 1024-thread workgroup per frame walking the
 // clusters in tiles with a running carry: a cluster that is not final yet
 // needs ceil(cnt / 256) chunks (one if it fits: that chunk's hull is final);
-// hc_choff receives the exclusive scan of the chunk counts and the frame's
-// total goes to kFsHullChunks.  Round 0 also clears the hull count and flag,
+// hc_choff receives the exclusive scan of the chunk counts, hc_chunk_c /
+// hc_chunk_i each chunk's cluster and index, and the frame's total goes to
+// kFsHullChunks.  Round 0 also clears the hull count and flag,
 // and every round clears the next round's candidate counts.
 constexpr int kPlanThreads = 1024;
 constexpr int kPlanItems = 4;
@@ -716,7 +705,17 @@ __global__ __launch_bounds__(kPlanThreads) void k_hull_plan(DevBuf d, Layout L, 
         uint32_t ex = block_exclusive_sum<kPlanThreads>(local, lds, &total) + carry;
 #pragma unroll
         for (int q = 0; q < kPlanItems; ++q) {
-            if (b + q < K) d.hc_choff[ob + b + q] = ex;
+            if (b + q < K) {
+                d.hc_choff[ob + b + q] = ex;
+                // The chunk map (cluster, index in the cluster) of each of
+                // the cluster's chunks, so a round's wavefront finds its
+                // chunk with one load.  A round has at most N chunks (a
+                // cluster's chunk count never exceeds its candidate count).
+                for (uint32_t k = 0; k < v[q] && ex + k < static_cast<uint32_t>(L.N); ++k) {
+                    d.hc_chunk_c[ob + ex + k] = b + q;
+                    d.hc_chunk_i[ob + ex + k] = k;
+                }
+            }
             ex += v[q];
         }
         carry += total;
@@ -739,13 +738,11 @@ __global__ __launch_bounds__(256, 3) void k_hull_round(DevBuf d, Layout L, int r
     __shared__ int nup[kWavesPerBlock];
     int tile, f;
     xcd_tile_frame(tile, f);
-    const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
-    const uint32_t total = d.fs[f * kFsCount + kFsHullChunks];
+    const uint32_t total = min(d.fs[f * kFsCount + kFsHullChunks], static_cast<uint32_t>(L.N));
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
     const size_t ob = (size_t)f * L.N;
     const size_t BN = (size_t)L.B * L.N;
-    const uint32_t* choff = d.hc_choff + ob;
     double* X = sx[w];
     double* Y = sy[w];
     int32_t* I = si[w];
@@ -754,8 +751,8 @@ __global__ __launch_bounds__(256, 3) void k_hull_round(DevBuf d, Layout L, int r
     const size_t in_b = (size_t)(r & 1) * BN;
     const size_t out_b = (size_t)((r + 1) & 1) * BN;
     for (uint32_t t = tile * kWavesPerBlock + w; t < total; t += gridDim.x * kWavesPerBlock) {
-        const int c = upper_bound_u32(choff, K, t) - 1;
-        const uint32_t ci = t - choff[c];
+        const int c = d.hc_chunk_c[ob + t];
+        const uint32_t ci = d.hc_chunk_i[ob + t];
         const uint32_t cnt = d.hc_count[in_b + ob + c];
         const uint32_t base = d.cl_off[ob + c];
         const bool final_chunk = cnt <= static_cast<uint32_t>(kHullChunk);

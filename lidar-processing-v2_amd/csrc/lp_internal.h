@@ -181,6 +181,8 @@ struct DevBuf {
     int32_t* hci;            // candidate obstacle index
     uint32_t* hc_count;      // per cluster candidates (two buffers)
     uint32_t* hc_choff;      // their exclusive scan
+    int32_t* hc_chunk_c;     // per chunk of a hull round: its cluster (= cell_items, free in the hull stage)
+    uint32_t* hc_chunk_i;    // and its index in the cluster (= cell_zkey)
     int32_t* hull_idx;       // final hull (obstacle indices), frame stride N
     uint32_t* hull_count;    // per cluster
     uint32_t* hull_off;      // per cluster (+1)
