@@ -179,6 +179,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.cell_items, B * N);
     go(x.cell_zkey, B * N);
     x.hc_chunk_c = x.cell_items;
+    go(x.hull_large, B * N);
     x.hc_chunk_i = x.cell_zkey;
     go(x.zmin, B * C);
     go(x.elev, B * C);

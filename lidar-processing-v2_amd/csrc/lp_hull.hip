@@ -1269,7 +1269,7 @@ __global__ __launch_bounds__(256) void k_hull_ref(DevBuf d, Layout L, const doub
 // vertices and kLocalSmallSize members (most of them) take one wavefront
 // each, with 2.6 KiB of LDS, so a CU holds as many clusters as it holds
 // wavefronts; the others take a 1024-thread workgroup with room for
-// kLocalMax vertices.  Each launch skips the other's clusters.
+// kLocalMax vertices, taken from a per-frame list the first launch writes.
 constexpr int kLocalMax = 512;
 constexpr int kLocalSmallV = 64;
 constexpr int kLocalSmallSize = 1024;
@@ -1302,13 +1302,20 @@ __global__ __launch_bounds__(kLocalThreads) void k_hull_local(DevBuf d, Layout L
     const int lane = lane_id();
     const size_t ob = (size_t)f * L.N;
     const size_t BN = (size_t)L.B * L.N;
-    for (int c = tile; c < K; c += gridDim.x) {
+    // The wavefront launch walks all clusters and lists the large ones; the
+    // workgroup launch takes list entries tile, tile + gridDim.x, ...
+    const int iters = kSmall ? K : static_cast<int>(d.fs[f * kFsCount + kFsHullLarge]);
+    for (int j = tile; j < iters; j += gridDim.x) {
+        const int c = kSmall ? j : d.hull_large[ob + j];
         const int V = static_cast<int>(d.hull_count[ob + c]);
         const uint32_t hoff = d.hull_off[ob + c];
         const uint32_t base = d.cl_off[ob + c];
         const int size = static_cast<int>(d.cl_size[ob + c]);
         const int32_t* members = d.cl_members + ob + base;
-        if ((V <= kLocalSmallV && size <= kLocalSmallSize) != kSmall) continue;  // the other launch's
+        if (kSmall && !(V <= kLocalSmallV && size <= kLocalSmallSize)) {
+            if (tid == 0) d.hull_large[ob + atomicAdd(&d.fs[f * kFsCount + kFsHullLarge], 1u)] = c;
+            continue;
+        }
         // A cluster left without a hull ran out of the chunk rounds' and the
         // final workgroup's capacity.
         if (V == 0 && tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
@@ -1468,6 +1475,7 @@ __global__ void k_hull_lists_init(DevBuf d, Layout L, const int32_t* offsets, in
         d.fs[kFsErr] = 0;
         d.fs[kFsHullInexact] = 0;
         d.fs[kFsHullRounded] = 0;
+        d.fs[kFsHullLarge] = 0;
     }
     if (t < K) {
         d.cl_off[t] = static_cast<uint32_t>(offsets[t]);
@@ -1525,10 +1533,9 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     const int small_grid = std::min(1024, std::max(8, 8192 / B));
     hipLaunchKernelGGL((k_hull_local<kWave, kLocalSmallV, true>), dim3(small_grid, B), dim3(kWave), 0, st, d, L,
                        list_xy);
-    // The large clusters are a few dozen per frame at most: about 4096
-    // workgroups in all, 32 to 256 per frame (empty workgroups still cost
-    // dispatch).
-    const int large_grid = std::min(256, std::max(32, 4096 / B));
+    // The large clusters, listed by the launch above, are a few dozen per
+    // frame at most: 16 to 256 workgroups per frame, so each takes about one.
+    const int large_grid = std::min(256, std::max(16, 2048 / B));
     hipLaunchKernelGGL((k_hull_local<1024, kLocalMax, false>), dim3(large_grid, B), dim3(1024), 0, st, d, L,
                        list_xy);
 }

@@ -182,6 +182,7 @@ struct DevBuf {
     uint32_t* hc_count;      // per cluster candidates (two buffers)
     uint32_t* hc_choff;      // their exclusive scan
     int32_t* hc_chunk_c;     // per chunk of a hull round: its cluster (= cell_items, free in the hull stage)
+    int32_t* hull_large;     // [B][N] clusters left to the workgroup local-rank launch
     uint32_t* hc_chunk_i;    // and its index in the cluster (= cell_zkey)
     int32_t* hull_idx;       // final hull (obstacle indices), frame stride N
     uint32_t* hull_count;    // per cluster
@@ -246,6 +247,7 @@ enum FsSlot : int {
     kFsErr,
     kFsBoxes,          // clusters with a box
     kFsBoxLarge,       // polygons listed for the workgroup box kernel
+    kFsHullLarge,      // clusters listed for the workgroup local-rank launch
     kFsCellsUsed,      // non-empty polar cells
     kFsGround,         // GROUND points (label clouds on)
     kFsVehicles,       // clusters classified as vehicles
