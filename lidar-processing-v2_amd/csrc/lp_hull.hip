@@ -979,133 +979,148 @@ __global__ __launch_bounds__(1024) void k_hull_big(DevBuf d, Layout L, int r_in)
     __shared__ uint16_t UP[kHullBig];
     __shared__ int nlo, nup, nlist[2];
     __shared__ bool nlist_done;
+    __shared__ int open_list[1024];
+    __shared__ int open_n;
     const int f = blockIdx.y;
     const int K = static_cast<int>(d.fs[f * kFsCount + kFsClusters]);
     const int tid = threadIdx.x;
     const size_t ob = (size_t)f * L.N;
     const size_t BN = (size_t)L.B * L.N;
     const size_t in_b = (size_t)(r_in & 1) * BN;
-    for (int c = blockIdx.x; c < K; c += gridDim.x) {
-        if (d.hull_count[ob + c] != 0) continue;
-        const uint32_t cnt = d.hc_count[in_b + ob + c];
-        if (cnt > static_cast<uint32_t>(kHullBig)) {
-            if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
-            continue;
-        }
-        const int n = static_cast<int>(cnt);
-        const uint32_t base = d.cl_off[ob + c];
-        int m = 1;
-        while (m < n) m <<= 1;
-        for (int i = tid; i < m; i += 1024) {
-            if (i < n) {
-                const size_t src = in_b + ob + base + i;
-                X[i] = d.hcx[src];
-                Y[i] = d.hcy[src];
-                I[i] = d.hci[src];
-            }
-            P[i] = static_cast<uint16_t>(i);
-        }
+    // Workgroup x takes the clusters c = x (mod gridDim.x): its threads test
+    // 1024 of them at once and list those without a hull yet.
+    const int T = static_cast<int>(gridDim.x);
+    for (int cb = 0; cb < K; cb += 1024 * T) {
+        if (tid == 0) open_n = 0;
         __syncthreads();
-        for (int k = 2; k <= m; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int q = tid; q < (m >> 1); q += 1024) {
-                    const int a = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                    const int b = a + j;
-                    const int pa = P[a], pb = P[b];
-                    const bool gt = (pa >= n) ? (pb < n || pa > pb) : (pb < n && pt_less(X, Y, I, pb, pa));
-                    if (gt == ((a & k) == 0)) {
-                        P[a] = static_cast<uint16_t>(pb);
-                        P[b] = static_cast<uint16_t>(pa);
+        const int ct = cb + tid * T + static_cast<int>(blockIdx.x);
+        if (ct < K && d.hull_count[ob + ct] == 0) open_list[atomicAdd(&open_n, 1)] = ct;
+        __syncthreads();
+        const int nopen = open_n;
+        for (int jo = 0; jo < nopen; ++jo) {
+            const int c = open_list[jo];
+            const uint32_t cnt = d.hc_count[in_b + ob + c];
+            if (cnt > static_cast<uint32_t>(kHullBig)) {
+                if (tid == 0) atomicOr(&d.fs[f * kFsCount + kFsErr], kErrHullCapacity);
+                continue;
+            }
+            const int n = static_cast<int>(cnt);
+            const uint32_t base = d.cl_off[ob + c];
+            int m = 1;
+            while (m < n) m <<= 1;
+            for (int i = tid; i < m; i += 1024) {
+                if (i < n) {
+                    const size_t src = in_b + ob + base + i;
+                    X[i] = d.hcx[src];
+                    Y[i] = d.hcy[src];
+                    I[i] = d.hci[src];
+                }
+                P[i] = static_cast<uint16_t>(i);
+            }
+            __syncthreads();
+            for (int k = 2; k <= m; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int q = tid; q < (m >> 1); q += 1024) {
+                        const int a = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                        const int b = a + j;
+                        const int pa = P[a], pb = P[b];
+                        const bool gt = (pa >= n) ? (pb < n || pa > pb) : (pb < n && pt_less(X, Y, I, pb, pa));
+                        if (gt == ((a & k) == 0)) {
+                            P[a] = static_cast<uint16_t>(pb);
+                            P[b] = static_cast<uint16_t>(pa);
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            if (n < 3) {
+                if (tid == 0) {
+                    uint32_t inexact = 0;
+                    nlo = chain_hull(X, Y, I, P, n, S, &inexact);
+                    nup = 2;
+                }
+            } else {
+                if (tid < kWave) {
+                    // Exact duplicates (copies from different chunks) are
+                    // adjacent now: keep the first of each, or two copies of a
+                    // single repeated point (as in the chunk rounds).
+                    const int lane = tid;
+                    int nd = 0;
+                    for (int i0 = 0; i0 < n; i0 += kWave) {
+                        const int i = i0 + lane;
+                        bool keep = false;
+                        int p = 0;
+                        if (i < n) {
+                            p = P[i];
+                            const int q = P[i > 0 ? i - 1 : 0];
+                            keep = i == 0 || X[p] != X[q] || Y[p] != Y[q];
+                        }
+                        const uint64_t mk = __ballot(keep);
+                        if (keep) LO[nd + lanes_below(mk)] = static_cast<uint16_t>(p);
+                        nd += static_cast<int>(__popcll(mk));
+                    }
+                    if (nd == 1) {
+                        if (lane == 0) LO[1] = P[1];
+                        nd = 2;
+                    }
+                    wave_sync();
+                    for (int i = lane; i < nd; i += kWave) P[i] = LO[i];
+                    wave_sync();
+                    int cl, cu;
+                    split_by_chord(X, Y, P, nd, LO, UP, &cl, &cu);
+                    wave_sync();
+                    bool dl, du;
+                    uint32_t inexact = 0;
+                    cl = prune_chain(X, Y, LO, S, cl, 1, kBigPrunePasses, &dl, &inexact);
+                    cu = prune_chain(X, Y, UP, S + kHullBig, cu, -1, kBigPrunePasses, &du, &inexact);
+                    if (dl && du) {
+                        for (int i = tid; i < cl; i += kWave) S[i] = LO[i];
+                        for (int i = tid; i < cu; i += kWave) S[kHullBig + i] = UP[cu - 1 - i];
+                    }
+                    inexact = wave_sum(inexact);
+                    if (tid == 0) {
+                        nlist[0] = cl;
+                        nlist[1] = cu;
+                        nlist_done = dl && du;
+                        if (dl && du) {
+                            nlo = cl;
+                            nup = cu;
+                        }
+                        if (inexact) {
+                            atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                            atomicOr(&d.hull_flag[ob + c], 1u);
+                        }
                     }
                 }
                 __syncthreads();
-            }
-        }
-        if (n < 3) {
-            if (tid == 0) {
-                uint32_t inexact = 0;
-                nlo = chain_hull(X, Y, I, P, n, S, &inexact);
-                nup = 2;
-            }
-        } else {
-            if (tid < kWave) {
-                // Exact duplicates (copies from different chunks) are
-                // adjacent now: keep the first of each, or two copies of a
-                // single repeated point (as in the chunk rounds).
-                const int lane = tid;
-                int nd = 0;
-                for (int i0 = 0; i0 < n; i0 += kWave) {
-                    const int i = i0 + lane;
-                    bool keep = false;
-                    int p = 0;
-                    if (i < n) {
-                        p = P[i];
-                        const int q = P[i > 0 ? i - 1 : 0];
-                        keep = i == 0 || X[p] != X[q] || Y[p] != Y[q];
-                    }
-                    const uint64_t mk = __ballot(keep);
-                    if (keep) LO[nd + lanes_below(mk)] = static_cast<uint16_t>(p);
-                    nd += static_cast<int>(__popcll(mk));
-                }
-                if (nd == 1) {
-                    if (lane == 0) LO[1] = P[1];
-                    nd = 2;
-                }
-                wave_sync();
-                for (int i = lane; i < nd; i += kWave) P[i] = LO[i];
-                wave_sync();
-                int cl, cu;
-                split_by_chord(X, Y, P, nd, LO, UP, &cl, &cu);
-                wave_sync();
-                bool dl, du;
-                uint32_t inexact = 0;
-                cl = prune_chain(X, Y, LO, S, cl, 1, kBigPrunePasses, &dl, &inexact);
-                cu = prune_chain(X, Y, UP, S + kHullBig, cu, -1, kBigPrunePasses, &du, &inexact);
-                if (dl && du) {
-                    for (int i = tid; i < cl; i += kWave) S[i] = LO[i];
-                    for (int i = tid; i < cu; i += kWave) S[kHullBig + i] = UP[cu - 1 - i];
-                }
-                inexact = wave_sum(inexact);
-                if (tid == 0) {
-                    nlist[0] = cl;
-                    nlist[1] = cu;
-                    nlist_done = dl && du;
-                    if (dl && du) {
-                        nlo = cl;
-                        nup = cu;
-                    }
-                    if (inexact) {
-                        atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
-                        atomicOr(&d.hull_flag[ob + c], 1u);
+                if (!nlist_done && tid < 2 * kWave) {
+                    // Wavefront 0 the lower chain, wavefront 1 the upper one, each
+                    // by the divide-and-conquer chain.
+                    const int wv = tid / kWave;
+                    uint32_t inexact = 0;
+                    const int len = merge_half_chain(X, Y, wv == 0 ? LO : UP, nlist[wv], wv == 1,
+                                                     wv == 1 ? S + kHullBig : S, &inexact);
+                    inexact = wave_sum(inexact);
+                    if (lane_id() == 0) {
+                        if (wv == 0)
+                            nlo = len;
+                        else
+                            nup = len;
+                        if (inexact) {
+                            atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
+                            atomicOr(&d.hull_flag[ob + c], 1u);
+                        }
                     }
                 }
             }
             __syncthreads();
-            if (!nlist_done && tid < 2 * kWave) {
-                // Wavefront 0 the lower chain, wavefront 1 the upper one, each
-                // by the divide-and-conquer chain.
-                const int wv = tid / kWave;
-                uint32_t inexact = 0;
-                const int len = merge_half_chain(X, Y, wv == 0 ? LO : UP, nlist[wv], wv == 1,
-                                                 wv == 1 ? S + kHullBig : S, &inexact);
-                inexact = wave_sum(inexact);
-                if (lane_id() == 0) {
-                    if (wv == 0)
-                        nlo = len;
-                    else
-                        nup = len;
-                    if (inexact) {
-                        atomicAdd(&d.fs[f * kFsCount + kFsHullInexact], inexact);
-                        atomicOr(&d.hull_flag[ob + c], 1u);
-                    }
-                }
-            }
+            const int kl = nlo;
+            const int kk = kl + nup - 2;
+            for (int i = tid; i < kk; i += 1024)
+                d.hull_idx[ob + base + i] = I[i < kl ? S[i] : S[kHullBig + i - kl + 1]];
+            if (tid == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
+            __syncthreads();
         }
-        __syncthreads();
-        const int kl = nlo;
-        const int kk = kl + nup - 2;
-        for (int i = tid; i < kk; i += 1024) d.hull_idx[ob + base + i] = I[i < kl ? S[i] : S[kHullBig + i - kl + 1]];
-        if (tid == 0) d.hull_count[ob + c] = static_cast<uint32_t>(kk);
         __syncthreads();
     }
 }
