@@ -180,6 +180,7 @@ size_t layout_bytes(const Layout& L, DevBuf* d, char* base) {
     go(x.cell_zkey, B * N);
     x.hc_chunk_c = x.cell_items;
     go(x.hull_large, B * N);
+    x.hc_chunk_b = reinterpret_cast<uint32_t*>(x.hull_large);
     x.hc_chunk_i = x.cell_zkey;
     go(x.zmin, B * C);
     go(x.elev, B * C);

@@ -667,8 +667,8 @@ __device__ int sort_dedupe_chunk(double* X, double* Y, int32_t* I, uint16_t* P, 
 // clusters in tiles with a running carry: a cluster that is not final yet
 // needs ceil(cnt / 256) chunks (one if it fits: that chunk's hull is final);
 // hc_choff receives the exclusive scan of the chunk counts, hc_chunk_c /
-// hc_chunk_i each chunk's cluster and index, and the frame's total goes to
-// kFsHullChunks.  Round 0 also clears the hull count and flag,
+// hc_chunk_i / hc_chunk_b each chunk's cluster, index, size and the cluster's
+// first candidate, and the frame's total goes to kFsHullChunks.  Round 0 also clears the hull count and flag,
 // and every round clears the next round's candidate counts.
 constexpr int kPlanThreads = 1024;
 constexpr int kPlanItems = 4;
@@ -684,10 +684,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_hull_plan(DevBuf d, Layout L, 
         uint32_t v[kPlanItems];
         uint32_t local = 0;
         const int b = base + static_cast<int>(threadIdx.x) * kPlanItems;
+        uint32_t cn[kPlanItems];
 #pragma unroll
         for (int q = 0; q < kPlanItems; ++q) {
             const int c = b + q;
             v[q] = 0;
+            cn[q] = 0;
             if (c < K) {
                 if (r == 0) {
                     d.hull_count[ob + c] = 0;
@@ -696,6 +698,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_hull_plan(DevBuf d, Layout L, 
                 if (r == 0 || d.hull_count[ob + c] == 0) {
                     const uint32_t cnt = d.hc_count[(r & 1) * BN + ob + c];
                     v[q] = cnt <= static_cast<uint32_t>(kHullChunk) ? 1u : (cnt + kHullChunk - 1) / kHullChunk;
+                    cn[q] = cnt;
                 }
                 d.hc_count[((r + 1) & 1) * BN + ob + c] = 0;
                 local += v[q];
@@ -707,13 +710,19 @@ __global__ __launch_bounds__(kPlanThreads) void k_hull_plan(DevBuf d, Layout L, 
         for (int q = 0; q < kPlanItems; ++q) {
             if (b + q < K) {
                 d.hc_choff[ob + b + q] = ex;
-                // The chunk map (cluster, index in the cluster) of each of
-                // the cluster's chunks, so a round's wavefront finds its
-                // chunk with one load.  A round has at most N chunks (a
-                // cluster's chunk count never exceeds its candidate count).
-                for (uint32_t k = 0; k < v[q] && ex + k < static_cast<uint32_t>(L.N); ++k) {
-                    d.hc_chunk_c[ob + ex + k] = b + q;
-                    d.hc_chunk_i[ob + ex + k] = k;
+                // The chunk map of each of the cluster's chunks, so a round's
+                // wavefront has its chunk's cluster, place and size with one
+                // round of loads.  A round has at most N chunks (a cluster's
+                // chunk count never exceeds its candidate count).
+                if (v[q]) {
+                    const uint32_t base = d.cl_off[ob + b + q];
+                    const uint32_t fin = cn[q] <= static_cast<uint32_t>(kHullChunk) ? 0x80000000u : 0u;
+                    for (uint32_t k = 0; k < v[q] && ex + k < static_cast<uint32_t>(L.N); ++k) {
+                        const uint32_t nk = min(cn[q] - k * kHullChunk, static_cast<uint32_t>(kHullChunk));
+                        d.hc_chunk_c[ob + ex + k] = static_cast<int32_t>(static_cast<uint32_t>(b + q) | fin);
+                        d.hc_chunk_i[ob + ex + k] = k | nk << 16;
+                        d.hc_chunk_b[ob + ex + k] = base;
+                    }
                 }
             }
             ex += v[q];
@@ -751,13 +760,13 @@ __global__ __launch_bounds__(256, 3) void k_hull_round(DevBuf d, Layout L, int r
     const size_t in_b = (size_t)(r & 1) * BN;
     const size_t out_b = (size_t)((r + 1) & 1) * BN;
     for (uint32_t t = tile * kWavesPerBlock + w; t < total; t += gridDim.x * kWavesPerBlock) {
-        const int c = d.hc_chunk_c[ob + t];
-        const uint32_t ci = d.hc_chunk_i[ob + t];
-        const uint32_t cnt = d.hc_count[in_b + ob + c];
-        const uint32_t base = d.cl_off[ob + c];
-        const bool final_chunk = cnt <= static_cast<uint32_t>(kHullChunk);
-        const uint32_t lo = ci * kHullChunk;
-        const int n = static_cast<int>(min(cnt - lo, static_cast<uint32_t>(kHullChunk)));
+        const uint32_t cw = static_cast<uint32_t>(d.hc_chunk_c[ob + t]);
+        const uint32_t iw = d.hc_chunk_i[ob + t];
+        const uint32_t base = d.hc_chunk_b[ob + t];
+        const int c = static_cast<int>(cw & 0x7fffffffu);
+        const bool final_chunk = (cw >> 31) != 0;
+        const uint32_t lo = (iw & 0xffffu) * kHullChunk;
+        const int n = static_cast<int>(iw >> 16);
         for (int i = lane; i < n; i += kWave) {
             const size_t src = in_b + ob + base + lo + i;
             X[i] = d.hcx[src];

@@ -181,9 +181,15 @@ struct DevBuf {
     int32_t* hci;            // candidate obstacle index
     uint32_t* hc_count;      // per cluster candidates (two buffers)
     uint32_t* hc_choff;      // their exclusive scan
-    int32_t* hc_chunk_c;     // per chunk of a hull round: its cluster (= cell_items, free in the hull stage)
+    // Per chunk of a hull round (written by the round's plan): its cluster,
+    // with bit 31 set when the chunk is the cluster's only one (= cell_items,
+    // free in the hull stage); its index in the cluster and, from bit 16, its
+    // point count (= cell_zkey); the cluster's first candidate (= hull_large,
+    // free until the local ranks).
+    int32_t* hc_chunk_c;
+    uint32_t* hc_chunk_i;
+    uint32_t* hc_chunk_b;
     int32_t* hull_large;     // [B][N] clusters left to the workgroup local-rank launch
-    uint32_t* hc_chunk_i;    // and its index in the cluster (= cell_zkey)
     int32_t* hull_idx;       // final hull (obstacle indices), frame stride N
     uint32_t* hull_count;    // per cluster
     uint32_t* hull_off;      // per cluster (+1)
