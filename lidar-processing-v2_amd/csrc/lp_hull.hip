@@ -1552,9 +1552,9 @@ void launch_hull(lp_context* c, int B, const double2* list_xy) {
     stage_mark(c, "hull");
     launch_scan_small(c, B, d.hull_count, d.hull_off, L.N, kCountSlot, kFsClusters, kFsHullTotal);
     stage_mark(c, "hull_local");
-    // About 8192 wavefronts for the small clusters (every CU full), at
-    // least 8 per frame; the large ones as before.
-    const int small_grid = std::min(1024, std::max(8, 8192 / B));
+    // About 16384 wavefronts for the small clusters (about one cluster each
+    // on 128 frames), at least 8 per frame.
+    const int small_grid = std::min(1024, std::max(8, 16384 / B));
     hipLaunchKernelGGL((k_hull_local<kWave, kLocalSmallV, true>), dim3(small_grid, B), dim3(kWave), 0, st, d, L,
                        list_xy);
     // The large clusters, listed by the launch above, are a few dozen per
